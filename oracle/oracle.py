@@ -1,0 +1,336 @@
+"""ORACLE -- test infrastructure only.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py
+import this module, and only as the checker or as the timed CPU baseline; the
+product library never loads it.
+
+A CPU restatement of the reference's host FMM for Laplace3D on cube
+kernel-independent surfaces:
+
+* the ghost-cell tree, its interaction lists and ``locate``: oracle/tree_oracle.c
+  (src/tree/builder.jl, src/tree/lists.jl, src/tree/locate.jl);
+* the FP64 direct sums and the per-point S2M / L2T / near-field loops:
+  oracle/fmm_oracle.c (src/kernels.jl:442-487, src/operators/ki_common.jl:256-315,
+  src/evaluate.jl:12-58);
+* the upward and downward passes (``upward_pass!``/``downward_pass!``,
+  src/fmm.jl:170-262) with numpy/BLAS for the GEMM stages, M2L either per far
+  offset through the symmetry-class representatives (``m2l_level!`` default,
+  src/fmm.jl:100-127, the form the reference device plan uses) or spectral
+  (``_spectral_m2l_level!``, src/operators/spectral_m2l.jl:328-451, the form the
+  reference CPU driver uses for the cube method);
+* ``evaluate_at_sources!`` / ``evaluate!`` (src/evaluate.jl:395-441).
+
+The operators (surfaces, pseudo-inverse factors, folded M2M/L2L, class
+representatives, spectral tables) are supplied by the caller from the host
+precompute ``ffmm.operators``; they are pinned independently by
+tests/test_operators.py (field error on all 316 offsets, M2M/L2L against the exact
+fields, class reproduction, spectral against dense).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """The built oracle library (``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.otree_build.restype = ctypes.c_int
+        L.otree_build.argtypes = [f64p, ctypes.c_int64, ctypes.c_int, f64p, ctypes.c_double, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_void_p)]
+        L.otree_last_error.restype = ctypes.c_char_p
+        L.otree_nlevels.argtypes = [ctypes.c_void_p]
+        L.otree_nnodes.argtypes = [ctypes.c_void_p]
+        L.otree_level_npairs.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.otree_level_npairs.restype = ctypes.c_int64
+        L.otree_get.argtypes = [ctypes.c_void_p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, u8p, u8p, u8p, u8p,
+                                i32p, i32p, i32p, f64p]
+        L.otree_get_lists.argtypes = [ctypes.c_void_p, ctypes.c_int, i32p, i32p, i32p]
+        L.otree_locate.argtypes = [ctypes.c_void_p, f64p, ctypes.c_int64, i32p]
+        L.otree_free.argtypes = [ctypes.c_void_p]
+        L.odirect.argtypes = [f64p, f64p, ctypes.c_int64, f64p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, f64p]
+        L.os2m.argtypes = [i32p, ctypes.c_int64, f64p, i32p, f64p, f64p, ctypes.c_int, i32p, i32p, f64p, f64p, f64p]
+        L.oeval.argtypes = [f64p, i32p, i32p, i64p, ctypes.c_int64, f64p, i64p, f64p, f64p, ctypes.c_int,
+                            i32p, f64p, i32p, i32p, i32p, f64p, f64p, ctypes.c_int, f64p]
+        _LIB = L
+    return _LIB
+
+
+class OracleError(ValueError):
+    pass
+
+
+@dataclass
+class Tree:
+    """The flat tree (0-based; -1 = none), the fields of the reference's ``Tree``
+    (src/tree/tree.jl:178-209) with ``points`` as ``plo``/``phi``."""
+    origin: np.ndarray
+    W0: float
+    level_start: np.ndarray
+    cell: np.ndarray
+    level: np.ndarray
+    parent: np.ndarray
+    child_first: np.ndarray
+    plo: np.ndarray
+    phi: np.ndarray
+    leaf: np.ndarray
+    ghost: np.ndarray
+    xlist: np.ndarray
+    has_source: np.ndarray
+    colleagues: np.ndarray     # (n, 27)
+    perm: np.ndarray
+    point_node: np.ndarray
+    points: np.ndarray         # sorted points (N, 3)
+    lists: list                # per level: (ptr, src, far), targets in node order
+
+    @property
+    def nlevels(self) -> int:
+        return len(self.level_start) - 1
+
+    @property
+    def nnodes(self) -> int:
+        return len(self.level)
+
+    def level_range(self, l: int) -> range:
+        return range(int(self.level_start[l]), int(self.level_start[l + 1]))
+
+    def group_parent(self, l: int) -> np.ndarray:
+        if l == 0:
+            return np.zeros(0, dtype=np.int32)
+        r = self.level_range(l)
+        return self.parent[r.start:r.stop:8].copy()
+
+    def width(self, l: int) -> float:
+        return float(np.ldexp(self.W0, -l))
+
+    def centers(self) -> np.ndarray:
+        w = np.ldexp(self.W0, -self.level.astype(np.int64))
+        return self.origin[None, :] + (self.cell.astype(np.float64) + 0.5) * w[:, None]
+
+    def locate(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        # the tree is rebuilt on demand from its arrays: keep a handle alive instead
+        out = np.empty(len(x), dtype=np.int32)
+        lib().otree_locate(self._handle, x, len(x), out)
+        if np.any(out < 0):
+            raise OracleError("point lies outside the tree's box")
+        return out
+
+
+def build_tree(points: np.ndarray, ppl: int, origin, W0: float, max_levels: int = 30) -> Tree:
+    """``Tree(points; ppl, bbox=(origin, W0), max_levels)`` (src/tree/builder.jl:482-495)."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    h = ctypes.c_void_p()
+    L = lib()
+    rc = L.otree_build(pts, len(pts), int(ppl), o, float(W0), int(max_levels), ctypes.byref(h))
+    if rc != 0:
+        raise OracleError(L.otree_last_error().decode())
+    nl, n, N = L.otree_nlevels(h), L.otree_nnodes(h), len(pts)
+    a = dict(level_start=np.empty(nl + 1, np.int32), cell=np.empty((n, 3), np.int32), level=np.empty(n, np.int32),
+             parent=np.empty(n, np.int32), child_first=np.empty(n, np.int32), plo=np.empty(n, np.int32),
+             phi=np.empty(n, np.int32), leaf=np.empty(n, np.uint8), ghost=np.empty(n, np.uint8),
+             xlist=np.empty(n, np.uint8), has_source=np.empty(n, np.uint8), colleagues=np.empty((n, 27), np.int32),
+             perm=np.empty(N, np.int32), point_node=np.empty(N, np.int32), points=np.empty((N, 3), np.float64))
+    L.otree_get(h, a["level_start"], a["cell"], a["level"], a["parent"], a["child_first"], a["plo"], a["phi"],
+                a["leaf"], a["ghost"], a["xlist"], a["has_source"], a["colleagues"], a["perm"], a["point_node"],
+                a["points"])
+    lists = []
+    for l in range(nl):
+        nlv = int(a["level_start"][l + 1] - a["level_start"][l])
+        npr = L.otree_level_npairs(h, l)
+        ptr, src, far = np.empty(nlv + 1, np.int32), np.empty(npr, np.int32), np.empty(npr, np.int32)
+        L.otree_get_lists(h, l, ptr, src, far)
+        lists.append((ptr, src, far))
+    t = Tree(origin=o.copy(), W0=float(W0), lists=lists, **a)
+    t._handle = h
+    return t
+
+
+def direct(sources, tau, targets, grad: bool = False) -> np.ndarray:
+    """FP64 direct sum at separate targets (``direct``, src/kernels.jl:463-472)."""
+    s = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    out = np.empty((len(t), 3) if grad else len(t))
+    lib().odirect(s, tau, len(s), t, len(t), None, int(grad), out)
+    return out
+
+
+def direct_self(sources, tau, grad: bool = False, rows=None) -> np.ndarray:
+    """FP64 direct sum at the sources, self excluded (``direct_self``,
+    src/kernels.jl:479-487); ``rows`` restricts it to some of the sources."""
+    s = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    rows = np.arange(len(s), dtype=np.int64) if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
+    t = np.ascontiguousarray(s[rows])
+    out = np.empty((len(rows), 3) if grad else len(rows))
+    lib().odirect(s, tau, len(s), t, len(t), rows.ctypes.data_as(ctypes.c_void_p), int(grad), out)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the passes
+# ---------------------------------------------------------------------------
+class FMM:
+    """``FMM(method, sources; ppl, bbox)`` with the operators ``ops`` (reference width
+    ``ops.width``) shared by every level through the homogeneous rescale
+    (src/fmm.jl:31-49, src/operators/cache.jl:224-237)."""
+
+    def __init__(self, ops, points, ppl: int, origin, W0: float, max_levels: int = 30):
+        self.ops = ops
+        self.tree = build_tree(points, ppl, origin, W0, max_levels)
+        t = self.tree
+        L = t.nlevels
+        self.lam = np.array([t.width(l) / ops.width for l in range(L)])
+        self.centers = np.ascontiguousarray(t.centers())
+        childless = t.child_first < 0
+        self.s2m_nodes = [np.ascontiguousarray([i for i in t.level_range(l) if childless[i] and t.has_source[i]],
+                                               dtype=np.int32) for l in range(L)]
+        self.leaf_nodes = [np.ascontiguousarray([i for i in t.level_range(l) if t.leaf[i]], dtype=np.int32)
+                           for l in range(L)]
+        self.surf = np.ascontiguousarray(ops.large)
+
+    # upward_pass! (src/fmm.jl:170-205)
+    def upward(self, tau_sorted: np.ndarray) -> np.ndarray:
+        t, ops = self.tree, self.ops
+        L = t.nlevels
+        Ns, Nl = ops.nsmall, ops.nlarge
+        Mp = np.zeros((Ns, t.nnodes))
+        for l in range(L - 1, 1, -1):
+            nodes = self.s2m_nodes[l]
+            if len(nodes):
+                pM = np.empty((len(nodes), Nl))          # column q of the Fortran-ordered pM
+                lib().os2m(nodes, len(nodes), self.centers, t.level, self.surf, self.lam, Nl, t.plo, t.phi,
+                           t.points, tau_sorted, pM)
+                Mp[:, nodes] = ops.P_up.apply(pM.T, self.lam[l])
+            if l < L - 1:
+                r = t.level_range(l + 1)
+                # the children block, one stacked column per sibling group
+                B = Mp[:, r.start:r.stop].T.reshape(-1, 8 * Ns).T
+                Mp[:, t.group_parent(l + 1)] = ops.M2M @ B
+        return Mp
+
+    # downward_pass! (src/fmm.jl:207-262)
+    def downward(self, Mp: np.ndarray, m2l: str = "dense"):
+        t, ops = self.tree, self.ops
+        L = t.nlevels
+        Ns, Nl = ops.nsmall, ops.nlarge
+        Pl = np.zeros((Ns, t.nnodes))
+        Lc = np.zeros((Nl, t.nnodes))
+        for l in range(2, L):
+            r = t.level_range(l)
+            nl = len(r)
+            if l > 2:
+                G = Pl[:, t.group_parent(l)]
+                pL = (ops.L2L @ G).T.reshape(nl, Ns).T.copy()
+            else:
+                pL = np.zeros((Ns, nl))
+            alpha = 1.0 / self.lam[l]
+            if m2l == "dense":
+                self._m2l_dense(pL, Mp, l, alpha)
+            else:
+                self._m2l_spectral(pL, Mp, l, alpha)
+            Pl[:, r.start:r.stop] = pL
+            leaves = self.leaf_nodes[l]
+            if len(leaves):
+                Lc[:, leaves] = ops.P_down.apply(Pl[:, leaves], self.lam[l])
+        return Pl, Lc
+
+    def _pairs_by_offset(self, l: int):
+        ptr, src, far = self.tree.lists[l]
+        lo = self.tree.level_start[l]
+        tgt = lo + np.repeat(np.arange(len(ptr) - 1, dtype=np.int64), np.diff(ptr))
+        return tgt, src.astype(np.int64), far.astype(np.int64)
+
+    def _m2l_dense(self, pL, Mp, l, alpha):
+        """Per far offset: gather, the class representative through the signed
+        permutation, scatter-add (a target appears at most once per offset)."""
+        ops = self.ops
+        sc = ops.sc
+        lo = int(self.tree.level_start[l])
+        tgt, src, far = self._pairs_by_offset(l)
+        order = np.argsort(far, kind="stable")
+        tgt, src, far = tgt[order], src[order], far[order]
+        bounds = np.searchsorted(far, np.arange(317))
+        for f in range(316):
+            a, b = bounds[f], bounds[f + 1]
+            if a == b:
+                continue
+            idx = sc.idx[f]
+            Y = ops.class_ops[sc.klass[f]] @ Mp[np.ix_(idx, src[a:b])]
+            pL[np.ix_(idx, tgt[a:b] - lo)] += alpha * Y
+
+    def _m2l_spectral(self, pL, Mp, l, alpha):
+        """The level-wise spectral translation: transform every box, Hadamard-accumulate
+        over the pairs, transform back and extract."""
+        ops = self.ops
+        n = 2 * ops.p
+        r = self.tree.level_range(l)
+        lo = r.start
+        tgt, src, far = self._pairs_by_offset(l)
+        if len(tgt) == 0:
+            return
+        nodes = np.unique(np.concatenate([src, tgt]))
+        slot = np.full(self.tree.nnodes, -1, np.int64)
+        slot[nodes] = np.arange(len(nodes))
+        lat = np.zeros((len(nodes), n ** 3))
+        lat[:, ops.embed] = Mp[:, nodes].T
+        S = np.fft.rfftn(lat.reshape(-1, n, n, n), axes=(1, 2, 3))
+        U = np.zeros_like(S)
+        np.add.at(U, slot[tgt], ops.tables[far] * S[slot[src]])
+        u = np.fft.irfftn(U, s=(n, n, n), axes=(1, 2, 3)).reshape(len(nodes), -1)[:, ops.embed]
+        ts = np.unique(tgt)
+        pL[:, ts - lo] += alpha * u[slot[ts]].T
+
+    # evaluate_at_sources! (src/evaluate.jl:425-441)
+    def evaluate_at_sources(self, Lc, tau_sorted, grad=False):
+        t = self.tree
+        N = len(t.perm)
+        out = np.empty((N, 3) if grad else N)
+        lib().oeval(t.points, t.point_node, np.arange(N, dtype=np.int32), t.perm.astype(np.int64), N,
+                    np.ascontiguousarray(Lc.T), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
+                    self.ops.nlarge, t.level, self.centers, t.colleagues, t.plo, t.phi, t.points, tau_sorted,
+                    int(grad), out)
+        return out
+
+    # evaluate! (src/evaluate.jl:395-410)
+    def evaluate(self, Lc, tau_sorted, targets, grad=False):
+        t = self.tree
+        x = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        nodes = t.locate(x)
+        nt = len(x)
+        out = np.empty((nt, 3) if grad else nt)
+        lib().oeval(x, nodes, np.full(nt, -1, np.int32), np.arange(nt, dtype=np.int64), nt,
+                    np.ascontiguousarray(Lc.T), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
+                    self.ops.nlarge, t.level, self.centers, t.colleagues, t.plo, t.phi, t.points, tau_sorted,
+                    int(grad), out)
+        return out
+
+    def matvec(self, tau, grad=False, m2l="dense", targets=None):
+        """``build_expansions`` then ``evaluate_at_sources`` (and ``evaluate`` at
+        ``targets`` when given); outputs in the original order."""
+        tau_sorted = np.ascontiguousarray(np.asarray(tau, dtype=np.float64)[self.tree.perm])
+        Mp = self.upward(tau_sorted)
+        Pl, Lc = self.downward(Mp, m2l)
+        self.multipoles, self.plocals, self.locals = Mp, Pl, Lc
+        u = self.evaluate_at_sources(Lc, tau_sorted, grad)
+        if targets is None:
+            return u
+        return u, self.evaluate(Lc, tau_sorted, targets, grad)
