@@ -1,0 +1,243 @@
+// Evaluation at the sources and at arbitrary targets: the local expansion of the
+// target's leaf (L2T over the large surface) plus the near field over the leaf's 27
+// colleagues, the target's own source excluded by index (a masked select, never a
+// multiply).
+//
+// One wavefront per real leaf, one lane per target: the surface, the leaf's local
+// density and the colleagues' sources are read at wave-uniform addresses, so every
+// lane of the wave consumes the same stream (scalar loads through the constant cache)
+// and the 64 lanes only differ in their target.  Targets at arbitrary points are
+// located by a device walk with the builder's child-slot rule and bucketed by leaf
+// with a stable sort, so a leaf's targets are contiguous.
+//
+// Follows _evaluate_kernel! (src/device.jl:102-125), evaluate_target / _direct_range
+// (src/evaluate.jl:12-58) and locate (src/tree/locate.jl:10-22).
+#include "ffmm_internal.h"
+
+#include <algorithm>
+
+namespace ffmm {
+
+constexpr int TB = 256;
+constexpr int EV_TB = 64;
+constexpr double C4PI = 0.07957747154594767;   // 1 / (4 pi)
+
+template <int OUT>
+__device__ __forceinline__ void accumulate(double d0, double d1, double d2, double w, double acc[4]) {
+  double ir = 1.0 / sqrt(fma(d2, d2, fma(d1, d1, d0 * d0)));
+  if (OUT != FFMM_GRADIENT) acc[0] += (C4PI * ir) * w;
+  if (OUT != FFMM_POTENTIAL) {
+    double a = -C4PI * ir * ir * ir;
+    acc[1] += (a * d0) * w;
+    acc[2] += (a * d1) * w;
+    acc[3] += (a * d2) * w;
+  }
+}
+template <int OUT>
+__device__ __forceinline__ void accumulate_masked(double d0, double d1, double d2, double w, bool keep, double acc[4]) {
+  double ir = 1.0 / sqrt(fma(d2, d2, fma(d1, d1, d0 * d0)));
+  if (OUT != FFMM_GRADIENT) acc[0] += keep ? (C4PI * ir) * w : 0.0;
+  if (OUT != FFMM_POTENTIAL) {
+    double a = -C4PI * ir * ir * ir;
+    acc[1] += keep ? (a * d0) * w : 0.0;
+    acc[2] += keep ? (a * d1) * w : 0.0;
+    acc[3] += keep ? (a * d2) * w : 0.0;
+  }
+}
+
+struct EvalArgs {
+  const int32_t *leaf_nodes;   // per leaf slot
+  const int32_t *tptr;         // targets of leaf slot s: [tptr[s], tptr[s+1]); null: the leaf's own sources
+  const double *tx, *ty, *tz;  // targets, grouped by leaf
+  const int32_t *outidx;       // output position of each target
+  const double *Lc;            // Nl x nleaves
+  int Nl;
+  const double *surf;          // 3 x Nl SoA at the reference width
+  const double *lam;
+  const int32_t *level;
+  const double *centers;
+  int64_t nn;
+  const int32_t *col, *plo, *phi;
+  const double *sx, *sy, *sz, *tau;
+  double *out;
+};
+
+template <int OUT>
+__global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
+  const int s = blockIdx.x;
+  const int i = a.leaf_nodes[s];
+  const bool self = a.tptr == nullptr;
+  const int32_t t0 = self ? a.plo[i] : a.tptr[s];
+  const int32_t t1 = self ? a.phi[i] + 1 : a.tptr[s + 1];
+  if (t0 >= t1) return;
+  const double c0 = a.centers[i], c1 = a.centers[a.nn + i], c2 = a.centers[2 * a.nn + i];
+  const double sc = a.lam[a.level[i]];
+  const double *__restrict__ L = a.Lc + (int64_t)s * a.Nl;
+  const double *__restrict__ X = a.surf;
+  const double *__restrict__ Y = a.surf + a.Nl;
+  const double *__restrict__ Z = a.surf + 2 * a.Nl;
+  int32_t nbr[NCOLL];
+#pragma unroll
+  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  for (int32_t tb = t0; tb < t1; tb += EV_TB) {
+    const int32_t j = tb + (int32_t)threadIdx.x;
+    const bool valid = j < t1;
+    const int32_t jj = valid ? j : t0;
+    const double x = self ? a.sx[jj] : a.tx[jj];
+    const double y = self ? a.sy[jj] : a.ty[jj];
+    const double z = self ? a.sz[jj] : a.tz[jj];
+    const double xr = x - c0, yr = y - c1, zr = z - c2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // l2t over the large surface (src/operators/ki_common.jl:293-315)
+    for (int k = 0; k < a.Nl; k++)
+      accumulate<OUT>(xr - X[k] * sc, yr - Y[k] * sc, zr - Z[k] * sc, L[k], acc);
+    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363)
+    const int32_t skip = self ? j : -1;
+    for (int m = 0; m < NCOLL; m++) {
+      const int32_t nb = nbr[m];
+      if (nb < 0) continue;
+      const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
+      for (int32_t q = p0; q <= p1; q++)
+        accumulate_masked<OUT>(x - a.sx[q], y - a.sy[q], z - a.sz[q], a.tau[q], q != skip, acc);
+    }
+    if (valid) {
+      const int64_t o = self ? a.outidx[j] : a.outidx[j];
+      if (OUT == FFMM_POTENTIAL) {
+        a.out[o] = acc[0];
+      } else if (OUT == FFMM_GRADIENT) {
+        a.out[3 * o] = acc[1];
+        a.out[3 * o + 1] = acc[2];
+        a.out[3 * o + 2] = acc[3];
+      } else {
+        a.out[4 * o] = acc[0];
+        a.out[4 * o + 1] = acc[1];
+        a.out[4 * o + 2] = acc[2];
+        a.out[4 * o + 3] = acc[3];
+      }
+    }
+  }
+}
+
+static void launch_eval(Plan &plan, EvalArgs &a, int output) {
+  const int64_t nleaves = plan.leaf_off[plan.L];
+  if (nleaves == 0) return;
+  hipStream_t s = plan.ctx->stream;
+  if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
+  else if (output == FFMM_GRADIENT) k_eval<FFMM_GRADIENT><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
+  else k_eval<2><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
+  FFMM_LAUNCH_CHECK();
+}
+
+static EvalArgs base_args(Plan &plan, double *out) {
+  EvalArgs a{};
+  a.leaf_nodes = plan.leaf_nodes.p;
+  a.Lc = plan.Lc.p;
+  a.Nl = plan.ops->Nl;
+  a.surf = plan.ops->large.p;
+  a.lam = plan.lam_d.p;
+  a.level = plan.level.p;
+  a.centers = plan.centers.p;
+  a.nn = plan.nnodes;
+  a.col = plan.col.p;
+  a.plo = plan.plo.p;
+  a.phi = plan.phi.p;
+  a.sx = plan.sx.p;
+  a.sy = plan.sy.p;
+  a.sz = plan.sz.p;
+  a.tau = plan.tau.p;
+  a.out = out;
+  return a;
+}
+
+// evaluate_at_sources! (src/device.jl:153-163): out in the user's order via perm
+void evaluate_at_sources(Plan &plan, double *out, int output) {
+  EvalArgs a = base_args(plan, out);
+  a.tptr = nullptr;
+  a.outidx = plan.perm.p;
+  launch_eval(plan, a, output);
+}
+
+// ---------------------------------------------------------------------------
+// targets: locate (src/tree/locate.jl:10-22), bucket_by_node (src/tree/locate.jl:173-191)
+// ---------------------------------------------------------------------------
+__global__ void k_locate(const double *__restrict__ t, int64_t nt, double ox, double oy, double oz, double W0,
+                         const double *__restrict__ centers, int64_t nn, const uint8_t *__restrict__ leaf,
+                         const int32_t *__restrict__ child_first, const int32_t *__restrict__ node_leafslot,
+                         uint32_t *__restrict__ key, int32_t *__restrict__ idx) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nt) return;
+  double x = t[3 * q], y = t[3 * q + 1], z = t[3 * q + 2];
+  bool inside = ox <= x && x <= ox + W0 && oy <= y && y <= oy + W0 && oz <= z && z <= oz + W0;
+  uint32_t k = 0xFFFFFFFFu;
+  if (inside) {
+    int32_t i = 0;
+    while (!leaf[i]) {
+      int slot = (x > centers[i]) | ((y > centers[nn + i]) << 1) | ((z > centers[2 * nn + i]) << 2);
+      i = child_first[i] + slot;
+    }
+    k = (uint32_t)node_leafslot[i];
+  }
+  key[q] = k;
+  idx[q] = (int32_t)q;
+}
+
+// first position of each leaf slot in the sorted keys
+__global__ void k_bucket_ptr(const uint32_t *__restrict__ skeys, int64_t nt, int64_t nleaves, int32_t *__restrict__ ptr) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nleaves) return;
+  int64_t a = 0, b = nt;
+  while (a < b) {
+    int64_t m = (a + b) >> 1;
+    if ((int64_t)skeys[m] < s) a = m + 1;
+    else b = m;
+  }
+  ptr[s] = (int32_t)a;
+}
+
+__global__ void k_gather_targets(const double *__restrict__ t, const int32_t *__restrict__ order, int64_t nt,
+                                 double *__restrict__ tx, double *__restrict__ ty, double *__restrict__ tz) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nt) return;
+  int64_t o = order[q];
+  tx[q] = t[3 * o];
+  ty[q] = t[3 * o + 1];
+  tz[q] = t[3 * o + 2];
+}
+
+void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output) {
+  if (nt <= 0) return;
+  if (nt >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many targets for 32-bit indices");
+  Context &ctx = *plan.ctx;
+  hipStream_t s = ctx.stream;
+  const int64_t nleaves = plan.leaf_off[plan.L];
+  DBuf<uint32_t> keys, skeys;
+  DBuf<int32_t> idx;
+  keys.resize(nt);
+  skeys.resize(nt);
+  idx.resize(nt);
+  plan.tgt_order.resize(nt);
+  k_locate<<<nblocks(nt, TB), TB, 0, s>>>(targets, nt, plan.origin[0], plan.origin[1], plan.origin[2], plan.W0,
+                                          plan.centers.p, plan.nnodes, plan.leaf.p, plan.child_first.p,
+                                          plan.node_leafslot.p, keys.p, idx.p);
+  FFMM_LAUNCH_CHECK();
+  sort_pairs_u32(ctx, keys.p, skeys.p, idx.p, plan.tgt_order.p, nt, 32);
+  uint32_t last = 0;
+  FFMM_HIP(hipMemcpyAsync(&last, skeys.p + nt - 1, sizeof last, hipMemcpyDeviceToHost, s));
+  ctx.sync();
+  if (last == 0xFFFFFFFFu) fail(FFMM_ERR_ARGUMENT, "a target lies outside the tree's box");
+  plan.tgt_ptr.resize(nleaves + 1);
+  k_bucket_ptr<<<nblocks(nleaves + 1, TB), TB, 0, s>>>(skeys.p, nt, nleaves, plan.tgt_ptr.p);
+  plan.tgt_sorted.resize(3 * nt);
+  double *tx = plan.tgt_sorted.p, *ty = tx + nt, *tz = ty + nt;
+  k_gather_targets<<<nblocks(nt, TB), TB, 0, s>>>(targets, plan.tgt_order.p, nt, tx, ty, tz);
+  FFMM_LAUNCH_CHECK();
+  EvalArgs a = base_args(plan, out);
+  a.tptr = plan.tgt_ptr.p;
+  a.tx = tx;
+  a.ty = ty;
+  a.tz = tz;
+  a.outidx = plan.tgt_order.p;
+  launch_eval(plan, a, output);
+}
+
+}  // namespace ffmm

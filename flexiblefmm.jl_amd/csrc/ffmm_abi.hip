@@ -1,0 +1,483 @@
+// The C ABI of libffmm_hip (include/ffmm.h): handles, error reporting, the operator
+// cache upload and the plan's life cycle.  Every entry point catches the library's
+// exceptions and turns them into a status code plus a thread-local message.
+#include "ffmm_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+struct ffmm_context_s { ffmm::Context c; };
+struct ffmm_opcache_s { ffmm::OpCache c; };
+struct ffmm_plan_s { ffmm::Plan c; };
+
+namespace ffmm {
+
+static thread_local std::string g_last_error;
+
+void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+template <class F>
+static int guard(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return FFMM_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return FFMM_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return FFMM_ERR_STATE;
+  }
+}
+
+static void set_device(Context &ctx) { FFMM_HIP(hipSetDevice(ctx.device)); }
+
+// Stage timers record event pairs on the plan's stream without synchronising; the
+// times are read when asked for (every entry point has synchronised by then).  The
+// M2L stage keeps one pair per level.
+static hipEvent_t &stage_event(Plan &p, int stage, int level, int which) {
+  if (stage != 4) return p.ev[2 * stage + which];
+  if ((int)p.m2l_ev.size() < 2 * (level + 1)) {
+    size_t old = p.m2l_ev.size();
+    p.m2l_ev.resize(2 * (level + 1), nullptr);
+    for (size_t k = old; k < p.m2l_ev.size(); k++) FFMM_HIP(hipEventCreate(&p.m2l_ev[k]));
+  }
+  return p.m2l_ev[2 * level + which];
+}
+StageTimer::StageTimer(Plan &p, int s, int l) : plan(p), stage(s), level(l) {
+  FFMM_HIP(hipEventRecord(stage_event(plan, stage, level, 0), plan.ctx->stream));
+}
+StageTimer::~StageTimer() {
+  (void)hipEventRecord(stage_event(plan, stage, level, 1), plan.ctx->stream);
+  if (stage == 4) plan.m2l_levels_timed = std::max(plan.m2l_levels_timed, level + 1);
+  else plan.stage_recorded[stage] = true;
+}
+
+// the boundary points of the p^3 lattice in CartesianIndices order (coordinate 0
+// fastest) and their linear index in the padded (2p)^3 lattice
+// (src/operators/spectral_m2l.jl:103-115)
+static std::vector<int32_t> cube_embed(int p) {
+  std::vector<int32_t> e;
+  const int n = 2 * p;
+  for (int i3 = 0; i3 < p; i3++)
+    for (int i2 = 0; i2 < p; i2++)
+      for (int i1 = 0; i1 < p; i1++) {
+        bool bnd = i1 == 0 || i1 == p - 1 || i2 == 0 || i2 == p - 1 || i3 == 0 || i3 == p - 1;
+        if (bnd) e.push_back(i1 + n * i2 + n * n * i3);
+      }
+  return e;
+}
+
+}  // namespace ffmm
+
+using namespace ffmm;
+
+extern "C" {
+
+const char *ffmm_last_error(void) { return g_last_error.c_str(); }
+int ffmm_abi_version(void) { return 1; }
+
+int ffmm_context_create(int device, void *stream, ffmm_context *out) {
+  return guard([&] {
+    if (!out) fail(FFMM_ERR_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int ndev = 0;
+    FFMM_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+      fail(FFMM_ERR_ARGUMENT, "device " + std::to_string(device) + " of " + std::to_string(ndev));
+    auto h = std::make_unique<ffmm_context_s>();
+    Context &c = h->c;
+    c.device = device;
+    set_device(c);
+    if (stream) {
+      c.stream = (hipStream_t)stream;
+    } else {
+      FFMM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      c.own_stream = true;
+    }
+    FFMM_BLAS(rocblas_create_handle(&c.blas));
+    FFMM_BLAS(rocblas_set_stream(c.blas, c.stream));
+    FFMM_BLAS(rocblas_set_pointer_mode(c.blas, rocblas_pointer_mode_host));
+    FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+    *out = h.release();
+  });
+}
+
+int ffmm_context_destroy(ffmm_context ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    Context &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    (void)hipStreamSynchronize(c.stream);
+    c.tmp.release();
+    if (c.pinned) (void)hipHostFree(c.pinned);
+    if (c.blas) rocblas_destroy_handle(c.blas);
+    if (c.own_stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+  });
+}
+
+int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite) {
+  return guard([&] {
+    if (!ctx || !lo || !hi || !all_finite) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(ctx->c);
+    bounds(ctx->c, xyz, n, lo, hi, all_finite);
+  });
+}
+
+int ffmm_opcache_create(ffmm_context ctx, int p, int pc, double d, ffmm_opcache *out) {
+  return guard([&] {
+    if (!ctx || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (p < 2) fail(FFMM_ERR_ARGUMENT, "p must be at least 2");
+    if (pc < p) fail(FFMM_ERR_ARGUMENT, "pc must be at least p");
+    if (!(d > 0 && d < 2.0 / 3.0)) fail(FFMM_ERR_ARGUMENT, "d must lie in (0, 2/3)");
+    auto h = std::make_unique<ffmm_opcache_s>();
+    OpCache &o = h->c;
+    o.ctx = &ctx->c;
+    o.p = p;
+    o.pc = pc;
+    o.d = d;
+    o.Ns = p * p * p - (p - 2) * (p - 2) * (p - 2);
+    o.Nl = pc * pc * pc - (pc - 2) * (pc - 2) * (pc - 2);
+    o.embed_h = cube_embed(p);
+    *out = h.release();
+  });
+}
+
+int ffmm_opcache_destroy(ffmm_opcache cache) {
+  return guard([&] {
+    if (!cache) return;
+    (void)hipSetDevice(cache->c.ctx->device);
+    delete cache;
+  });
+}
+
+int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small, const double *large, int r_up,
+                          const double *Uh_up, const double *invS_up, const double *V_up, int r_dn,
+                          const double *Uh_dn, const double *invS_dn, const double *V_dn, const double *M2M,
+                          const double *L2L, int nclass, const double *class_ops, const int32_t *far_class,
+                          const int32_t *idx, const int8_t *sgn, const double *tables) {
+  return guard([&] {
+    if (!cache) fail(FFMM_ERR_ARGUMENT, "null cache");
+    OpCache &o = cache->c;
+    Context &ctx = *o.ctx;
+    set_device(ctx);
+    if (!(w_ref > 0)) fail(FFMM_ERR_ARGUMENT, "the reference width must be positive");
+    const int Ns = o.Ns, Nl = o.Nl;
+    if (r_up < 1 || r_up > Ns || r_dn < 1 || r_dn > Ns) fail(FFMM_ERR_ARGUMENT, "pseudo-inverse ranks out of range");
+    if (nclass < 1 || nclass > 32) fail(FFMM_ERR_ARGUMENT, "nclass out of range");
+    if (!small || !large || !Uh_up || !invS_up || !V_up || !Uh_dn || !invS_dn || !V_dn || !M2M || !L2L ||
+        !class_ops || !far_class || !idx || !sgn)
+      fail(FFMM_ERR_ARGUMENT, "null operator");
+    hipStream_t s = ctx.stream;
+    o.w_ref = w_ref;
+    o.r_up = r_up;
+    o.r_dn = r_dn;
+    o.nclass = nclass;
+    o.small_h.assign(small, small + 3 * Ns);
+    o.large_h.assign(large, large + 3 * Nl);
+    std::vector<double> soa(3 * Nl);
+    for (int k = 0; k < Nl; k++)
+      for (int d = 0; d < 3; d++) soa[(size_t)d * Nl + k] = large[3 * k + d];
+    o.large.upload(soa.data(), soa.size(), s);
+    std::vector<float> soa32(soa.begin(), soa.end());
+    o.large32.upload(soa32.data(), soa32.size(), s);
+    o.Uh_up.upload(Uh_up, (size_t)r_up * Nl, s);
+    o.Uh_dn.upload(Uh_dn, (size_t)r_dn * Ns, s);
+    std::vector<double> vs((size_t)Ns * r_up);
+    for (int j = 0; j < r_up; j++)
+      for (int i = 0; i < Ns; i++) vs[(size_t)j * Ns + i] = V_up[(size_t)j * Ns + i] * invS_up[j];
+    o.Vs_up.upload(vs.data(), vs.size(), s);
+    std::vector<double> vd((size_t)Nl * r_dn);
+    for (int j = 0; j < r_dn; j++)
+      for (int i = 0; i < Nl; i++) vd[(size_t)j * Nl + i] = V_dn[(size_t)j * Nl + i] * invS_dn[j];
+    o.Vs_dn.upload(vd.data(), vd.size(), s);
+    o.M2M.upload(M2M, (size_t)Ns * NCHILD * Ns, s);
+    o.L2L.upload(L2L, (size_t)NCHILD * Ns * Ns, s);
+    o.class_ops.upload(class_ops, (size_t)nclass * Ns * Ns, s);
+    std::vector<float> c32(class_ops, class_ops + (size_t)nclass * Ns * Ns);
+    o.class_ops32.upload(c32.data(), c32.size(), s);
+    o.far_class_h.assign(far_class, far_class + NFAR);
+    for (int f = 0; f < NFAR; f++)
+      if (far_class[f] < 0 || far_class[f] >= nclass) fail(FFMM_ERR_ARGUMENT, "far offset class out of range");
+    o.far_class.upload(far_class, NFAR, s);
+    std::vector<int32_t> inv((size_t)NFAR * Ns, -1);
+    for (int f = 0; f < NFAR; f++)
+      for (int i = 0; i < Ns; i++) {
+        int32_t k = idx[(size_t)f * Ns + i];
+        if (k < 0 || k >= Ns || inv[(size_t)f * Ns + k] >= 0) fail(FFMM_ERR_ARGUMENT, "idx is not a permutation");
+        inv[(size_t)f * Ns + k] = i;
+      }
+    o.idx.upload(idx, (size_t)NFAR * Ns, s);
+    o.inv.upload(inv.data(), inv.size(), s);
+    o.sgn.upload(sgn, (size_t)NFAR * Ns, s);
+    o.embed.upload(o.embed_h.data(), o.embed_h.size(), s);
+    o.has_tables = tables != nullptr;
+    if (tables) {
+      const int n = 2 * o.p;
+      const size_t nfreq = (size_t)n * n * (o.p + 1);
+      o.tables.upload(tables, 2 * nfreq * NFAR, s);
+      std::vector<float> t32(tables, tables + 2 * nfreq * NFAR);
+      o.tables32.upload(t32.data(), t32.size(), s);
+    }
+    ctx.sync();
+    o.ready = true;
+  });
+}
+
+int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, const double origin[3], double width,
+                     int e0, int max_levels, int precision, int m2l, ffmm_plan *out) {
+  return guard([&] {
+    if (!cache || !out || !origin) fail(FFMM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    OpCache &o = cache->c;
+    if (!o.ready) fail(FFMM_ERR_STATE, "the operator cache holds no operators");
+    if (n <= 0) fail(FFMM_ERR_ARGUMENT, "no points");
+    if (!xyz) fail(FFMM_ERR_ARGUMENT, "null points");
+    if (ppl < 1) fail(FFMM_ERR_ARGUMENT, "ppl must be positive");
+    if (!(width > 0)) fail(FFMM_ERR_ARGUMENT, "box width must be positive");
+    if (precision != FFMM_PREC_F64 && precision != FFMM_PREC_MIXED) fail(FFMM_ERR_ARGUMENT, "unknown precision");
+    if (m2l != FFMM_M2L_DENSE && m2l != FFMM_M2L_SPECTRAL) fail(FFMM_ERR_ARGUMENT, "unknown M2L form");
+    if (m2l == FFMM_M2L_SPECTRAL && !o.has_tables) fail(FFMM_ERR_ARGUMENT, "the cache holds no spectral tables");
+    if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
+    Context &ctx = *o.ctx;
+    set_device(ctx);
+    auto h = std::make_unique<ffmm_plan_s>();
+    Plan &p = h->c;
+    p.ctx = &ctx;
+    p.ops = &o;
+    p.N = n;
+    p.ppl = ppl;
+    p.max_levels = max_levels;
+    p.precision = precision;
+    p.m2l = m2l;
+    for (int d = 0; d < 3; d++) p.origin[d] = origin[d];
+    p.W0 = width;
+    p.e0 = e0;
+    for (auto &e : p.ev) FFMM_HIP(hipEventCreate(&e));
+    {
+      StageTimer tm(p, 0);
+      build_tree(p, xyz);
+    }
+    p.lam.resize(p.L);
+    for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, e0 - l);
+    p.lam_d.upload(p.lam.data(), p.L, ctx.stream);
+    {
+      // node -> leaf slot
+      std::vector<int32_t> slot(p.nnodes, -1), leaves(p.leaf_off[p.L]);
+      p.leaf_nodes.download(leaves.data(), leaves.size(), ctx.stream);
+      ctx.sync();
+      for (size_t s = 0; s < leaves.size(); s++) slot[leaves[s]] = (int32_t)s;
+      p.node_leafslot.upload(slot.data(), slot.size(), ctx.stream);
+    }
+    {
+      StageTimer tm(p, 1);
+      build_lists(p);
+    }
+    allocate_pass_buffers(p);
+    ctx.sync();
+    *out = h.release();
+  });
+}
+
+int ffmm_plan_destroy(ffmm_plan plan) {
+  return guard([&] {
+    if (!plan) return;
+    (void)hipSetDevice(plan->c.ctx->device);
+    (void)hipStreamSynchronize(plan->c.ctx->stream);
+    for (auto &e : plan->c.ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : plan->c.m2l_ev)
+      if (e) (void)hipEventDestroy(e);
+    delete plan;
+  });
+}
+
+int ffmm_plan_sizes(ffmm_plan plan, int *nlevels, int64_t *nnodes, int64_t *npoints, int64_t *level_start,
+                    int64_t *level_count, int64_t *level_pairs, int64_t *level_s2m, int64_t *level_leaves) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    if (nlevels) *nlevels = p.L;
+    if (nnodes) *nnodes = p.nnodes;
+    if (npoints) *npoints = p.N;
+    for (int l = 0; l < p.L; l++) {
+      if (level_start) level_start[l] = p.lo[l];
+      if (level_count) level_count[l] = p.nl[l];
+      if (level_pairs) level_pairs[l] = p.lists[l].np;
+      if (level_s2m) level_s2m[l] = p.s2m_off[l + 1] - p.s2m_off[l];
+      if (level_leaves) level_leaves[l] = p.leaf_off[l + 1] - p.leaf_off[l];
+    }
+  });
+}
+
+int ffmm_plan_download_tree(ffmm_plan plan, int32_t *cell, int32_t *level, int32_t *parent, int32_t *child_first,
+                            int32_t *plo, int32_t *phi, uint8_t *leaf, uint8_t *ghost, uint8_t *xlist,
+                            uint8_t *has_source, int32_t *colleagues, int32_t *perm, int32_t *point_node,
+                            double *sorted_points) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    Context &ctx = *p.ctx;
+    set_device(ctx);
+    hipStream_t s = ctx.stream;
+    const int64_t nn = p.nnodes, N = p.N;
+    if (cell) {
+      std::vector<int32_t> c(3 * nn);
+      p.cell.download(c.data(), c.size(), s);
+      ctx.sync();
+      for (int64_t i = 0; i < nn; i++)
+        for (int d = 0; d < 3; d++) cell[3 * i + d] = c[d * nn + i];
+    }
+    if (colleagues) {
+      std::vector<int32_t> c((size_t)NCOLL * nn);
+      p.col.download(c.data(), c.size(), s);
+      ctx.sync();
+      for (int64_t i = 0; i < nn; i++)
+        for (int k = 0; k < NCOLL; k++) colleagues[NCOLL * i + k] = c[(size_t)k * nn + i];
+    }
+    if (level) p.level.download(level, nn, s);
+    if (parent) p.parent.download(parent, nn, s);
+    if (child_first) p.child_first.download(child_first, nn, s);
+    if (plo) p.plo.download(plo, nn, s);
+    if (phi) p.phi.download(phi, nn, s);
+    if (leaf) p.leaf.download(leaf, nn, s);
+    if (ghost) p.ghost.download(ghost, nn, s);
+    if (xlist) p.xlist.download(xlist, nn, s);
+    if (has_source) p.has_source.download(has_source, nn, s);
+    if (perm) p.perm.download(perm, N, s);
+    if (point_node) p.point_node.download(point_node, N, s);
+    if (sorted_points) {
+      std::vector<double> x(N), y(N), z(N);
+      p.sx.download(x.data(), N, s);
+      p.sy.download(y.data(), N, s);
+      p.sz.download(z.data(), N, s);
+      ctx.sync();
+      for (int64_t j = 0; j < N; j++) {
+        sorted_points[3 * j] = x[j];
+        sorted_points[3 * j + 1] = y[j];
+        sorted_points[3 * j + 2] = z[j];
+      }
+    }
+    ctx.sync();
+  });
+}
+
+int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *src, int32_t *far) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    if (level < 0 || level >= p.L) fail(FFMM_ERR_ARGUMENT, "level out of range");
+    set_device(*p.ctx);
+    LevelLists &li = p.lists[level];
+    hipStream_t s = p.ctx->stream;
+    if (ptr) li.ptr.download(ptr, p.nl[level] + 1, s);
+    if (src && li.np) li.src.download(src, li.np, s);
+    if (far && li.np) li.far.download(far, li.np, s);
+    p.ctx->sync();
+  });
+}
+
+int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted) {
+  return guard([&] {
+    if (!plan || !tau) fail(FFMM_ERR_ARGUMENT, "null argument");
+    Plan &p = plan->c;
+    set_device(*p.ctx);
+    {
+      StageTimer tm(p, 7);
+      gather_inputs(p, tau, sorted);
+    }
+    {
+      StageTimer tm(p, 2);
+      upward_pass(p);
+    }
+    p.m2l_levels_timed = 0;
+    {
+      StageTimer tm(p, 3);
+      downward_pass(p);
+    }
+    p.ctx->sync();
+    p.expansions_ready = true;
+  });
+}
+
+int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output) {
+  return guard([&] {
+    if (!plan || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    Plan &p = plan->c;
+    if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
+    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    set_device(*p.ctx);
+    {
+      StageTimer tm(p, 5);
+      evaluate_at_sources(p, out, output);
+    }
+    p.ctx->sync();
+  });
+}
+
+int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out, int output) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
+    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (nt < 0) fail(FFMM_ERR_DIMENSION, "negative target count");
+    if (nt == 0) return;
+    if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(*p.ctx);
+    {
+      StageTimer tm(p, 6);
+      evaluate_targets(p, targets, nt, out, output);
+    }
+    p.ctx->sync();
+  });
+}
+
+int ffmm_matvec(ffmm_plan plan, const double *tau, int sorted, double *out, int output) {
+  int rc = ffmm_build_expansions(plan, tau, sorted);
+  if (rc != FFMM_OK) return rc;
+  return ffmm_evaluate_at_sources(plan, out, output);
+}
+
+int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *plocals, double *leaf_locals) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    set_device(*p.ctx);
+    hipStream_t s = p.ctx->stream;
+    if (multipoles) p.Mp.download(multipoles, (size_t)p.ops->Ns * p.nnodes, s);
+    if (plocals) p.Pl.download(plocals, (size_t)p.ops->Ns * p.nnodes, s);
+    if (leaf_locals) p.Lc.download(leaf_locals, (size_t)p.ops->Nl * p.leaf_off[p.L], s);
+    p.ctx->sync();
+  });
+}
+
+int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n) {
+  return guard([&] {
+    if (!plan || !ms) fail(FFMM_ERR_ARGUMENT, "null argument");
+    Plan &p = plan->c;
+    set_device(*p.ctx);
+    p.ctx->sync();
+    for (int i = 0; i < n && i < 8; i++) {
+      float t = 0.f;
+      double v = 0.0;
+      if (i == 4) {
+        for (int l = 0; l < p.m2l_levels_timed; l++)
+          if (p.m2l_ev[2 * l] && hipEventElapsedTime(&t, p.m2l_ev[2 * l], p.m2l_ev[2 * l + 1]) == hipSuccess) v += t;
+      } else if (p.stage_recorded[i] && hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
+        v = t;
+      }
+      ms[i] = v;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+// Internal declarations of libffmm_hip: device buffers, the context, the operator
+// cache and the plan.  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ffmm.h"
+
+namespace ffmm {
+
+constexpr int NCHILD = 8;
+constexpr int NCOLL = 27;
+constexpr int SELF_SLOT = 13;
+constexpr int NFAR = 316;
+constexpr int MAX_KEY_LEVELS = 21;   // 3 bits per level in a 63-bit key
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define FFMM_HIP(call)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) ::ffmm::fail(FFMM_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define FFMM_BLAS(call)                                                                         \
+  do {                                                                                          \
+    rocblas_status s_ = (call);                                                                 \
+    if (s_ != rocblas_status_success)                                                           \
+      ::ffmm::fail(FFMM_ERR_DEVICE, std::string(#call) + ": " + rocblas_status_to_string(s_));  \
+  } while (0)
+#define FFMM_LAUNCH_CHECK() FFMM_HIP(hipGetLastError())
+
+inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// A device array; resize() discards the contents, grow() keeps them.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0, cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap) { o.p = nullptr; o.n = o.cap = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; cap = o.cap; o.p = nullptr; o.n = o.cap = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = cap = 0;
+  }
+  void resize(size_t m) {
+    if (m > cap) {
+      release();
+      size_t bytes = (m ? m : 1) * sizeof(T);
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        p = nullptr;
+        fail(FFMM_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+      }
+      cap = m;
+    }
+    n = m;
+  }
+  void grow(size_t m, hipStream_t s) {
+    if (m <= cap) { n = m; return; }
+    size_t c = cap ? cap : 16;
+    while (c < m) c *= 2;
+    T *q = nullptr;
+    hipError_t e = hipMalloc(&q, c * sizeof(T));
+    if (e != hipSuccess) fail(FFMM_ERR_MEMORY, "device allocation failed while growing a buffer");
+    if (n) FFMM_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    FFMM_HIP(hipStreamSynchronize(s));
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = c;
+    n = m;
+  }
+  T *data() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+  void upload(const T *h, size_t m, hipStream_t s) {
+    resize(m);
+    if (m) FFMM_HIP(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void download(T *h, size_t m, hipStream_t s) const {
+    if (m) FFMM_HIP(hipMemcpyAsync(h, p, m * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  rocblas_handle blas = nullptr;
+  int64_t *pinned = nullptr;   // small pinned readback area (64 entries)
+  DBuf<char> tmp;               // scratch of the primitives
+  void sync() { FFMM_HIP(hipStreamSynchronize(stream)); }
+  // read back `n` int64 values from device memory (blocking)
+  void readback(int64_t *host, const int64_t *dev, int n);
+};
+
+struct OpCache {
+  Context *ctx = nullptr;
+  int p = 8, pc = 16, Ns = 0, Nl = 0, nclass = 0, r_up = 0, r_dn = 0;
+  double d = 0.1, w_ref = 0.0;
+  bool ready = false;
+  bool has_tables = false;
+  std::vector<double> small_h, large_h;     // host copies of the surfaces (3 x n)
+  DBuf<double> large;                        // 3 x Nl SoA: x[Nl], y[Nl], z[Nl]
+  DBuf<float> large32;
+  DBuf<double> Uh_up, Vs_up, Uh_dn, Vs_dn;   // Vs = V diag(invS)
+  DBuf<double> M2M, L2L, class_ops;
+  DBuf<float> class_ops32;
+  DBuf<int32_t> far_class, idx, inv;         // idx/inv: [f * Ns + i]
+  DBuf<int8_t> sgn;
+  DBuf<double> tables;                       // 316 x nfreq complex (interleaved)
+  DBuf<float> tables32;
+  DBuf<int32_t> embed;                       // Ns: lattice index of each small-surface point
+  std::vector<int32_t> embed_h;
+  std::vector<int32_t> far_class_h;
+};
+
+// per-level lists of far pairs (targets in node order) and their class-major form
+struct LevelLists {
+  int64_t np = 0, nruns = 0;
+  DBuf<int32_t> ptr, src, far;          // by target: ptr (nl + 1, 0-based pair index)
+  DBuf<int32_t> csrc, cfar, rtgt, rptr; // class major; runs: (level-local target, first pair)
+  std::vector<int64_t> class_pair;      // nclass + 1 offsets into csrc
+  std::vector<int64_t> class_run;       // nclass + 1 offsets into rtgt
+};
+
+struct Plan {
+  Context *ctx = nullptr;
+  OpCache *ops = nullptr;
+  int64_t N = 0;
+  int ppl = 0, max_levels = 30, precision = FFMM_PREC_F64, m2l = FFMM_M2L_DENSE;
+  double origin[3] = {0, 0, 0}, W0 = 1.0;
+  int e0 = 0;
+  int L = 0;
+  int64_t nnodes = 0;
+  std::vector<int64_t> lo, nl;
+  std::vector<double> lam;   // level width / reference width
+  DBuf<double> lam_d;
+  DBuf<int32_t> node_leafslot;   // node -> position in leaf_nodes, -1 if not a real leaf
+  // the flat tree
+  DBuf<int32_t> cell;        // 3 x n SoA
+  DBuf<int32_t> level, parent, child_first, plo, phi, col, perm, point_node;   // col: [k * n + i]
+  DBuf<uint8_t> leaf, ghost, xlist, has_source;
+  DBuf<double> sx, sy, sz;   // sorted points
+  DBuf<double> centers;      // 3 x n SoA
+  DBuf<float> sx32, sy32, sz32;
+  // node lists per level (concatenated): childless sourced nodes, real leaves
+  std::vector<int64_t> s2m_off, leaf_off;   // L + 1 offsets
+  DBuf<int32_t> s2m_nodes, leaf_nodes;
+  std::vector<LevelLists> lists;
+  // expansions
+  DBuf<double> Mp, Pl, Lc;   // Ns x n, Ns x n, Nl x nleaves
+  DBuf<float> Lc32;
+  DBuf<double> tau;          // sorted densities
+  DBuf<float> tau32;
+  // work buffers
+  DBuf<double> pM, tup, Mup, tM, G, tdn, Gc, Yc;
+  DBuf<float> Gc32, Yc32, pM32;
+  // spectral M2L workspace
+  DBuf<double> spec_S, spec_U;
+  DBuf<float> spec_S32, spec_U32;
+  DBuf<int32_t> spec_slot, spec_tgts;
+  // targets
+  DBuf<double> tgt_sorted;
+  DBuf<int32_t> tgt_node, tgt_leafslot, tgt_order, tgt_ptr;
+  DBuf<uint32_t> tgt_keys;
+  hipEvent_t ev[16] = {};
+  std::vector<hipEvent_t> m2l_ev;   // per level: start, end
+  int m2l_levels_timed = 0;
+  bool stage_recorded[8] = {};
+  bool expansions_ready = false;
+};
+
+// ---- primitives (prims.hip) ----
+// stable LSD radix sort of (key, value) pairs on the bits [0, end_bit)
+void sort_pairs_u64(Context &ctx, const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout,
+                    int64_t n, int end_bit);
+void sort_pairs_u32(Context &ctx, const uint32_t *kin, uint32_t *kout, const int32_t *vin, int32_t *vout,
+                    int64_t n, int end_bit);
+// out[i] = sum of in[0..i), i < n; returns the total (blocking read back)
+int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_t n);
+// indices i (ascending) with flag[i] != 0; returns their number (blocking read back)
+int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n);
+
+// ---- tree (tree.hip) ----
+void bounds(Context &ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite);
+void build_tree(Plan &plan, const double *xyz);
+
+// ---- lists (lists.hip) ----
+void build_lists(Plan &plan);
+
+// ---- passes (passes.hip) ----
+void allocate_pass_buffers(Plan &plan);
+void gather_inputs(Plan &plan, const double *tau, int sorted);
+void upward_pass(Plan &plan);
+void downward_pass(Plan &plan);
+void m2l_dense_level(Plan &plan, int l);
+void m2l_spectral_level(Plan &plan, int l);
+
+// ---- evaluation (evaluate.hip) ----
+void evaluate_at_sources(Plan &plan, double *out, int output);
+void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output);
+
+// ---- timing ----
+struct StageTimer {
+  Plan &plan;
+  int stage, level;
+  StageTimer(Plan &p, int s, int l = 0);
+  ~StageTimer();
+};
+
+}  // namespace ffmm

@@ -1,0 +1,236 @@
+// The interaction lists of every level, built on the device from the flat tree.
+//
+// Per level (0-based levels >= 2): one work-item per target counts its far pairs --
+// the sourced children of its parent's colleagues at Chebyshev offset >= 2, in the
+// enumeration order of the colleagues and their children -- a scan places them and a
+// fill writes (source, far offset).  Then the pairs are counted per (target, class),
+// placed class-major (the pairs of a class with its targets in node order, each
+// target's consecutive) and filled, and the runs (one per (target, class) with pairs)
+// come from a compaction of the counts.
+//
+// Follows src/device_lists.jl:261-344, 425-484 (= src/tree/lists.jl:73-124 for the
+// pairs and src/operators/symmetry.jl:207-236 for the class lists); 0-based.
+#include "ffmm_internal.h"
+
+namespace ffmm {
+
+constexpr int TB = 256;
+
+__constant__ int32_t c_far_index[343];   // far_index_table (src/util.jl:87-97), -1: near
+__constant__ int32_t c_far_class[NFAR];
+
+struct TreePtrs {
+  const int32_t *cell, *parent, *child_first, *col;
+  const uint8_t *ghost, *has_source;
+  int64_t nn;
+};
+
+__device__ __forceinline__ bool far_offset(int32_t o0, int32_t o1, int32_t o2) {
+  return max(abs(o0), max(abs(o1), abs(o2))) >= 2;
+}
+
+// _pair_count! (src/device_lists.jl:263-285)
+__global__ void k_pair_count(TreePtrs t, int64_t lo, int64_t nl, int32_t *__restrict__ cnt) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nl) return;
+  int64_t i = lo + q;
+  int32_t c = 0;
+  if (!t.ghost[i]) {
+    int32_t p = t.parent[i];
+    int32_t ci0 = t.cell[i], ci1 = t.cell[t.nn + i], ci2 = t.cell[2 * t.nn + i];
+    for (int k = 0; k < NCOLL; k++) {
+      int32_t pc = t.col[(int64_t)k * t.nn + p];
+      if (pc < 0) continue;
+      int32_t s0 = t.child_first[pc];
+      if (s0 < 0) continue;
+      for (int32_t s = s0; s < s0 + NCHILD; s++) {
+        if (!t.has_source[s]) continue;
+        if (far_offset(t.cell[s] - ci0, t.cell[t.nn + s] - ci1, t.cell[2 * t.nn + s] - ci2)) c++;
+      }
+    }
+  }
+  cnt[q] = c;
+}
+
+// _pair_fill! (src/device_lists.jl:286-312)
+__global__ void k_pair_fill(TreePtrs t, int64_t lo, int64_t nl, const int32_t *__restrict__ ptr,
+                            int32_t *__restrict__ src, int32_t *__restrict__ far) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nl) return;
+  int64_t i = lo + q;
+  if (t.ghost[i]) return;
+  int32_t pos = ptr[q];
+  int32_t p = t.parent[i];
+  int32_t ci0 = t.cell[i], ci1 = t.cell[t.nn + i], ci2 = t.cell[2 * t.nn + i];
+  for (int k = 0; k < NCOLL; k++) {
+    int32_t pc = t.col[(int64_t)k * t.nn + p];
+    if (pc < 0) continue;
+    int32_t s0 = t.child_first[pc];
+    if (s0 < 0) continue;
+    for (int32_t s = s0; s < s0 + NCHILD; s++) {
+      if (!t.has_source[s]) continue;
+      int32_t o0 = t.cell[s] - ci0, o1 = t.cell[t.nn + s] - ci1, o2 = t.cell[2 * t.nn + s] - ci2;
+      if (!far_offset(o0, o1, o2)) continue;
+      src[pos] = s;
+      far[pos] = c_far_index[(o0 + 3) + 7 * (o1 + 3) + 49 * (o2 + 3)];
+      pos++;
+    }
+  }
+}
+
+// _class_count! (src/device_lists.jl:314-324), class-major: cnt[c * nl + q]
+__global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__restrict__ ptr, int64_t nl, int nclass,
+                              int32_t *__restrict__ cnt) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nl) return;
+  int32_t c16[32];
+  for (int c = 0; c < nclass; c++) c16[c] = 0;
+  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) c16[c_far_class[far[p]]]++;
+  for (int c = 0; c < nclass; c++) cnt[(int64_t)c * nl + q] = c16[c];
+}
+
+// _class_fill! (src/device_lists.jl:325-334)
+__global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__restrict__ far,
+                             const int32_t *__restrict__ ptr, const int32_t *__restrict__ cstart, int64_t nl,
+                             int nclass, int32_t *__restrict__ csrc, int32_t *__restrict__ cfar) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nl) return;
+  int32_t cur[32];
+  for (int c = 0; c < nclass; c++) cur[c] = cstart[(int64_t)c * nl + q];
+  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) {
+    int32_t f = far[p];
+    int c = c_far_class[f];
+    int32_t pos = cur[c]++;
+    csrc[pos] = src[p];
+    cfar[pos] = f;
+  }
+}
+
+__global__ void k_flag_pos(const int32_t *__restrict__ cnt, int64_t n, uint8_t *__restrict__ flag) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) flag[e] = cnt[e] > 0;
+}
+
+// _run_fill! (src/device_lists.jl:335-344): run r = compacted entry e of the class-major
+// (class, target) table: level-local target e % nl, first pair cstart[e]
+__global__ void k_run_fill(const int32_t *__restrict__ ent, int64_t nruns, const int32_t *__restrict__ cstart,
+                           int64_t nl, int32_t *__restrict__ rtgt, int32_t *__restrict__ rptr) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  int64_t e = ent[r];
+  rtgt[r] = (int32_t)(e % nl);
+  rptr[r] = cstart[e];
+}
+
+__global__ void k_set_i32(int32_t *a, int32_t v) { a[0] = v; }
+
+// per class c: the first pair (cstart at entry c * nl) and the first run (the first
+// compacted entry >= c * nl; the entries are ascending)
+__global__ void k_class_offsets(const int32_t *__restrict__ cstart, const int32_t *__restrict__ ent, int64_t nruns,
+                                int64_t nl, int nclass, int32_t *__restrict__ out) {
+  int c = threadIdx.x;
+  if (c >= nclass) return;
+  int64_t key = (int64_t)c * nl;
+  out[c] = cstart[key];
+  int64_t a = 0, b = nruns;
+  while (a < b) {
+    int64_t m = (a + b) >> 1;
+    if (ent[m] < key) a = m + 1;
+    else b = m;
+  }
+  out[nclass + c] = (int32_t)a;
+}
+
+void build_lists(Plan &plan) {
+  Context &ctx = *plan.ctx;
+  OpCache &ops = *plan.ops;
+  hipStream_t s = ctx.stream;
+  {
+    int32_t fi[343];
+    for (int u = 0, nf = 0; u < 343; u++) {
+      int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
+      int m = std::max(std::abs(x), std::max(std::abs(y), std::abs(z)));
+      fi[u] = m >= 2 ? nf++ : -1;
+    }
+    FFMM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_far_index), fi, sizeof fi, 0, hipMemcpyHostToDevice, s));
+    FFMM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_far_class), ops.far_class_h.data(), sizeof(int32_t) * NFAR, 0,
+                                    hipMemcpyHostToDevice, s));
+    ctx.sync();
+  }
+  const int nclass = ops.nclass;
+  if (nclass > 32) fail(FFMM_ERR_ARGUMENT, "at most 32 symmetry classes");
+  TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};
+  plan.lists.clear();
+  plan.lists.resize(plan.L);
+  DBuf<int32_t> cnt, ccnt, cstart, ent, offs;
+  DBuf<uint8_t> flag;
+  offs.resize(2 * nclass);
+  for (int l = 0; l < plan.L; l++) {
+    LevelLists &li = plan.lists[l];
+    int64_t lo = plan.lo[l], nl = plan.nl[l];
+    li.class_pair.assign(nclass + 1, 0);
+    li.class_run.assign(nclass + 1, 0);
+    li.ptr.resize(nl + 1);
+    if (l < 2) {
+      FFMM_HIP(hipMemsetAsync(li.ptr.p, 0, sizeof(int32_t) * (nl + 1), s));
+      continue;
+    }
+    cnt.resize(nl);
+    k_pair_count<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, cnt.p);
+    FFMM_LAUNCH_CHECK();
+    int64_t np = exclusive_scan_i32(ctx, cnt.p, li.ptr.p, nl);
+    if (np >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many far pairs on one level for 32-bit pair indices");
+    k_set_i32<<<1, 1, 0, s>>>(li.ptr.p + nl, (int32_t)np);
+    li.np = np;
+    li.src.resize(np);
+    li.far.resize(np);
+    if (np > 0) {
+      k_pair_fill<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, li.ptr.p, li.src.p, li.far.p);
+      FFMM_LAUNCH_CHECK();
+    }
+    // the class lists
+    int64_t ne = (int64_t)nclass * nl;
+    ccnt.resize(ne);
+    cstart.resize(ne);
+    k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ccnt.p);
+    FFMM_LAUNCH_CHECK();
+    exclusive_scan_i32(ctx, ccnt.p, cstart.p, ne);
+    li.csrc.resize(np);
+    li.cfar.resize(np);
+    if (np > 0) {
+      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, cstart.p, nl, nclass, li.csrc.p,
+                                                  li.cfar.p);
+      FFMM_LAUNCH_CHECK();
+    }
+    // the runs
+    flag.resize(ne);
+    ent.resize(ne);
+    k_flag_pos<<<nblocks(ne, TB), TB, 0, s>>>(ccnt.p, ne, flag.p);
+    FFMM_LAUNCH_CHECK();
+    int64_t nruns = compact_flags_u8(ctx, flag.p, ent.p, ne);
+    li.nruns = nruns;
+    li.rtgt.resize(nruns);
+    li.rptr.resize(nruns + 1);
+    if (nruns > 0) {
+      k_run_fill<<<nblocks(nruns, TB), TB, 0, s>>>(ent.p, nruns, cstart.p, nl, li.rtgt.p, li.rptr.p);
+      FFMM_LAUNCH_CHECK();
+    }
+    k_set_i32<<<1, 1, 0, s>>>(li.rptr.p + nruns, (int32_t)np);
+    // per-class offsets of the pairs (the class-major scan at each class's first entry)
+    // and of the runs (the first compacted entry of each class), one read back
+    k_class_offsets<<<1, 64, 0, s>>>(cstart.p, ent.p, nruns, nl, nclass, offs.p);
+    FFMM_LAUNCH_CHECK();
+    std::vector<int32_t> ho(2 * nclass);
+    offs.download(ho.data(), 2 * nclass, s);
+    ctx.sync();
+    for (int c = 0; c < nclass; c++) {
+      li.class_pair[c] = ho[c];
+      li.class_run[c] = ho[nclass + c];
+    }
+    li.class_run[nclass] = nruns;
+    li.class_pair[nclass] = np;
+  }
+  ctx.sync();
+}
+
+}  // namespace ffmm

@@ -1,0 +1,311 @@
+// The upward and downward passes on the device.
+//
+// Upward (levels L-1 .. 2, 0-based): S2M of the childless sourced nodes (check
+// potentials on the large surface, one workgroup per node with the node's sources
+// staged in LDS), the factored pseudo-inverse UC2E as two FP64 GEMMs, and M2M as one
+// GEMM per level straight on the children block of the multipole array (the children
+// of a sibling group are consecutive columns).  Downward (levels 2 .. L-1): L2L as one
+// GEMM whose output is the level's block of partial locals, the far field (M2L), and
+// the pseudo-inverse DC2E of the real leaves, written straight into the compact leaf
+// locals.  Follows device_build_expansions! (src/device_passes.jl:511-597) and the
+// host passes it restates (src/fmm.jl:170-262).
+#include "ffmm_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ffmm {
+
+constexpr int TB = 256;
+constexpr double C4PI = 0.07957747154594767;   // 1 / (4 pi), as 1 / (4 * Float64(pi))
+
+// ---------------------------------------------------------------------------
+// S2M: s2m_accumulate! (src/operators/ki_common.jl:256-272), _s2m_kernel!
+// (src/device_passes.jl:378-403).  pM[k, q] = sum_j tau_j K(x_j - c, y_k).
+// ---------------------------------------------------------------------------
+constexpr int S2M_TB = 256;
+constexpr int S2M_KP = 6;   // check points per thread: ceil(1352 / 256)
+
+__global__ void __launch_bounds__(S2M_TB) k_s2m(const int32_t *__restrict__ nodes, const double *__restrict__ centers,
+                                                int64_t nn, const int32_t *__restrict__ level,
+                                                const double *__restrict__ lam, const double *__restrict__ surf, int Nl,
+                                                const int32_t *__restrict__ plo, const int32_t *__restrict__ phi,
+                                                const double *__restrict__ sx, const double *__restrict__ sy,
+                                                const double *__restrict__ sz, const double *__restrict__ tau,
+                                                double *__restrict__ pM) {
+  __shared__ double xs[S2M_TB], ys[S2M_TB], zs[S2M_TB], ts[S2M_TB];
+  const int64_t q = blockIdx.x;
+  const int node = nodes[q];
+  const double c0 = centers[node], c1 = centers[nn + node], c2 = centers[2 * nn + node];
+  const double s = lam[level[node]];
+  const int t = threadIdx.x;
+  double acc[S2M_KP], y0[S2M_KP], y1[S2M_KP], y2[S2M_KP];
+#pragma unroll
+  for (int m = 0; m < S2M_KP; m++) {
+    int k = t + S2M_TB * m;
+    acc[m] = 0.0;
+    if (k < Nl) {
+      y0[m] = surf[k] * s;
+      y1[m] = surf[Nl + k] * s;
+      y2[m] = surf[2 * Nl + k] * s;
+    } else {
+      y0[m] = y1[m] = y2[m] = 1.0;
+    }
+  }
+  const int32_t a = plo[node], b = phi[node];
+  for (int32_t j0 = a; j0 <= b; j0 += S2M_TB) {
+    int32_t j = j0 + t;
+    if (j <= b) {
+      xs[t] = sx[j] - c0;
+      ys[t] = sy[j] - c1;
+      zs[t] = sz[j] - c2;
+      ts[t] = tau[j];
+    }
+    __syncthreads();
+    int cnt = min(S2M_TB, b - j0 + 1);
+    for (int jj = 0; jj < cnt; jj++) {
+      const double xr = xs[jj], yr = ys[jj], zr = zs[jj], tj = ts[jj];
+#pragma unroll
+      for (int m = 0; m < S2M_KP; m++) {
+        double d0 = y0[m] - xr, d1 = y1[m] - yr, d2 = y2[m] - zr;
+        double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+        acc[m] += (C4PI * (1.0 / sqrt(r2))) * tj;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < S2M_KP; m++) {
+    int k = t + S2M_TB * m;
+    if (k < Nl) pM[q * Nl + k] = acc[m];
+  }
+}
+
+// dst[i, cols[q]] = alpha * src[i, q]
+__global__ void k_scatter_cols(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ cols,
+                               int rows, int64_t ncols, double alpha) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ncols) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  dst[(int64_t)cols[q] * rows + i] = alpha * src[g];
+}
+// dst[i, q] = src[i, cols[q]]
+__global__ void k_gather_cols(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ cols,
+                              int rows, int64_t ncols) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ncols) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  dst[g] = src[(int64_t)cols[q] * rows + i];
+}
+// the parents of the sibling groups starting at node `first`: dst[i, parent[first + 8g]] = alpha src[i, g]
+__global__ void k_scatter_parents(double *__restrict__ dst, const double *__restrict__ src,
+                                  const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng, double alpha) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ng) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  dst[(int64_t)parent[first + NCHILD * q] * rows + i] = alpha * src[g];
+}
+__global__ void k_gather_parents(double *__restrict__ dst, const double *__restrict__ src,
+                                 const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ng) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  dst[g] = src[(int64_t)parent[first + NCHILD * q] * rows + i];
+}
+__global__ void k_fill(double *__restrict__ a, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+// the densities into the tree's order: _gather_perm! (src/device_lists.jl:369-372)
+__global__ void k_gather_perm(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ perm,
+                              int64_t n) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) dst[j] = src[perm[j]];
+}
+
+// ---------------------------------------------------------------------------
+// dense M2L by symmetry class: _class_gather! / _class_scatter! (src/device_passes.jl:426-473)
+// Gc[i, q] = sgn[f, i] Mp[idx[f, i], src[q]]  (P_f^T m)
+// ---------------------------------------------------------------------------
+__global__ void k_class_gather(double *__restrict__ Gc, const double *__restrict__ Mp, int Ns,
+                               const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int64_t npk,
+                               const int32_t *__restrict__ idx, const int8_t *__restrict__ sgn) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)Ns * npk) return;
+  int64_t q = g / Ns;
+  int i = (int)(g % Ns);
+  int64_t f = cfar[q];
+  Gc[g] = (double)sgn[f * Ns + i] * Mp[(int64_t)csrc[q] * Ns + idx[f * Ns + i]];
+}
+// target-owned scatter of P_f y over the runs of a class that fall in the piece
+// [a, a + npk): pL[j, rtgt[r]] += alpha sum_p sgn[f, i] Yc[i, p - a], i = inv[f, j]
+__global__ void k_class_scatter(double *__restrict__ pL, const double *__restrict__ Yc, int Ns,
+                                const int32_t *__restrict__ rtgt, const int32_t *__restrict__ rptr, int64_t r0,
+                                int64_t nr, const int32_t *__restrict__ cfar, int64_t a, int64_t npk,
+                                const int32_t *__restrict__ inv, const int8_t *__restrict__ sgn, double alpha) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)Ns * nr) return;
+  int64_t r = r0 + g / Ns;
+  int j = (int)(g % Ns);
+  int64_t p0 = max((int64_t)rptr[r], a), p1 = min((int64_t)rptr[r + 1], a + npk);
+  if (p0 >= p1) return;
+  double acc = 0.0;
+  for (int64_t p = p0; p < p1; p++) {
+    int64_t f = cfar[p];
+    int i = inv[f * Ns + j];
+    acc += (double)sgn[f * Ns + i] * Yc[(p - a) * Ns + i];
+  }
+  pL[(int64_t)rtgt[r] * Ns + j] += alpha * acc;
+}
+
+static void gemm(Context &ctx, int m, int n, int k, double alpha, const double *A, int lda, const double *B, int ldb,
+                 double beta, double *C, int ldc) {
+  if (m == 0 || n == 0) return;
+  FFMM_BLAS(rocblas_dgemm(ctx.blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb,
+                          &beta, C, ldc));
+}
+
+static constexpr int64_t PIECE = 1 << 18;   // pairs per dense M2L GEMM
+
+void allocate_pass_buffers(Plan &plan) {
+  OpCache &o = *plan.ops;
+  const int Ns = o.Ns, Nl = o.Nl;
+  int64_t nq = 1, ng = 1, nlmax = 1;
+  for (int l = 0; l < plan.L; l++) {
+    nq = std::max(nq, plan.s2m_off[l + 1] - plan.s2m_off[l]);
+    nq = std::max(nq, plan.leaf_off[l + 1] - plan.leaf_off[l]);
+    ng = std::max(ng, plan.nl[l] / NCHILD);
+    nlmax = std::max(nlmax, plan.nl[l]);
+  }
+  int64_t nleaves = plan.leaf_off[plan.L];
+  plan.Mp.resize((size_t)Ns * plan.nnodes);
+  plan.Pl.resize((size_t)Ns * plan.nnodes);
+  plan.Lc.resize((size_t)Nl * std::max<int64_t>(nleaves, 1));
+  plan.tau.resize(plan.N);
+  plan.pM.resize((size_t)Nl * nq);
+  plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);
+  plan.Mup.resize((size_t)Ns * nq);
+  plan.tM.resize((size_t)Ns * ng);
+  plan.G.resize((size_t)Ns * std::max(nq, ng));
+  int64_t pmax = 1;
+  for (auto &li : plan.lists) pmax = std::max(pmax, std::min<int64_t>(PIECE, li.np));
+  if (plan.m2l == FFMM_M2L_DENSE) {
+    plan.Gc.resize((size_t)Ns * pmax);
+    plan.Yc.resize((size_t)Ns * pmax);
+  }
+}
+
+void gather_inputs(Plan &plan, const double *tau, int sorted) {
+  Context &ctx = *plan.ctx;
+  if (sorted)
+    FFMM_HIP(hipMemcpyAsync(plan.tau.p, tau, sizeof(double) * plan.N, hipMemcpyDeviceToDevice, ctx.stream));
+  else
+    k_gather_perm<<<nblocks(plan.N, TB), TB, 0, ctx.stream>>>(plan.tau.p, tau, plan.perm.p, plan.N);
+  FFMM_LAUNCH_CHECK();
+}
+
+// upward pass: src/device_passes.jl:523-547
+void upward_pass(Plan &plan) {
+  Context &ctx = *plan.ctx;
+  OpCache &o = *plan.ops;
+  hipStream_t s = ctx.stream;
+  const int Ns = o.Ns, Nl = o.Nl;
+  // every column is written before it is read except by M2M of empty groups: clear
+  k_fill<<<nblocks(plan.Mp.n, TB), TB, 0, s>>>(plan.Mp.p, plan.Mp.n, 0.0);
+  FFMM_LAUNCH_CHECK();
+  for (int l = plan.L - 1; l >= 2; l--) {
+    const int64_t nq = plan.s2m_off[l + 1] - plan.s2m_off[l];
+    const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];
+    if (nq > 0) {
+      k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p, o.large.p,
+                                            Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p, plan.tau.p,
+                                            plan.pM.p);
+      FFMM_LAUNCH_CHECK();
+      // UC2E: M = pinv_scale V diag(invS) (Uh pM)  (src/device_passes.jl:496-502)
+      gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
+      gemm(ctx, Ns, (int)nq, o.r_up, plan.lam[l], o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
+      k_scatter_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq, 1.0);
+      FFMM_LAUNCH_CHECK();
+    }
+    if (l < plan.L - 1) {
+      const int64_t ng = plan.nl[l + 1] / NCHILD;
+      const double *B = plan.Mp.p + plan.lo[l + 1] * Ns;
+      gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, o.M2M.p, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
+      k_scatter_parents<<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM.p, plan.parent.p,
+                                                                     plan.lo[l + 1], Ns, ng, 1.0);
+      FFMM_LAUNCH_CHECK();
+    }
+  }
+}
+
+void m2l_dense_level(Plan &plan, int l) {
+  Context &ctx = *plan.ctx;
+  OpCache &o = *plan.ops;
+  hipStream_t s = ctx.stream;
+  const int Ns = o.Ns;
+  LevelLists &li = plan.lists[l];
+  double *pL = plan.Pl.p + plan.lo[l] * Ns;
+  const double alpha = 1.0 / plan.lam[l];   // m2l_scale of the level: (w_l / w_ref)^(-1)
+  for (int c = 0; c < o.nclass; c++) {
+    const int64_t pa = li.class_pair[c], pb = li.class_pair[c + 1];
+    const int64_t r0 = li.class_run[c], nr = li.class_run[c + 1] - r0;
+    for (int64_t a = pa; a < pb; a += PIECE) {
+      const int64_t npk = std::min(PIECE, pb - a);
+      k_class_gather<<<nblocks((int64_t)Ns * npk, TB), TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, li.csrc.p + a,
+                                                                   li.cfar.p + a, npk, o.idx.p, o.sgn.p);
+      FFMM_LAUNCH_CHECK();
+      gemm(ctx, Ns, (int)npk, Ns, 1.0, o.class_ops.p + (size_t)c * Ns * Ns, Ns, plan.Gc.p, Ns, 0.0, plan.Yc.p, Ns);
+      k_class_scatter<<<nblocks((int64_t)Ns * nr, TB), TB, 0, s>>>(pL, plan.Yc.p, Ns, li.rtgt.p, li.rptr.p, r0, nr,
+                                                                   li.cfar.p, a, npk, o.inv.p, o.sgn.p, alpha);
+      FFMM_LAUNCH_CHECK();
+    }
+  }
+}
+
+// downward pass: src/device_passes.jl:548-595
+void downward_pass(Plan &plan) {
+  Context &ctx = *plan.ctx;
+  OpCache &o = *plan.ops;
+  hipStream_t s = ctx.stream;
+  const int Ns = o.Ns, Nl = o.Nl;
+  // levels 0 and 1 have no far field: their partial locals and leaf densities are zero
+  const int l2 = std::min(2, plan.L);
+  const int64_t n01 = l2 < plan.L ? plan.lo[l2] : plan.nnodes;
+  k_fill<<<nblocks((int64_t)Ns * n01, TB), TB, 0, s>>>(plan.Pl.p, (int64_t)Ns * n01, 0.0);
+  k_fill<<<nblocks((int64_t)Nl * plan.leaf_off[l2], TB), TB, 0, s>>>(plan.Lc.p, (int64_t)Nl * plan.leaf_off[l2], 0.0);
+  FFMM_LAUNCH_CHECK();
+  for (int l = 2; l < plan.L; l++) {
+    const int64_t nl = plan.nl[l], ng = nl / NCHILD;
+    double *pL = plan.Pl.p + plan.lo[l] * Ns;
+    if (l > 2) {
+      k_gather_parents<<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, plan.parent.p, plan.lo[l],
+                                                                    Ns, ng);
+      FFMM_LAUNCH_CHECK();
+      // the stacked children output of L2L is the level's block of partial locals
+      gemm(ctx, NCHILD * Ns, (int)ng, Ns, 1.0, o.L2L.p, NCHILD * Ns, plan.G.p, Ns, 0.0, pL, NCHILD * Ns);
+    } else {
+      k_fill<<<nblocks((int64_t)Ns * nl, TB), TB, 0, s>>>(pL, (int64_t)Ns * nl, 0.0);
+      FFMM_LAUNCH_CHECK();
+    }
+    StageTimer tm(plan, 4, l);
+    if (plan.m2l == FFMM_M2L_SPECTRAL) m2l_spectral_level(plan, l);
+    else m2l_dense_level(plan, l);
+  }
+  // the densities of the real leaves: DC2E (src/device_passes.jl:582-593)
+  for (int l = 2; l < plan.L; l++) {
+    const int64_t nq = plan.leaf_off[l + 1] - plan.leaf_off[l];
+    if (nq == 0) continue;
+    k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, plan.leaf_nodes.p + plan.leaf_off[l],
+                                                               Ns, nq);
+    FFMM_LAUNCH_CHECK();
+    gemm(ctx, o.r_dn, (int)nq, Ns, 1.0, o.Uh_dn.p, o.r_dn, plan.G.p, Ns, 0.0, plan.tup.p, o.r_dn);
+    gemm(ctx, Nl, (int)nq, o.r_dn, plan.lam[l], o.Vs_dn.p, Nl, plan.tup.p, o.r_dn, 0.0,
+         plan.Lc.p + plan.leaf_off[l] * Nl, Nl);
+  }
+}
+
+}  // namespace ffmm

@@ -1,0 +1,10 @@
+"""ffmm: an MI355X (gfx950) backend for the device passes of FlexibleFMM.jl.
+
+The compute path is libffmm_hip.so (hand-written HIP kernels behind the C ABI of
+include/ffmm.h); this package is the host side: the reference's Julia API restated
+in Python over ctypes, and the host FP64 operator precompute.
+"""
+from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
+from .fmm import (FMM, CubeKernelIndependent, DevicePlan, FMMState, Gradient, Laplace3D, OperatorCache,  # noqa: F401
+                  Potential, PotentialGradient, build_expansions, build_expansions_, device_build_expansions_,
+                  device_matvec_, evaluate, evaluate_at_sources, square_bbox)

@@ -1,0 +1,507 @@
+"""The host API, mirroring the reference's Julia interface on top of libffmm_hip.
+
+Reference entry points and their counterparts here (same argument meaning, same
+error phrases; ``!`` becomes a trailing underscore):
+
+=====================================================  ==========================================
+``Laplace3D()``  (src/kernels.jl:179-200)               ``Laplace3D()``
+``CubeKernelIndependent{3}(K; p, pc, d, pinv_rtol)``    ``CubeKernelIndependent(K, p=, pc=, d=)``
+  (src/operators/cubeki.jl:386-392)
+``OperatorCache(method)``, ``fit_box!``                 ``OperatorCache(method)``, ``.fit_box``
+  (src/operators/cache.jl:196-274)
+``FMM(method, sources; ppl, bbox, cache, max_levels)``  ``FMM(method, sources, ppl=, bbox=, ...)``
+  (src/fmm.jl:31-49)
+``build_expansions(f, sources, tau)`` (src/fmm.jl:306)  ``build_expansions(f, sources, tau)``
+``build_expansions!(state, f, sources, tau)`` (:271)    ``build_expansions_(state, f, sources, tau)``
+``evaluate_at_sources(f, st; output)``                  ``evaluate_at_sources(f, st, output=)``
+  (src/evaluate.jl:443)
+``evaluate(f, st, targets; output)`` (:412)             ``evaluate(f, st, targets, output=)``
+``DevicePlan(m, xs; ppl, bbox, cache, max_levels)``     ``DevicePlan(m, xs, ppl=, bbox=, ...)``
+  (src/device_passes.jl:113-134)
+``device_build_expansions!(plan, ds; sorted)`` (:511)   ``device_build_expansions_(plan, ds, sorted=)``
+``device_matvec!(out, plan, ds; output, sorted)`` (:619)``device_matvec_(out, plan, ds, output=, ...)``
+``device_tree`` / ``Tree(dt)`` (src/tree/device_builder.jl:398, 508)  ``DevicePlan.download_tree()``
+=====================================================  ==========================================
+
+Device memory and the hand-off of device pointers go through torch tensors (plumbing
+only); every computation runs in libffmm_hip.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, call
+from . import operators as _ops
+
+
+# ---------------------------------------------------------------------------
+# kernels, outputs, methods
+# ---------------------------------------------------------------------------
+class Laplace3D:
+    """Free-space Laplace Green's function 1/(4 pi |t - s|) (src/kernels.jl:175-200)."""
+    homogeneity = -1
+
+    def __repr__(self):
+        return "Laplace3D()"
+
+
+class Potential:
+    code = _lib.POTENTIAL
+    ncomp = 1
+
+
+class Gradient:
+    code = _lib.GRADIENT
+    ncomp = 3
+
+
+class PotentialGradient:
+    """Potential and gradient from one evaluation (4 values per point: u, du/dx, du/dy, du/dz)."""
+    code = _lib.POTENTIAL_GRADIENT
+    ncomp = 4
+
+
+def _output(o):
+    o = o() if isinstance(o, type) else o
+    if not hasattr(o, "code"):
+        raise ArgumentError(f"output {o!r} is not available")
+    return o
+
+
+class CubeKernelIndependent:
+    """``CubeKernelIndependent{3}(kernel; p=8, pc=2p, d=0.1, pinv_rtol=50 eps)``
+    (src/operators/cubeki.jl:375-392): p points per edge on the small surface, pc on the
+    large one, margin d."""
+
+    def __init__(self, kernel, p: int = 8, pc: int | None = None, d: float = 0.1,
+                 pinv_rtol: float = 50 * np.finfo(np.float64).eps):
+        pc = 2 * p if pc is None else pc
+        if p < 2:
+            raise ArgumentError("p must be at least 2")
+        if pc < p:
+            raise ArgumentError("pc must be at least p")
+        if not (0 < d < 2 / 3):
+            raise ArgumentError("d must lie in (0, 2/3): the large surface, at (3 − 2d)w/2, must lie outside the "
+                                "small one, at (1 + d)w/2")
+        if not isinstance(kernel, Laplace3D):
+            raise ArgumentError("the device passes of this build take Laplace3D")
+        self.kernel, self.p, self.pc, self.d, self.pinv_rtol = kernel, int(p), int(pc), float(d), float(pinv_rtol)
+
+    @property
+    def nsurface(self) -> int:
+        return self.p ** 3 - (self.p - 2) ** 3
+
+    @property
+    def nlarge(self) -> int:
+        return self.pc ** 3 - (self.pc - 2) ** 3
+
+    def __repr__(self):
+        return f"CubeKernelIndependent{{3}}({self.kernel!r}; p={self.p}, pc={self.pc}, d={self.d})"
+
+
+# ---------------------------------------------------------------------------
+# device contexts (one per device, on its own stream)
+# ---------------------------------------------------------------------------
+_CONTEXTS: dict = {}
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU is visible: libffmm_hip needs an MI355X (gfx950)")
+    return torch
+
+
+def context(device: int = 0):
+    h = _CONTEXTS.get(device)
+    if h is None:
+        _torch()
+        h = ctypes.c_void_p()
+        call("ffmm_context_create", device, None, ctypes.byref(h))
+        _CONTEXTS[device] = h
+    return h
+
+
+def _dev_f64(x, device: int, cols: int | None = None):
+    """A contiguous float64 tensor on the device (copied when it is not one already)."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=f"cuda:{device}", dtype=torch.float64).contiguous()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.float64))).to(f"cuda:{device}")
+    if cols is not None:
+        if t.ndim == 1 and cols == 1:
+            pass
+        elif t.ndim != 2 or t.shape[1] != cols:
+            raise DimensionMismatch(f"expected an N × {cols} array, got {tuple(t.shape)}")
+    torch.cuda.synchronize(device)
+    return t
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# operator cache (src/operators/cache.jl)
+# ---------------------------------------------------------------------------
+class OperatorCache:
+    """Operators of one method by width exponent ``w = ldexp(W0, e)``; for the
+    homogeneous Laplace kernel one reference set at ``W0`` serves every width.  The
+    device copies are built once per device and shared by every plan made with this
+    cache (OperatorCache.device, src/device_passes.jl:136-146)."""
+
+    def __init__(self, method: CubeKernelIndependent):
+        self.method = method
+        self.W0 = math.nan
+        self._host = None
+        self._device = {}
+
+    def has_base_width(self) -> bool:
+        return not math.isnan(self.W0)
+
+    def fit_box(self, lo, hi, bbox):
+        """``fit_box!(cache, lo, hi, bbox)`` (src/operators/cache.jl:254-274): a box of
+        width ``ldexp(W0, e)`` containing ``bbox``; the first call fixes ``W0``."""
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        if not (np.all(np.isfinite(lo)) and np.all(np.isfinite(hi))):
+            raise ArgumentError("the points must be finite")
+        origin, W = np.asarray(bbox[0], dtype=np.float64).copy(), float(bbox[1])
+        if not W > 0:
+            raise ArgumentError("box width must be positive")
+        if not (np.all(origin <= lo) and np.all(origin + W >= hi)):
+            raise ArgumentError("points lie outside the requested box")
+        if not self.has_base_width():
+            self.W0 = W
+            return origin, W, 0
+        f, e = math.frexp(W / self.W0)
+        e = e - 1 if f == 0.5 else e
+        cen = origin + W / 2
+        for _ in range(2):
+            Wn = math.ldexp(self.W0, e)
+            on = np.minimum(cen - Wn / 2, lo)
+            if np.all(on + Wn >= hi):
+                return on, Wn, e
+            e += 1
+        raise ArgumentError("could not fit a box")
+
+    def host_operators(self) -> _ops.CubeOperators:
+        if not self.has_base_width():
+            raise ArgumentError("fit a box before requesting operators")
+        if self._host is None:
+            m = self.method
+            self._host = _ops.level_operators(self.W0, p=m.p, pc=m.pc, d=m.d, rtol=m.pinv_rtol)
+        return self._host
+
+    def device(self, device: int = 0):
+        h = self._device.get(device)
+        if h is None:
+            o = self.host_operators()
+            m = self.method
+            h = ctypes.c_void_p()
+            call("ffmm_opcache_create", context(device), m.p, m.pc, m.d, ctypes.byref(h))
+            F = lambda a: np.asfortranarray(a, dtype=np.float64)   # noqa: E731 (column major)
+            keep = dict(small=np.ascontiguousarray(o.small), large=np.ascontiguousarray(o.large),
+                        Uh_up=F(o.P_up.Uh), invS_up=np.ascontiguousarray(o.P_up.invS), V_up=F(o.P_up.V),
+                        Uh_dn=F(o.P_down.Uh), invS_dn=np.ascontiguousarray(o.P_down.invS), V_dn=F(o.P_down.V),
+                        M2M=F(o.M2M), L2L=F(o.L2L),
+                        class_ops=np.ascontiguousarray(np.stack([c.T for c in o.class_ops])),
+                        far_class=np.ascontiguousarray(o.sc.klass, dtype=np.int32),
+                        idx=np.ascontiguousarray(o.sc.idx, dtype=np.int32),
+                        sgn=np.ascontiguousarray(o.sc.sgn, dtype=np.int8),
+                        tables=np.ascontiguousarray(o.tables.astype(np.complex128)))
+            P = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+            call("ffmm_opcache_put_cube", h, o.width, P(keep["small"]), P(keep["large"]), o.P_up.rank,
+                 P(keep["Uh_up"]), P(keep["invS_up"]), P(keep["V_up"]), o.P_down.rank, P(keep["Uh_dn"]),
+                 P(keep["invS_dn"]), P(keep["V_dn"]), P(keep["M2M"]), P(keep["L2L"]), len(o.class_ops),
+                 P(keep["class_ops"]), P(keep["far_class"]), P(keep["idx"]), P(keep["sgn"]), P(keep["tables"]))
+            self._device[device] = h
+        return h
+
+    def __len__(self):
+        return 0 if self._host is None else 1
+
+
+def square_bbox(lo, hi):
+    """The square box centred on the bounding box ``[lo, hi]`` with side its largest
+    extent (1 if all points coincide), widened until it contains it
+    (src/tree/tree.jl:235-252)."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    if not (np.all(np.isfinite(lo)) and np.all(np.isfinite(hi))):
+        raise ArgumentError("the points must be finite")
+    W0 = float(np.max(hi - lo))
+    if not math.isfinite(W0):
+        raise ArgumentError("the points' extent overflows")
+    if not W0 > 0:
+        W0 = 1.0
+    c = lo + (hi - lo) / 2
+    origin = c - W0 / 2
+    for _ in range(64):
+        if np.all(origin <= lo) and np.all(origin + W0 >= hi):
+            return origin, W0
+        W0 *= 1 + 4 * np.finfo(np.float64).eps
+        origin = c - W0 / 2
+    raise ArgumentError("could not fit a box around the points")
+
+
+# ---------------------------------------------------------------------------
+# the device plan
+# ---------------------------------------------------------------------------
+_PREC = {"f64": _lib.PREC_F64, "mixed": _lib.PREC_MIXED}
+_M2L = {"dense": _lib.M2L_DENSE, "spectral": _lib.M2L_SPECTRAL}
+
+
+@dataclass
+class TreeArrays:
+    """The flat tree downloaded from the device (0-based, -1 = none): the fields of the
+    reference's ``Tree`` (src/tree/tree.jl:178-209) with ``points`` as ``plo``/``phi``."""
+    origin: np.ndarray
+    W0: float
+    level_start: np.ndarray
+    cell: np.ndarray
+    level: np.ndarray
+    parent: np.ndarray
+    child_first: np.ndarray
+    plo: np.ndarray
+    phi: np.ndarray
+    leaf: np.ndarray
+    ghost: np.ndarray
+    xlist: np.ndarray
+    has_source: np.ndarray
+    colleagues: np.ndarray
+    perm: np.ndarray
+    point_node: np.ndarray
+    points: np.ndarray
+
+    @property
+    def nlevels(self):
+        return len(self.level_start) - 1
+
+
+class DevicePlan:
+    """``DevicePlan(method, xs; ppl=50, bbox, cache, max_levels=30)``: the device tree,
+    its lists, the operators' device copies and the work buffers
+    (src/device_passes.jl:98-134).  ``precision`` is ``"f64"`` or ``"mixed"``; ``m2l`` is
+    ``"dense"`` (class GEMMs) or ``"spectral"`` (batched FFT)."""
+
+    def __init__(self, method: CubeKernelIndependent, xs, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
+                 max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
+        cache = OperatorCache(method) if cache is None else cache
+        if cache.method is not method:
+            raise ArgumentError("the cache was built for a different method")
+        if precision not in _PREC:
+            raise ArgumentError(f"unknown precision {precision!r}")
+        if m2l not in _M2L:
+            raise ArgumentError(f"unknown M2L form {m2l!r}")
+        self.method, self.cache, self.device = method, cache, device
+        self.precision, self.m2l = precision, m2l
+        xs = _dev_f64(xs, device, cols=3)
+        n = xs.shape[0]
+        if n == 0:
+            raise ArgumentError("no points")
+        ctx = context(device)
+        lo, hi, fin = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), ctypes.c_int()
+        call("ffmm_bounds", ctx, _ptr(xs), n, lo, hi, ctypes.byref(fin))
+        if not fin.value:
+            raise ArgumentError("the points must be finite")
+        lo, hi = np.array(lo[:]), np.array(hi[:])
+        if bbox is None:
+            bbox = square_bbox(lo, hi)
+        origin, W, e0 = cache.fit_box(lo, hi, bbox)
+        self.origin, self.W0, self.e0 = np.asarray(origin, dtype=np.float64), float(W), int(e0)
+        oc = cache.device(device)
+        h = ctypes.c_void_p()
+        org = (ctypes.c_double * 3)(*self.origin)
+        call("ffmm_plan_create", oc, _ptr(xs), n, int(ppl), org, self.W0, self.e0, int(max_levels), _PREC[precision],
+             _M2L[m2l], ctypes.byref(h))
+        self._h = h
+        self.N = n
+        self.ppl = int(ppl)
+        self._read_sizes()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().ffmm_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _read_sizes(self):
+        nl, nn, np_ = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        call("ffmm_plan_sizes", self._h, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(np_), None, None, None, None,
+             None)
+        L = nl.value
+        arrs = [np.zeros(L, dtype=np.int64) for _ in range(5)]
+        P = [a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
+        call("ffmm_plan_sizes", self._h, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(np_), *P)
+        self.nlevels, self.nnodes = L, nn.value
+        self.level_start, self.level_count, self.level_pairs, self.level_s2m, self.level_leaves = arrs
+
+    # -- downloads (tests) --
+    def download_tree(self) -> TreeArrays:
+        n, N = self.nnodes, self.N
+        a = dict(cell=np.empty((n, 3), np.int32), level=np.empty(n, np.int32), parent=np.empty(n, np.int32),
+                 child_first=np.empty(n, np.int32), plo=np.empty(n, np.int32), phi=np.empty(n, np.int32),
+                 leaf=np.empty(n, np.uint8), ghost=np.empty(n, np.uint8), xlist=np.empty(n, np.uint8),
+                 has_source=np.empty(n, np.uint8), colleagues=np.empty((n, 27), np.int32),
+                 perm=np.empty(N, np.int32), point_node=np.empty(N, np.int32), points=np.empty((N, 3), np.float64))
+        order = ["cell", "level", "parent", "child_first", "plo", "phi", "leaf", "ghost", "xlist", "has_source",
+                 "colleagues", "perm", "point_node", "points"]
+        call("ffmm_plan_download_tree", self._h, *[a[k].ctypes.data_as(ctypes.c_void_p) for k in order])
+        ls = np.concatenate([self.level_start, [self.nnodes]]).astype(np.int32)
+        return TreeArrays(origin=self.origin.copy(), W0=self.W0, level_start=ls, **a)
+
+    def download_lists(self, level: int):
+        nl = int(self.level_count[level])
+        npairs = int(self.level_pairs[level])
+        ptr, src, far = np.empty(nl + 1, np.int32), np.empty(npairs, np.int32), np.empty(npairs, np.int32)
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        call("ffmm_plan_download_lists", self._h, level, P(ptr), P(src), P(far))
+        return ptr, src, far
+
+    def download_expansions(self):
+        o = self.cache.host_operators()
+        Ns, Nl = o.nsmall, o.nlarge
+        nleaves = int(self.level_leaves.sum())
+        M = np.empty((self.nnodes, Ns))
+        Pl = np.empty((self.nnodes, Ns))
+        Lc = np.empty((nleaves, Nl))
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        call("ffmm_plan_download_expansions", self._h, P(M), P(Pl), P(Lc))
+        return M.T, Pl.T, Lc.T
+
+    def stage_times(self) -> dict:
+        ms = (ctypes.c_double * 8)()
+        call("ffmm_plan_stage_times", self._h, ms, 8)
+        names = ["tree", "lists", "upward", "downward", "m2l", "eval_sources", "eval_targets", "gather"]
+        return dict(zip(names, ms[:]))
+
+    # -- the passes (device tensors in and out) --
+    def build_expansions(self, tau, sorted: bool = False):
+        t = _dev_f64(tau, self.device)
+        if t.ndim != 1 or t.shape[0] != self.N:
+            raise DimensionMismatch(f"expected {self.N} entries in group 1")
+        call("ffmm_build_expansions", self._h, _ptr(t), int(bool(sorted)))
+        return self
+
+    def evaluate_at_sources(self, output=Potential, out=None):
+        o = _output(output)
+        torch = _torch()
+        shape = (self.N,) if o.ncomp == 1 else (self.N, o.ncomp)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        elif tuple(out.shape) != shape:
+            raise DimensionMismatch(f"expected {self.N} outputs for group 1")
+        call("ffmm_evaluate_at_sources", self._h, _ptr(out), o.code)
+        return out
+
+    def evaluate(self, targets, output=Potential, out=None):
+        o = _output(output)
+        torch = _torch()
+        t = _dev_f64(targets, self.device, cols=3)
+        nt = t.shape[0]
+        shape = (nt,) if o.ncomp == 1 else (nt, o.ncomp)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        elif tuple(out.shape) != shape:
+            raise DimensionMismatch("out and targets differ in length")
+        call("ffmm_evaluate", self._h, _ptr(t), nt, _ptr(out), o.code)
+        return out
+
+    def matvec(self, tau, output=Potential, sorted: bool = False, out=None):
+        self.build_expansions(tau, sorted)
+        return self.evaluate_at_sources(output, out)
+
+
+def device_build_expansions_(plan: DevicePlan, ds, sorted: bool = False) -> DevicePlan:
+    """``device_build_expansions!(plan, ds; sorted)`` (src/device_passes.jl:511)."""
+    return plan.build_expansions(ds["tau"] if isinstance(ds, dict) else ds[1], sorted)
+
+
+def device_matvec_(out, plan: DevicePlan, ds, output=Potential, sorted: bool = False):
+    """``device_matvec!(out, plan, ds; output, sorted)`` (src/device_passes.jl:619):
+    ``out`` (a device tensor) receives the output at every source in the original order."""
+    tau = ds["tau"] if isinstance(ds, dict) else ds[1]
+    return plan.matvec(tau, output, sorted, out)
+
+
+# ---------------------------------------------------------------------------
+# the FMM interface of the host driver (src/fmm.jl, src/evaluate.jl)
+# ---------------------------------------------------------------------------
+class FMM:
+    """``FMM(method, sources; ppl=50, bbox=None, cache=OperatorCache(method), max_levels=30)``
+    (src/fmm.jl:24-49): here the tree, its lists and the operators live on the GPU (a
+    ``DevicePlan``)."""
+
+    def __init__(self, method, sources, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
+                 max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
+        if isinstance(sources, tuple):
+            if len(sources) != 1:
+                raise ArgumentError("this build takes one group of sources")
+            sources = sources[0]
+        self.method = method
+        self.plan = DevicePlan(method, sources, ppl=ppl, bbox=bbox, cache=cache, max_levels=max_levels,
+                               precision=precision, m2l=m2l, device=device)
+        self.cache = self.plan.cache
+
+    @property
+    def nlevels(self):
+        return self.plan.nlevels
+
+    def __repr__(self):
+        return (f"FMM(CubeKernelIndependent, Tree{{3,Float64}}: {self.plan.N} points, {self.plan.nnodes} nodes on "
+                f"{self.plan.nlevels} levels)")
+
+
+class FMMState:
+    """The expansions of one right-hand side (src/fmm.jl:58-94); they live in the plan."""
+
+    def __init__(self, f: FMM):
+        self.fmm = f
+        self.tau = None
+
+
+def build_expansions_(state: FMMState, f: FMM, sources, tau) -> FMMState:
+    """``build_expansions!(state, fmm, sources, tau)`` (src/fmm.jl:264-298)."""
+    if state.fmm is not f:
+        raise ArgumentError("the state belongs to another FMM")
+    n = f.plan.N
+    if sources is not None and len(sources) != n:
+        raise DimensionMismatch(f"expected {n} sources in group 1")
+    tau = np.asarray(tau) if not hasattr(tau, "data_ptr") else tau
+    if len(tau) != n:
+        raise DimensionMismatch(f"expected {n} densities in group 1")
+    f.plan.build_expansions(tau)
+    state.tau = tau
+    return state
+
+
+def build_expansions(f: FMM, sources, tau) -> FMMState:
+    """``build_expansions(fmm, sources, tau)`` (src/fmm.jl:300-314)."""
+    return build_expansions_(FMMState(f), f, sources, tau)
+
+
+def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -> np.ndarray:
+    """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:418-446): the
+    output at every source in the original order."""
+    if g != 1:
+        raise ArgumentError(f"group {g} of 1")
+    if state.fmm is not f:
+        raise ArgumentError("the state belongs to another FMM")
+    return f.plan.evaluate_at_sources(output).cpu().numpy()
+
+
+def evaluate(f: FMM, state: FMMState, targets, output=Potential) -> np.ndarray:
+    """``evaluate(fmm, state, targets; output)`` (src/evaluate.jl:384-416)."""
+    if state.fmm is not f:
+        raise ArgumentError("the state belongs to another FMM")
+    return f.plan.evaluate(targets, output).cpu().numpy()

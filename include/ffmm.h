@@ -1,0 +1,150 @@
+/*
+ * ffmm.h -- the C ABI of libffmm_hip.so, an MI355X (gfx950) backend for the device
+ * passes of the FlexibleFMM.jl fast multipole method (Laplace3D on cube
+ * kernel-independent surfaces).
+ *
+ * Every entry point returns 0 on success or one of the FFMM_ERR_* codes; the message
+ * of the last failure on the calling thread is ffmm_last_error().  Handles are
+ * opaque.  Device pointers are plain device addresses on the context's device; the
+ * library reads and writes them on the context's stream and synchronises that stream
+ * before returning (as the reference synchronises at the end of each pass and
+ * evaluation).  Matrices are column major.  Indices are 0-based (-1 = none).
+ *
+ * One host thread drives a plan at a time (as in the reference: nothing makes
+ * concurrent calls on one plan safe); distinct plans may be driven from distinct
+ * threads.
+ */
+#ifndef FFMM_H
+#define FFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes; the first two mirror the reference's ArgumentError and DimensionMismatch */
+#define FFMM_OK 0
+#define FFMM_ERR_ARGUMENT 1
+#define FFMM_ERR_DIMENSION 2
+#define FFMM_ERR_DEVICE 3
+#define FFMM_ERR_STATE 4
+#define FFMM_ERR_MEMORY 5
+
+/* arithmetic of the passes */
+#define FFMM_PREC_F64 0    /* everything in FP64 */
+#define FFMM_PREC_MIXED 1  /* FP32 S2M, M2L, L2T and near field; FP64 pseudo-inverses, M2M, L2L */
+
+/* M2L form */
+#define FFMM_M2L_DENSE 0     /* symmetry-class GEMMs (the reference device plan's form) */
+#define FFMM_M2L_SPECTRAL 1  /* batched FFT + per-pair Hadamard products (the reference CPU form) */
+
+/* outputs */
+#define FFMM_POTENTIAL 0
+#define FFMM_GRADIENT 1
+
+typedef struct ffmm_context_s *ffmm_context;
+typedef struct ffmm_opcache_s *ffmm_opcache;
+typedef struct ffmm_plan_s *ffmm_plan;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *ffmm_last_error(void);
+/* ABI version (increments on any signature change). */
+int ffmm_abi_version(void);
+
+/* A context on one device (hipSetDevice numbering); its own stream when stream == NULL,
+ * else the caller's hipStream_t.  No reference counterpart: the reference's "backend"
+ * (KernelAbstractions get_backend, src/device.jl:4-7) plays this role. */
+int ffmm_context_create(int device, void *stream, ffmm_context *out);
+int ffmm_context_destroy(ffmm_context ctx);
+
+/* Componentwise bounds of n device points (xyz, 3 x n AoS, FP64), NaN propagating, and
+ * whether every coordinate is finite: _device_bounds (src/tree/device_builder.jl:124-148). */
+int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite);
+
+/* The device copies of the operators of one method, shared by every plan built with
+ * the cache: OperatorCache.device (src/operators/cache.jl:196-213,
+ * src/device_passes.jl:136-146).  The method is CubeKernelIndependent{3}(Laplace3D(); p, pc, d)
+ * (src/operators/cubeki.jl:386-392). */
+int ffmm_opcache_create(ffmm_context ctx, int p, int pc, double d, ffmm_opcache *out);
+int ffmm_opcache_destroy(ffmm_opcache cache);
+
+/* Upload the reference-width operators (host FP64, column major), built by
+ * level_operators(method, w_ref) (src/operators/cubeki.jl:404-430) and
+ * class_operators (src/operators/symmetry.jl:125-131):
+ *   small (3 x Ns), large (3 x Nl) surfaces relative to the box centre;
+ *   P_up = V_up diag(invS_up) Uh_up (Uh_up: r_up x Nl, V_up: Ns x r_up);
+ *   P_down = V_dn diag(invS_dn) Uh_dn (Uh_dn: r_dn x Ns, V_dn: Nl x r_dn);
+ *   M2M (Ns x 8Ns), L2L (8Ns x Ns);
+ *   nclass representatives (Ns x Ns each, consecutive), the class of each of the 316
+ *   far offsets, and per offset the signed permutation idx (Ns x 316, 0-based), sgn (Ns x 316)
+ *   with M2L_f m = P_f A_rep (P_f^T m), (P^T m)[i] = sgn[i] m[idx[i]];
+ *   tables: the spectral kernel tables, 316 x (2p)(2p)(p+1) complex (interleaved re, im),
+ *   or NULL to disable the spectral M2L. */
+int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small, const double *large,
+                          int r_up, const double *Uh_up, const double *invS_up, const double *V_up,
+                          int r_dn, const double *Uh_dn, const double *invS_dn, const double *V_dn,
+                          const double *M2M, const double *L2L, int nclass, const double *class_ops,
+                          const int32_t *far_class, const int32_t *idx, const int8_t *sgn, const double *tables);
+
+/* The plan from device points xyz (3 x n AoS, FP64, any order) in the box (origin, width):
+ * DevicePlan(m, xs; ppl, bbox, cache, max_levels) (src/device_passes.jl:113-134) with the
+ * device tree device_tree (src/tree/device_builder.jl:398-474), its lists
+ * (_device_level_lists, src/device_lists.jl:425-484) and the buffers of _device_plan
+ * (src/device_passes.jl:155-338).  The box must already be snapped to a width
+ * width = w_ref * 2^e0 of the cache (fit_box!, src/operators/cache.jl:254-274).
+ * precision: FFMM_PREC_*; m2l: FFMM_M2L_*. */
+int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, const double origin[3],
+                     double width, int e0, int max_levels, int precision, int m2l, ffmm_plan *out);
+int ffmm_plan_destroy(ffmm_plan plan);
+
+/* Tree and list sizes: nlevels, nnodes, npoints, and per level (arrays of length
+ * nlevels, any may be NULL) the first node, the node count, the far pairs, the childless
+ * sourced nodes and the real leaves. */
+int ffmm_plan_sizes(ffmm_plan plan, int *nlevels, int64_t *nnodes, int64_t *npoints, int64_t *level_start,
+                    int64_t *level_count, int64_t *level_pairs, int64_t *level_s2m, int64_t *level_leaves);
+
+/* Download the flat tree (host buffers sized by ffmm_plan_sizes; any may be NULL):
+ * Tree(dt::DeviceTree) (src/tree/device_builder.jl:508-525).  cell: 3 x nnodes AoS;
+ * colleagues: 27 x nnodes (27 per node); flags as 0/1 bytes; sorted_points: 3 x N AoS. */
+int ffmm_plan_download_tree(ffmm_plan plan, int32_t *cell, int32_t *level, int32_t *parent, int32_t *child_first,
+                            int32_t *plo, int32_t *phi, uint8_t *leaf, uint8_t *ghost, uint8_t *xlist,
+                            uint8_t *has_source, int32_t *colleagues, int32_t *perm, int32_t *point_node,
+                            double *sorted_points);
+
+/* Download one level's far pairs, targets in node order: ptr (count + 1, relative to the
+ * level's first node), src (global node), far (0-based far offset index), as
+ * _pair_fill! writes them (src/device_lists.jl:286-312; src/tree/lists.jl:96-124). */
+int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *src, int32_t *far);
+
+/* The upward and downward passes from densities tau (N, FP64, device): in the user's
+ * order unless sorted != 0 (then in the tree's order):
+ * device_build_expansions!(plan, ds; sorted) (src/device_passes.jl:511-597). */
+int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted);
+
+/* Output at every source, in the user's order, from all other sources:
+ * evaluate_at_sources! (src/device.jl:153-163).  out: N (potential) or 3 x N AoS (gradient). */
+int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output);
+
+/* Output at nt device targets (3 x nt AoS), located on the device: evaluate!
+ * (src/device.jl:134-145) with the host locate (src/tree/locate.jl:10-66) replaced by a
+ * device walk.  Targets must lie in the root box (FFMM_ERR_ARGUMENT otherwise). */
+int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out, int output);
+
+/* build_expansions then evaluate_at_sources: device_matvec! (src/device_passes.jl:619-640). */
+int ffmm_matvec(ffmm_plan plan, const double *tau, int sorted, double *out, int output);
+
+/* Download expansions (host, column major, any may be NULL): multipoles (Ns x nnodes),
+ * partial locals (Ns x nnodes) and the locals of the real leaves (Nl x nleaves, in the
+ * order of the leaves level by level). */
+int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *plocals, double *leaf_locals);
+
+/* Per-stage device times of the last calls, in milliseconds (hipEvent pairs):
+ * [0] tree, [1] lists, [2] upward, [3] downward, [4] M2L (within downward),
+ * [5] evaluation at sources, [6] evaluation at targets, [7] input gather. */
+int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""The device passes and evaluation against the oracle (the restated host FMM on the
+same tree and operators) and against the FP64 direct sum.  Tolerances follow the
+reference's device-vs-CPU checks (test/test_device_passes.jl:141-166: matvec 1e-11,
+locals 100x that; vs direct 1e-6 at p = 8, test/test_3d.jl:155-163)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pointsets import mixed, uniform, sphere
+
+pytestmark = pytest.mark.gpu
+
+BOX = (np.zeros(3), 1.0)
+
+
+def mrd(a, b):
+    """max-norm relative difference (test/testutils.jl:9)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.ndim == 2:
+        return np.max(np.linalg.norm(a - b, axis=1)) / np.max(np.linalg.norm(b, axis=1))
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import ffmm
+    rng = np.random.default_rng(45)
+    N = 6000
+    pts = mixed(rng, N)
+    tau = rng.standard_normal(N)
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    plan = ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX, cache=cache)
+    ops = cache.host_operators()
+    ref = O.FMM(ops, pts, 30, BOX[0], BOX[1])
+    u_ref = ref.matvec(tau)
+    return dict(ffmm=ffmm, pts=pts, tau=tau, plan=plan, ref=ref, u_ref=u_ref, rng=rng, m=m, cache=cache)
+
+
+def test_matvec_f64_dense_vs_oracle_and_direct(setup):
+    s = setup
+    print(f"[fmm] f64 dense N={len(s['pts'])} levels={s['plan'].nlevels}", flush=True)
+    assert s["plan"].nlevels >= 5
+    u = s["plan"].matvec(s["tau"]).cpu().numpy()
+    assert mrd(u, s["u_ref"]) < 1e-11
+    rows = s["rng"].choice(len(u), 400, replace=False)
+    ud = O.direct_self(s["pts"], s["tau"], rows=rows)
+    assert mrd(u[rows], ud) < 1e-6
+
+
+def test_expansions_vs_oracle(setup):
+    s = setup
+    plan, ref = s["plan"], s["ref"]
+    plan.build_expansions(s["tau"])
+    M, Pl, Lc = plan.download_expansions()
+    assert mrd(M.ravel(), ref.multipoles.ravel()) < 1e-11
+    assert mrd(Pl.ravel(), ref.plocals.ravel()) < 1e-11
+    leaves = np.concatenate(ref.leaf_nodes)
+    assert mrd(Lc.ravel(), ref.locals[:, leaves].ravel()) < 1e-9
+    nonleaf = np.ones(ref.tree.nnodes, bool)
+    nonleaf[leaves] = False
+    assert np.all(ref.locals[:, nonleaf] == 0)
+
+
+@pytest.mark.parametrize("out", ["potential", "gradient", "both"])
+def test_targets_vs_oracle_and_direct(setup, out):
+    s = setup
+    ffmm = s["ffmm"]
+    plan, ref = s["plan"], s["ref"]
+    targets = uniform(np.random.default_rng(3), 3000)
+    plan.build_expansions(s["tau"])
+    output = dict(potential=ffmm.Potential, gradient=ffmm.Gradient, both=ffmm.PotentialGradient)[out]
+    v = plan.evaluate(targets, output).cpu().numpy()
+    tau_sorted = s["tau"][ref.tree.perm]
+    ref.matvec(s["tau"])
+    if out in ("potential", "both"):
+        vr = ref.evaluate(ref.locals, tau_sorted, targets, grad=False)
+        vp = v if out == "potential" else v[:, 0]
+        assert mrd(vp, vr) < 1e-11
+        assert mrd(vp, O.direct(s["pts"], s["tau"], targets)) < 1e-6
+    if out in ("gradient", "both"):
+        gr = ref.evaluate(ref.locals, tau_sorted, targets, grad=True)
+        vg = v if out == "gradient" else v[:, 1:]
+        assert mrd(vg, gr) < 1e-11
+        assert mrd(vg, O.direct(s["pts"], s["tau"], targets, grad=True)) < 1e-5
+
+
+def test_gradient_at_sources(setup):
+    s = setup
+    ffmm = s["ffmm"]
+    g = s["plan"].matvec(s["tau"], ffmm.Gradient).cpu().numpy()
+    gr = s["ref"].evaluate_at_sources(s["ref"].locals, s["tau"][s["ref"].tree.perm], grad=True)
+    assert mrd(g, gr) < 1e-11
+
+
+def test_determinism_sorted_inputs_linearity_nan_recovery(setup):
+    s = setup
+    plan, tau = s["plan"], s["tau"]
+    u0 = plan.matvec(tau).cpu().numpy()
+    assert np.array_equal(plan.matvec(tau).cpu().numpy(), u0)              # bitwise repeatable
+    perm = plan.download_tree().perm
+    assert np.array_equal(plan.matvec(tau[perm], sorted=True).cpu().numpy(), u0)
+    t2 = np.random.default_rng(9).standard_normal(len(tau))
+    u2 = plan.matvec(t2).cpu().numpy()
+    u12 = plan.matvec(tau + 2 * t2).cpu().numpy()
+    assert mrd(u12, u0 + 2 * u2) < 1e-12
+    bad = tau.copy()
+    bad[5] = np.nan
+    assert np.any(np.isnan(plan.matvec(bad).cpu().numpy()))
+    assert np.array_equal(plan.matvec(tau).cpu().numpy(), u0)              # clean again, bitwise
+
+
+def test_host_api_mirror(setup):
+    """FMM / build_expansions / evaluate_at_sources / evaluate, as the reference's
+    host driver is called (src/fmm.jl:31-49, 300-314; src/evaluate.jl:412-446)."""
+    s = setup
+    ffmm = s["ffmm"]
+    f = ffmm.FMM(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"])
+    st = ffmm.build_expansions(f, s["pts"], s["tau"])
+    u = ffmm.evaluate_at_sources(f, st)
+    assert mrd(u, s["u_ref"]) < 1e-11
+    targets = uniform(np.random.default_rng(4), 500)
+    v = ffmm.evaluate(f, st, targets)
+    assert mrd(v, O.direct(s["pts"], s["tau"], targets)) < 1e-6
+    with pytest.raises(ffmm.DimensionMismatch):
+        ffmm.build_expansions(f, s["pts"], s["tau"][:-1])
+    with pytest.raises(ffmm.ArgumentError, match="outside"):
+        ffmm.evaluate(f, st, targets + 2.0)
+
+
+def test_analytic_charged_sphere():
+    """Unit surface charge on a sphere of radius a: a^2/r outside, a inside
+    (test/test_3d.jl:240-251)."""
+    import ffmm
+    c, a = np.array([0.5, 0.5, 0.5]), 0.3
+    nth, nph = 40, 80
+    x, wg = np.polynomial.legendre.leggauss(nth)
+    pts, wts = [], []
+    for i in range(nth):
+        st = np.sqrt(1 - x[i] ** 2)
+        for j in range(nph):
+            ph = 2 * np.pi * j / nph
+            nrm = np.array([st * np.cos(ph), st * np.sin(ph), x[i]])
+            pts.append(c + a * nrm)
+            wts.append(a * a * wg[i] * 2 * np.pi / nph)
+    pts, wts = np.array(pts), np.array(wts)
+    assert abs(wts.sum() - 4 * np.pi * a * a) < 1e-12
+    rng = np.random.default_rng(11)
+    cand = rng.random((3000, 3))
+    ext = cand[np.linalg.norm(cand - c, axis=1) > 0.45][:300]
+    cand2 = c + 0.4 * (rng.random((3000, 3)) - 0.5)
+    inn = cand2[np.linalg.norm(cand2 - c, axis=1) < 0.2][:300]
+    f = ffmm.FMM(ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8), pts, ppl=40, bbox=BOX)
+    assert f.nlevels >= 4
+    st = ffmm.build_expansions(f, pts, wts)
+    ue = ffmm.evaluate(f, st, ext)
+    ui = ffmm.evaluate(f, st, inn)
+    assert mrd(ue, a * a / np.linalg.norm(ext - c, axis=1)) < 1e-6
+    assert mrd(ui, np.full(len(inn), a)) < 1e-6
