@@ -1,0 +1,198 @@
+#!/usr/bin/env python
+"""Laplace3D cube kernel-independent FMM (p = 8) on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 4,000,000 sources
+from a mixture of 32 isotropic Gaussians (centres uniform in [0.1, 0.9]^3, sigma
+log-uniform in [0.005, 0.05], clipped to the unit cube), charges N(0, 1), ppl = 64,
+box ((0,0,0), 1); 1,000,000 uniform targets.  One step is a whole solve from
+device-resident points: the device tree and its lists (the plan), the upward and
+downward passes, the potential at every source, and the potential and gradient at
+every target.  value = sources / second of one step.
+
+After the timed region: the relative L2 error against the FP64 direct sum on 10,000
+sampled points (5,000 sources, 5,000 targets), the roofline of the dominant kernel
+from its live HIP-event duration, and the CPU baseline: the oracle (the restated
+reference CPU algorithm) timed on the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "flexiblefmm.jl_amd"))
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+FLOPS_POT = 11               # flop conventions of the workload: Laplace potential
+FLOPS_POTGRAD = 20           # potential + gradient
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def interaction_counts(tree, nleaf_targets=None):
+    """P2P source counts per real leaf: sum over its colleagues of their point counts."""
+    n = tree.phi - tree.plo + 1
+    col = tree.colleagues
+    cnt = np.where(col >= 0, n[np.maximum(col, 0)], 0).sum(axis=1)
+    leaves = np.nonzero(tree.leaf)[0]
+    return leaves, cnt[leaves], n[leaves]
+
+
+def cpu_baseline(n_sample: int, ppl: int):
+    """The oracle's restated reference CPU FMM on a Gaussian-mixture sample of the same
+    generator: tree + passes + potential at the sources, points per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from ffmm import operators, workloads as W
+    pts = W.gaussian_mixture(n_sample, seed=0)
+    tau = W.charges(n_sample, seed=1)
+    ops = operators.level_operators(1.0, p=8, with_tables=False)
+    t0 = time.perf_counter()
+    f = O.FMM(ops, pts, ppl, np.zeros(3), 1.0)
+    f.matvec(tau)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n_sample / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"{n_sample} Gaussian-mixture sources (same generator), ppl={ppl}, p=8, tree + passes + "
+                      f"potential at the sources, FP64, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=4_000_000)
+    ap.add_argument("--nt", type=int, default=1_000_000)
+    ap.add_argument("--ppl", type=int, default=64)
+    ap.add_argument("--precision", default="f64", choices=["f64", "mixed"])
+    ap.add_argument("--m2l", default="dense", choices=["dense", "spectral"])
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nerr", type=int, default=10_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus N with N > 1 is not supported "
+                                   "(the multi-GPU sharding is a design only; see the design notes)"}))
+        sys.exit(2)
+
+    import torch
+    import ffmm
+    from ffmm import workloads as W
+
+    log(f"[bench] workload: {args.n} Gaussian-mixture sources, {args.nt} uniform targets, ppl={args.ppl}, p=8, "
+        f"{args.precision}, M2L {args.m2l}")
+    pts = W.gaussian_mixture(args.n, seed=0)
+    tau = W.charges(args.n, seed=1)
+    tgt = W.uniform(args.nt, seed=2)
+    dev = "cuda:0"
+    xs_d = torch.from_numpy(pts).to(dev)
+    tau_d = torch.from_numpy(tau).to(dev)
+    tgt_d = torch.from_numpy(tgt).to(dev)
+    u_d = torch.empty(args.n, dtype=torch.float64, device=dev)
+    v_d = torch.empty((args.nt, 4), dtype=torch.float64, device=dev)
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    box = (np.zeros(3), 1.0)
+    torch.cuda.synchronize()
+
+    def step():
+        plan = ffmm.DevicePlan(m, xs_d, ppl=args.ppl, bbox=box, cache=cache, precision=args.precision, m2l=args.m2l)
+        plan.build_expansions(tau_d)
+        plan.evaluate_at_sources(ffmm.Potential, out=u_d)
+        plan.evaluate(tgt_d, ffmm.PotentialGradient, out=v_d)
+        return plan
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        plan = step()
+        log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s")
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        plan = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {k}: {times[-1]:.3f} s")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = 1e3 * total / args.steps
+    stages = plan.stage_times()
+    log("[bench] stages (ms, last step): " + ", ".join(f"{k} {v:.2f}" for k, v in stages.items()))
+    log(f"[bench] tree: {plan.nlevels} levels, {plan.nnodes} nodes, {int(plan.level_pairs.sum())} far pairs, "
+        f"{int(plan.level_leaves.sum())} leaves")
+    # matvec alone (plan reused): expansions + potential at the sources
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        plan.build_expansions(tau_d)
+        plan.evaluate_at_sources(ffmm.Potential, out=u_d)
+    torch.cuda.synchronize()
+    matvec_ms = 1e3 * (time.perf_counter() - t) / args.steps
+
+    # accuracy on sampled points against the FP64 direct sum
+    rng = np.random.default_rng(123)
+    ns_err = args.nerr // 2
+    rows = np.sort(rng.choice(args.n, ns_err, replace=False))
+    trows = np.sort(rng.choice(args.nt, args.nerr - ns_err, replace=False))
+    u = u_d.cpu().numpy()
+    v = v_d.cpu().numpy()
+    ud = ffmm.direct_self(ffmm.Laplace3D(), pts, tau, rows=rows)
+    vd = ffmm.direct(ffmm.Laplace3D(), pts, tau, tgt[trows], ffmm.PotentialGradient)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))   # noqa: E731
+    err = {"rel_l2_potential_sources": rel(u[rows], ud),
+           "rel_l2_potential_targets": rel(v[trows, 0], vd[:, 0]),
+           "rel_l2_gradient_targets": rel(v[trows, 1:], vd[:, 1:]),
+           "max_rel_potential_sources": float(np.max(np.abs(u[rows] - ud)) / np.max(np.abs(ud)))}
+    log("[bench] errors: " + json.dumps(err))
+
+    # roofline of the dominant kernel: the evaluation at the sources (one launch of
+    # k_eval per call, VALU-bound): algorithmic flops = L2T + P2P interactions x 11
+    tree = plan.download_tree()
+    leaves, near, npts = interaction_counts(tree)
+    nl = m.nlarge
+    p2p = float(np.sum(npts * near) - args.n)            # self excluded
+    l2t = float(args.n) * nl
+    flops_eval = (p2p + l2t) * FLOPS_POT
+    t_eval = stages["eval_sources"] * 1e-3
+    achieved = flops_eval / t_eval / 1e12
+    roofline = {"bound": "valu-fp64", "kernel": "k_eval<potential> (evaluation at the sources)",
+                "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": None, "algorithmic_flops": flops_eval, "launch_ms": stages["eval_sources"]}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: oracle on {args.cpu_sample} points")
+        cpu = cpu_baseline(args.cpu_sample, args.ppl)
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} points/s")
+
+    line = {"metric": "Laplace3D 4M-pt FMM points/s at 1e-6 rel L2; served p50/p99 solve latency",
+            "value": args.n / (ms * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if args.precision == "f64" else "f32+f64",
+            "data": "synthetic (seeded Gaussian mixture / uniform)",
+            "config": {"workload": f"Laplace3D CubeKI p=8, {args.n} Gaussian-mixture sources + {args.nt} uniform "
+                                   f"targets, potential at sources + potential and gradient at targets, ppl={args.ppl}, "
+                                   f"{args.precision}, M2L {args.m2l}",
+                       "nsources": args.n, "ntargets": args.nt, "ppl": args.ppl, "p": 8},
+            "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
+            "tree": {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
+                     "leaves": int(plan.level_leaves.sum())},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

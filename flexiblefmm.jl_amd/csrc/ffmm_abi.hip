@@ -54,9 +54,16 @@ StageTimer::StageTimer(Plan &p, int s, int l) : plan(p), stage(s), level(l) {
   FFMM_HIP(hipEventRecord(stage_event(plan, stage, level, 0), plan.ctx->stream));
 }
 StageTimer::~StageTimer() {
-  (void)hipEventRecord(stage_event(plan, stage, level, 1), plan.ctx->stream);
-  if (stage == 4) plan.m2l_levels_timed = std::max(plan.m2l_levels_timed, level + 1);
-  else plan.stage_recorded[stage] = true;
+  if (hipEventRecord(stage_event(plan, stage, level, 1), plan.ctx->stream) != hipSuccess) {
+    (void)hipGetLastError();   // a failed record only loses the time: clear the error state
+    return;
+  }
+  if (stage == 4) {
+    if ((int)plan.m2l_recorded.size() <= level) plan.m2l_recorded.resize(level + 1, 0);
+    plan.m2l_recorded[level] = 1;
+  } else {
+    plan.stage_recorded[stage] = true;
+  }
 }
 
 // the boundary points of the p^3 lattice in CartesianIndices order (coordinate 0
@@ -128,6 +135,20 @@ int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], do
     if (!ctx || !lo || !hi || !all_finite) fail(FFMM_ERR_ARGUMENT, "null argument");
     set_device(ctx->c);
     bounds(ctx->c, xyz, n, lo, hi, all_finite);
+  });
+}
+
+int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
+                const int64_t *skip, double *out, int output) {
+  return guard([&] {
+    if (!ctx) fail(FFMM_ERR_ARGUMENT, "null context");
+    if (ns < 0 || nt < 0) fail(FFMM_ERR_DIMENSION, "negative sizes");
+    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (nt == 0) return;
+    if (!tgt || !out || (ns > 0 && (!src || !tau))) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(ctx->c);
+    direct(ctx->c, src, tau, ns, tgt, nt, skip, out, output);
+    ctx->c.sync();
   });
 }
 
@@ -398,7 +419,7 @@ int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted) {
       StageTimer tm(p, 2);
       upward_pass(p);
     }
-    p.m2l_levels_timed = 0;
+    std::fill(p.m2l_recorded.begin(), p.m2l_recorded.end(), 0);
     {
       StageTimer tm(p, 3);
       downward_pass(p);
@@ -470,13 +491,14 @@ int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n) {
       float t = 0.f;
       double v = 0.0;
       if (i == 4) {
-        for (int l = 0; l < p.m2l_levels_timed; l++)
-          if (p.m2l_ev[2 * l] && hipEventElapsedTime(&t, p.m2l_ev[2 * l], p.m2l_ev[2 * l + 1]) == hipSuccess) v += t;
+        for (size_t l = 0; l < p.m2l_recorded.size(); l++)
+          if (p.m2l_recorded[l] && hipEventElapsedTime(&t, p.m2l_ev[2 * l], p.m2l_ev[2 * l + 1]) == hipSuccess) v += t;
       } else if (p.stage_recorded[i] && hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
         v = t;
       }
       ms[i] = v;
     }
+    (void)hipGetLastError();   // a failed query only loses a time: leave no error state behind
   });
 }
 

@@ -38,7 +38,8 @@ struct Error : std::runtime_error {
   } while (0)
 #define FFMM_LAUNCH_CHECK() FFMM_HIP(hipGetLastError())
 
-inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// workgroups covering n work-items (at least one: every kernel checks its index against n)
+inline unsigned nblocks(int64_t n, int b) { return n > 0 ? (unsigned)((n + b - 1) / b) : 1u; }
 
 // A device array; resize() discards the contents, grow() keeps them.
 template <class T>
@@ -181,7 +182,7 @@ struct Plan {
   DBuf<uint32_t> tgt_keys;
   hipEvent_t ev[16] = {};
   std::vector<hipEvent_t> m2l_ev;   // per level: start, end
-  int m2l_levels_timed = 0;
+  std::vector<char> m2l_recorded;   // per level: both events recorded by the last pass
   bool stage_recorded[8] = {};
   bool expansions_ready = false;
 };
@@ -211,6 +212,10 @@ void upward_pass(Plan &plan);
 void downward_pass(Plan &plan);
 void m2l_dense_level(Plan &plan, int l);
 void m2l_spectral_level(Plan &plan, int l);
+
+// ---- direct sums (direct.hip) ----
+void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
+            const int64_t *skip, double *out, int output);
 
 // ---- evaluation (evaluate.hip) ----
 void evaluate_at_sources(Plan &plan, double *out, int output);

@@ -7,4 +7,4 @@ in Python over ctypes, and the host FP64 operator precompute.
 from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 from .fmm import (FMM, CubeKernelIndependent, DevicePlan, FMMState, Gradient, Laplace3D, OperatorCache,  # noqa: F401
                   Potential, PotentialGradient, build_expansions, build_expansions_, device_build_expansions_,
-                  device_matvec_, evaluate, evaluate_at_sources, square_bbox)
+                  device_matvec_, direct, direct_self, evaluate, evaluate_at_sources, square_bbox)

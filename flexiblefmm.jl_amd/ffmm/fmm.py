@@ -148,6 +148,41 @@ def _ptr(t) -> ctypes.c_void_p:
 
 
 # ---------------------------------------------------------------------------
+# direct sums (src/kernels.jl:442-506), on the device in FP64
+# ---------------------------------------------------------------------------
+def direct(K, sources, tau, targets, output=Potential, skip=None, device: int = 0) -> np.ndarray:
+    """``direct(K, sources, tau, targets; output)``: the O(N M) sum at the targets,
+    excluding source ``skip[j]`` at target ``j`` when ``skip`` is given."""
+    if not isinstance(K, Laplace3D):
+        raise ArgumentError("the device direct sum takes Laplace3D")
+    o = _output(output)
+    torch = _torch()
+    s = _dev_f64(sources, device, cols=3)
+    t = _dev_f64(tau, device)
+    x = _dev_f64(targets, device, cols=3)
+    if t.ndim != 1 or t.shape[0] != s.shape[0]:
+        raise DimensionMismatch("sources and τ differ in length")
+    nt = x.shape[0]
+    out = torch.empty((nt,) if o.ncomp == 1 else (nt, o.ncomp), dtype=torch.float64, device=f"cuda:{device}")
+    sk = None
+    if skip is not None:
+        sk = torch.as_tensor(np.asarray(skip, dtype=np.int64)).to(f"cuda:{device}")
+        if sk.shape[0] != nt:
+            raise DimensionMismatch("skip and targets differ in length")
+    call("ffmm_direct", context(device), _ptr(s), _ptr(t), s.shape[0], _ptr(x), nt,
+         None if sk is None else _ptr(sk), _ptr(out), o.code)
+    return out.cpu().numpy()
+
+
+def direct_self(K, sources, tau, output=Potential, rows=None, device: int = 0) -> np.ndarray:
+    """``direct_self(K, sources, tau; output)``: the sum at every source (or at the
+    sources ``rows``) over all other sources."""
+    src = np.asarray(sources, dtype=np.float64) if not hasattr(sources, "data_ptr") else sources.cpu().numpy()
+    rows = np.arange(len(src), dtype=np.int64) if rows is None else np.asarray(rows, dtype=np.int64)
+    return direct(K, src, tau, src[rows], output, skip=rows, device=device)
+
+
+# ---------------------------------------------------------------------------
 # operator cache (src/operators/cache.jl)
 # ---------------------------------------------------------------------------
 class OperatorCache:

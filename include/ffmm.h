@@ -39,9 +39,10 @@ extern "C" {
 #define FFMM_M2L_DENSE 0     /* symmetry-class GEMMs (the reference device plan's form) */
 #define FFMM_M2L_SPECTRAL 1  /* batched FFT + per-pair Hadamard products (the reference CPU form) */
 
-/* outputs */
+/* outputs: Potential(), Gradient() (src/sources.jl:31-43), or both from one pass */
 #define FFMM_POTENTIAL 0
 #define FFMM_GRADIENT 1
+#define FFMM_POTENTIAL_GRADIENT 2
 
 typedef struct ffmm_context_s *ffmm_context;
 typedef struct ffmm_opcache_s *ffmm_opcache;
@@ -61,6 +62,14 @@ int ffmm_context_destroy(ffmm_context ctx);
 /* Componentwise bounds of n device points (xyz, 3 x n AoS, FP64), NaN propagating, and
  * whether every coordinate is finite: _device_bounds (src/tree/device_builder.jl:124-148). */
 int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite);
+
+/* FP64 direct sums on the device: direct! / direct_self! (src/kernels.jl:442-487).
+ * out[j] = sum_i target_field(K, src_i, tgt_j, tau_i, output) over the ns sources
+ * (3 x ns AoS) at the nt targets (3 x nt AoS), excluding source skip[j] when skip is
+ * not NULL (skip[j] < 0: none).  output: FFMM_POTENTIAL (nt), FFMM_GRADIENT (3 x nt),
+ * FFMM_POTENTIAL_GRADIENT (4 x nt). */
+int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
+                const int64_t *skip, double *out, int output);
 
 /* The device copies of the operators of one method, shared by every plan built with
  * the cache: OperatorCache.device (src/operators/cache.jl:196-213,

@@ -114,3 +114,35 @@ void oeval(const double *tx, const int32_t *tnode, const int32_t *tskip, const i
         else out[o] = acc[0];
     }
 }
+
+/* The class-wise M2L gather and scatter of the dense translations (the reference's
+ * m2l_level! through the class representatives, src/fmm.jl:100-127 with
+ * src/operators/symmetry.jl:104-117): expansions are node major (row per node).
+ *   G[q, i] = sgn[f_q, i] Mp[src_q, idx[f_q, i]]                      (P^T m)
+ *   pL[t, idx[f, i]] += alpha sgn[f, i] Y[q, i]  over the pairs q of a run (P y),
+ * the pairs of one target consecutive (runs), runs in parallel. */
+void om2l_gather(const double *Mp, int Ns, const int64_t *src, const int64_t *far, int64_t np,
+                 const int32_t *idx, const int8_t *sgn, double *G) {
+    #pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < np; q++) {
+        const double *m = &Mp[(int64_t)Ns * src[q]];
+        const int32_t *ix = &idx[(int64_t)Ns * far[q]];
+        const int8_t *sg = &sgn[(int64_t)Ns * far[q]];
+        double *g = &G[(int64_t)Ns * q];
+        for (int i = 0; i < Ns; i++) g[i] = (double)sg[i] * m[ix[i]];
+    }
+}
+
+void om2l_scatter(double *pL, int Ns, const int64_t *tgt, const int64_t *far, const int64_t *runs, int64_t nruns,
+                  const int32_t *idx, const int8_t *sgn, const double *Y, double alpha) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nruns; r++) {
+        for (int64_t q = runs[r]; q < runs[r + 1]; q++) {
+            double *p = &pL[(int64_t)Ns * tgt[q]];
+            const int32_t *ix = &idx[(int64_t)Ns * far[q]];
+            const int8_t *sg = &sgn[(int64_t)Ns * far[q]];
+            const double *y = &Y[(int64_t)Ns * q];
+            for (int i = 0; i < Ns; i++) p[ix[i]] += alpha * ((double)sg[i] * y[i]);
+        }
+    }
+}

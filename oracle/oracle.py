@@ -68,6 +68,10 @@ def lib():
         L.os2m.argtypes = [i32p, ctypes.c_int64, f64p, i32p, f64p, f64p, ctypes.c_int, i32p, i32p, f64p, f64p, f64p]
         L.oeval.argtypes = [f64p, i32p, i32p, i64p, ctypes.c_int64, f64p, i64p, f64p, f64p, ctypes.c_int,
                             i32p, f64p, i32p, i32p, i32p, f64p, f64p, ctypes.c_int, f64p]
+        i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+        L.om2l_gather.argtypes = [f64p, ctypes.c_int, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p]
+        L.om2l_scatter.argtypes = [f64p, ctypes.c_int, i64p, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p,
+                                   ctypes.c_double]
         _LIB = L
     return _LIB
 
@@ -193,10 +197,20 @@ class FMM:
     ``ops.width``) shared by every level through the homogeneous rescale
     (src/fmm.jl:31-49, src/operators/cache.jl:224-237)."""
 
-    def __init__(self, ops, points, ppl: int, origin, W0: float, max_levels: int = 30):
+    def __init__(self, ops, points, ppl: int, origin, W0: float, max_levels: int = 30, perm=None):
         self.ops = ops
         self.tree = build_tree(points, ppl, origin, W0, max_levels)
         t = self.tree
+        if perm is not None:
+            # the same tree with another order of the points inside each leaf (the device
+            # sorts them by their full key): sums then run in that order
+            perm = np.asarray(perm, dtype=np.int32)
+            for i in np.nonzero(t.leaf)[0]:
+                a, b = t.plo[i], t.phi[i] + 1
+                if not np.array_equal(np.sort(perm[a:b]), np.sort(t.perm[a:b])):
+                    raise OracleError("perm reorders points across leaves")
+            t.perm = perm.copy()
+            t.points = np.ascontiguousarray(np.asarray(points, dtype=np.float64)[perm])
         L = t.nlevels
         self.lam = np.array([t.width(l) / ops.width for l in range(L)])
         self.centers = np.ascontiguousarray(t.centers())
@@ -207,24 +221,26 @@ class FMM:
                            for l in range(L)]
         self.surf = np.ascontiguousarray(ops.large)
 
-    # upward_pass! (src/fmm.jl:170-205)
+    # upward_pass! (src/fmm.jl:170-205); expansions are stored node major (row per node)
     def upward(self, tau_sorted: np.ndarray) -> np.ndarray:
         t, ops = self.tree, self.ops
         L = t.nlevels
         Ns, Nl = ops.nsmall, ops.nlarge
-        Mp = np.zeros((Ns, t.nnodes))
+        Mp = np.zeros((t.nnodes, Ns))
+        P = ops.P_up
         for l in range(L - 1, 1, -1):
             nodes = self.s2m_nodes[l]
             if len(nodes):
-                pM = np.empty((len(nodes), Nl))          # column q of the Fortran-ordered pM
+                pM = np.empty((len(nodes), Nl))          # row q: the check potentials of node q
                 lib().os2m(nodes, len(nodes), self.centers, t.level, self.surf, self.lam, Nl, t.plo, t.phi,
                            t.points, tau_sorted, pM)
-                Mp[:, nodes] = ops.P_up.apply(pM.T, self.lam[l])
+                # partial_to_multipole!: pinv_scale V (invS .* (Uh pM))
+                Mp[nodes] = self.lam[l] * (((pM @ P.Uh.T) * P.invS[None, :]) @ P.V.T)
             if l < L - 1:
                 r = t.level_range(l + 1)
-                # the children block, one stacked column per sibling group
-                B = Mp[:, r.start:r.stop].T.reshape(-1, 8 * Ns).T
-                Mp[:, t.group_parent(l + 1)] = ops.M2M @ B
+                # the children block, one stacked row of 8 children per sibling group
+                B = Mp[r.start:r.stop].reshape(-1, 8 * Ns)
+                Mp[t.group_parent(l + 1)] = B @ ops.M2M.T
         return Mp
 
     # downward_pass! (src/fmm.jl:207-262)
@@ -232,72 +248,80 @@ class FMM:
         t, ops = self.tree, self.ops
         L = t.nlevels
         Ns, Nl = ops.nsmall, ops.nlarge
-        Pl = np.zeros((Ns, t.nnodes))
-        Lc = np.zeros((Nl, t.nnodes))
+        Pl = np.zeros((t.nnodes, Ns))
+        Lc = np.zeros((t.nnodes, Nl))
+        P = ops.P_down
         for l in range(2, L):
             r = t.level_range(l)
-            nl = len(r)
             if l > 2:
-                G = Pl[:, t.group_parent(l)]
-                pL = (ops.L2L @ G).T.reshape(nl, Ns).T.copy()
+                G = Pl[t.group_parent(l)]
+                pL = (G @ ops.L2L.T).reshape(-1, Ns)
             else:
-                pL = np.zeros((Ns, nl))
+                pL = np.zeros((len(r), Ns))
             alpha = 1.0 / self.lam[l]
             if m2l == "dense":
                 self._m2l_dense(pL, Mp, l, alpha)
             else:
                 self._m2l_spectral(pL, Mp, l, alpha)
-            Pl[:, r.start:r.stop] = pL
+            Pl[r.start:r.stop] = pL
             leaves = self.leaf_nodes[l]
             if len(leaves):
-                Lc[:, leaves] = ops.P_down.apply(Pl[:, leaves], self.lam[l])
+                # partial_to_local!: pinv_scale V (invS .* (Uh pL)) of the real leaves
+                Lc[leaves] = self.lam[l] * (((Pl[leaves] @ P.Uh.T) * P.invS[None, :]) @ P.V.T)
         return Pl, Lc
 
-    def _pairs_by_offset(self, l: int):
+    def _pairs(self, l: int):
         ptr, src, far = self.tree.lists[l]
-        lo = self.tree.level_start[l]
-        tgt = lo + np.repeat(np.arange(len(ptr) - 1, dtype=np.int64), np.diff(ptr))
+        tgt = np.repeat(np.arange(len(ptr) - 1, dtype=np.int64), np.diff(ptr))   # level-local targets
         return tgt, src.astype(np.int64), far.astype(np.int64)
 
     def _m2l_dense(self, pL, Mp, l, alpha):
-        """Per far offset: gather, the class representative through the signed
-        permutation, scatter-add (a target appears at most once per offset)."""
+        """Per symmetry class: gather P_f^T m_s of the class's pairs, one GEMM with the
+        class representative, scatter P_f y into the targets."""
         ops = self.ops
         sc = ops.sc
-        lo = int(self.tree.level_start[l])
-        tgt, src, far = self._pairs_by_offset(l)
-        order = np.argsort(far, kind="stable")
-        tgt, src, far = tgt[order], src[order], far[order]
-        bounds = np.searchsorted(far, np.arange(317))
-        for f in range(316):
-            a, b = bounds[f], bounds[f + 1]
+        Ns = ops.nsmall
+        tgt, src, far = self._pairs(l)
+        if len(tgt) == 0:
+            return
+        klass = sc.klass[far]
+        order = np.lexsort((tgt, klass))
+        tgt, src, far, klass = tgt[order], src[order], far[order], klass[order]
+        bounds = np.searchsorted(klass, np.arange(len(ops.class_ops) + 1))
+        idx = np.ascontiguousarray(sc.idx, dtype=np.int32)
+        sgn = np.ascontiguousarray(sc.sgn, dtype=np.int8)
+        for c in range(len(ops.class_ops)):
+            a, b = bounds[c], bounds[c + 1]
             if a == b:
                 continue
-            idx = sc.idx[f]
-            Y = ops.class_ops[sc.klass[f]] @ Mp[np.ix_(idx, src[a:b])]
-            pL[np.ix_(idx, tgt[a:b] - lo)] += alpha * Y
+            G = np.empty((b - a, Ns))
+            lib().om2l_gather(Mp, Ns, src[a:b].copy(), far[a:b].copy(), b - a, idx, sgn, G)
+            Y = G @ ops.class_ops[c].T
+            tc = tgt[a:b]
+            runs = np.concatenate([[0], np.nonzero(np.diff(tc))[0] + 1, [b - a]]).astype(np.int64)
+            lib().om2l_scatter(pL, Ns, tc.copy(), far[a:b].copy(), runs, len(runs) - 1, idx, sgn, Y, alpha)
 
     def _m2l_spectral(self, pL, Mp, l, alpha):
         """The level-wise spectral translation: transform every box, Hadamard-accumulate
         over the pairs, transform back and extract."""
         ops = self.ops
         n = 2 * ops.p
-        r = self.tree.level_range(l)
-        lo = r.start
-        tgt, src, far = self._pairs_by_offset(l)
+        lo = self.tree.level_start[l]
+        tgt, src, far = self._pairs(l)
         if len(tgt) == 0:
             return
+        tgt = tgt + lo
         nodes = np.unique(np.concatenate([src, tgt]))
         slot = np.full(self.tree.nnodes, -1, np.int64)
         slot[nodes] = np.arange(len(nodes))
         lat = np.zeros((len(nodes), n ** 3))
-        lat[:, ops.embed] = Mp[:, nodes].T
+        lat[:, ops.embed] = Mp[nodes]
         S = np.fft.rfftn(lat.reshape(-1, n, n, n), axes=(1, 2, 3))
         U = np.zeros_like(S)
         np.add.at(U, slot[tgt], ops.tables[far] * S[slot[src]])
         u = np.fft.irfftn(U, s=(n, n, n), axes=(1, 2, 3)).reshape(len(nodes), -1)[:, ops.embed]
         ts = np.unique(tgt)
-        pL[:, ts - lo] += alpha * u[slot[ts]].T
+        pL[ts - lo] += alpha * u[slot[ts]]
 
     # evaluate_at_sources! (src/evaluate.jl:425-441)
     def evaluate_at_sources(self, Lc, tau_sorted, grad=False):
@@ -305,7 +329,7 @@ class FMM:
         N = len(t.perm)
         out = np.empty((N, 3) if grad else N)
         lib().oeval(t.points, t.point_node, np.arange(N, dtype=np.int32), t.perm.astype(np.int64), N,
-                    np.ascontiguousarray(Lc.T), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
+                    np.ascontiguousarray(Lc), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
                     self.ops.nlarge, t.level, self.centers, t.colleagues, t.plo, t.phi, t.points, tau_sorted,
                     int(grad), out)
         return out
@@ -318,7 +342,7 @@ class FMM:
         nt = len(x)
         out = np.empty((nt, 3) if grad else nt)
         lib().oeval(x, nodes, np.full(nt, -1, np.int32), np.arange(nt, dtype=np.int64), nt,
-                    np.ascontiguousarray(Lc.T), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
+                    np.ascontiguousarray(Lc), np.arange(t.nnodes, dtype=np.int64), self.surf, self.lam,
                     self.ops.nlarge, t.level, self.centers, t.colleagues, t.plo, t.phi, t.points, tau_sorted,
                     int(grad), out)
         return out
@@ -329,7 +353,9 @@ class FMM:
         tau_sorted = np.ascontiguousarray(np.asarray(tau, dtype=np.float64)[self.tree.perm])
         Mp = self.upward(tau_sorted)
         Pl, Lc = self.downward(Mp, m2l)
-        self.multipoles, self.plocals, self.locals = Mp, Pl, Lc
+        self.Lc = Lc
+        # column per node (Ns x nnodes, Nl x nnodes) views, as the reference stores them
+        self.multipoles, self.plocals, self.locals = Mp.T, Pl.T, Lc.T
         u = self.evaluate_at_sources(Lc, tau_sorted, grad)
         if targets is None:
             return u

@@ -53,14 +53,28 @@ def test_matvec_f64_dense_vs_oracle_and_direct(setup):
 
 
 def test_expansions_vs_oracle(setup):
+    """Expansions against the oracle summing in the device's point order.  The
+    equivalent densities (multipoles, locals) come out of pseudo-inverses that keep
+    modes down to 50 eps of the largest singular value, so two GEMM libraries leave
+    them apart by that rounding amplified in the near-null modes (1e-9 at p = 8);
+    what they represent is pinned: the multipoles through their potentials on the
+    upward check surface (K_up M), the locals through theirs on the downward check
+    surface (K_down L), the partial locals (check data) directly.  The reference's own
+    comparison, at p = 6, is test/test_device_passes.jl:158-161."""
+    from ffmm import operators as fo
     s = setup
-    plan, ref = s["plan"], s["ref"]
-    plan.build_expansions(s["tau"])
+    plan = s["plan"]
+    ops = s["cache"].host_operators()
+    ref = O.FMM(ops, s["pts"], 30, BOX[0], BOX[1], perm=plan.download_tree().perm)
+    u_ref = ref.matvec(s["tau"])
+    assert mrd(plan.matvec(s["tau"]).cpu().numpy(), u_ref) < 1e-12
     M, Pl, Lc = plan.download_expansions()
-    assert mrd(M.ravel(), ref.multipoles.ravel()) < 1e-11
+    K_up = fo.laplace3d_matrix(ops.small, ops.large)        # multipole -> upward check potentials
+    K_dn = fo.laplace3d_matrix(ops.large, ops.small)        # local density -> downward check potentials
+    assert mrd((K_up @ M).ravel(), (K_up @ ref.multipoles).ravel()) < 1e-11
     assert mrd(Pl.ravel(), ref.plocals.ravel()) < 1e-11
     leaves = np.concatenate(ref.leaf_nodes)
-    assert mrd(Lc.ravel(), ref.locals[:, leaves].ravel()) < 1e-9
+    assert mrd((K_dn @ Lc).ravel(), (K_dn @ ref.locals[:, leaves]).ravel()) < 1e-11
     nonleaf = np.ones(ref.tree.nnodes, bool)
     nonleaf[leaves] = False
     assert np.all(ref.locals[:, nonleaf] == 0)
@@ -78,12 +92,12 @@ def test_targets_vs_oracle_and_direct(setup, out):
     tau_sorted = s["tau"][ref.tree.perm]
     ref.matvec(s["tau"])
     if out in ("potential", "both"):
-        vr = ref.evaluate(ref.locals, tau_sorted, targets, grad=False)
+        vr = ref.evaluate(ref.Lc, tau_sorted, targets, grad=False)
         vp = v if out == "potential" else v[:, 0]
         assert mrd(vp, vr) < 1e-11
         assert mrd(vp, O.direct(s["pts"], s["tau"], targets)) < 1e-6
     if out in ("gradient", "both"):
-        gr = ref.evaluate(ref.locals, tau_sorted, targets, grad=True)
+        gr = ref.evaluate(ref.Lc, tau_sorted, targets, grad=True)
         vg = v if out == "gradient" else v[:, 1:]
         assert mrd(vg, gr) < 1e-11
         assert mrd(vg, O.direct(s["pts"], s["tau"], targets, grad=True)) < 1e-5
@@ -93,7 +107,7 @@ def test_gradient_at_sources(setup):
     s = setup
     ffmm = s["ffmm"]
     g = s["plan"].matvec(s["tau"], ffmm.Gradient).cpu().numpy()
-    gr = s["ref"].evaluate_at_sources(s["ref"].locals, s["tau"][s["ref"].tree.perm], grad=True)
+    gr = s["ref"].evaluate_at_sources(s["ref"].Lc, s["tau"][s["ref"].tree.perm], grad=True)
     assert mrd(g, gr) < 1e-11
 
 
