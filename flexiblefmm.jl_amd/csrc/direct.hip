@@ -1,8 +1,12 @@
 // FP64 direct sums on the device: direct! and direct_self! (src/kernels.jl:442-487),
 // out[j] = sum_i target_field(K, s_i, t_j, tau_i, output), the source skip[j] excluded
 // (a masked select).  One work-item per target; the sources stream through LDS in
-// tiles shared by the workgroup.
+// tiles shared by the workgroup.  The sources are split into chunks over the second
+// grid axis so that a few thousand targets still fill the chip; the chunks' partial
+// sums are added in chunk order by a second pass (deterministic).
 #include "ffmm_internal.h"
+
+#include <algorithm>
 
 namespace ffmm {
 
@@ -12,26 +16,28 @@ constexpr double C4PI = 0.07957747154594767;   // 1 / (4 pi)
 template <int OUT>
 __global__ void __launch_bounds__(DT) k_direct(const double *__restrict__ src, const double *__restrict__ tau, int64_t ns,
                                                const double *__restrict__ tgt, int64_t nt,
-                                               const int64_t *__restrict__ skip, double *__restrict__ out) {
+                                               const int64_t *__restrict__ skip, int64_t chunk,
+                                               double *__restrict__ part) {
   __shared__ double xs[DT], ys[DT], zs[DT], ts[DT];
   const int64_t j = (int64_t)blockIdx.x * DT + threadIdx.x;
   const int64_t jj = j < nt ? j : nt - 1;
   const double x = tgt[3 * jj], y = tgt[3 * jj + 1], z = tgt[3 * jj + 2];
   const int64_t sk = skip ? skip[jj] : -1;
+  const int64_t ib = (int64_t)blockIdx.y * chunk, ie = std::min<int64_t>(ns, ib + chunk);
   double acc[4] = {0, 0, 0, 0};
-  for (int64_t i0 = 0; i0 < ns; i0 += DT) {
+  for (int64_t i0 = ib; i0 < ie; i0 += DT) {
     const int64_t i = i0 + threadIdx.x;
-    if (i < ns) {
+    if (i < ie) {
       xs[threadIdx.x] = src[3 * i];
       ys[threadIdx.x] = src[3 * i + 1];
       zs[threadIdx.x] = src[3 * i + 2];
       ts[threadIdx.x] = tau[i];
     }
     __syncthreads();
-    const int cnt = (int)min<int64_t>(DT, ns - i0);
+    const int cnt = (int)std::min<int64_t>(DT, ie - i0);
     for (int q = 0; q < cnt; q++) {
       const double d0 = x - xs[q], d1 = y - ys[q], d2 = z - zs[q];
-      const double ir = 1.0 / sqrt(fma(d2, d2, fma(d1, d1, d0 * d0)));
+      const double ir = rsqrt_f64(fma(d2, d2, fma(d1, d1, d0 * d0)));
       const bool keep = i0 + q != sk;
       if (OUT != FFMM_GRADIENT) acc[0] += keep ? (C4PI * ir) * ts[q] : 0.0;
       if (OUT != FFMM_POTENTIAL) {
@@ -44,9 +50,20 @@ __global__ void __launch_bounds__(DT) k_direct(const double *__restrict__ src, c
     __syncthreads();
   }
   if (j >= nt) return;
-  if (OUT == FFMM_POTENTIAL) {
+  double *p = part + ((int64_t)blockIdx.y * nt + j) * 4;
+  p[0] = acc[0]; p[1] = acc[1]; p[2] = acc[2]; p[3] = acc[3];
+}
+
+__global__ void k_direct_reduce(const double *__restrict__ part, int64_t nt, int nchunk, int output,
+                                double *__restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  double acc[4] = {0, 0, 0, 0};
+  for (int c = 0; c < nchunk; c++)
+    for (int k = 0; k < 4; k++) acc[k] += part[((int64_t)c * nt + j) * 4 + k];
+  if (output == FFMM_POTENTIAL) {
     out[j] = acc[0];
-  } else if (OUT == FFMM_GRADIENT) {
+  } else if (output == FFMM_GRADIENT) {
     out[3 * j] = acc[1]; out[3 * j + 1] = acc[2]; out[3 * j + 2] = acc[3];
   } else {
     out[4 * j] = acc[0]; out[4 * j + 1] = acc[1]; out[4 * j + 2] = acc[2]; out[4 * j + 3] = acc[3];
@@ -56,11 +73,22 @@ __global__ void __launch_bounds__(DT) k_direct(const double *__restrict__ src, c
 void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
             const int64_t *skip, double *out, int output) {
   if (nt <= 0) return;
-  unsigned g = nblocks(nt, DT);
-  if (output == FFMM_POTENTIAL) k_direct<FFMM_POTENTIAL><<<g, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, out);
-  else if (output == FFMM_GRADIENT) k_direct<FFMM_GRADIENT><<<g, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, out);
-  else k_direct<2><<<g, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, out);
+  const unsigned gx = nblocks(nt, DT);
+  // about 4096 workgroups in all, chunks of at least 4096 sources
+  int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / gx, (ns + 4095) / 4096));
+  nchunk = std::min<int64_t>(nchunk, 65535);
+  const int64_t chunk = std::max<int64_t>(1, (ns + nchunk - 1) / nchunk);
+  nchunk = std::max<int64_t>(1, (ns + chunk - 1) / chunk);
+  DBuf<double> part;
+  part.resize((size_t)nchunk * nt * 4);
+  dim3 grid(gx, (unsigned)nchunk);
+  if (output == FFMM_POTENTIAL) k_direct<FFMM_POTENTIAL><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
+  else if (output == FFMM_GRADIENT) k_direct<FFMM_GRADIENT><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
+  else k_direct<2><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
   FFMM_LAUNCH_CHECK();
+  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part.p, nt, (int)nchunk, output, out);
+  FFMM_LAUNCH_CHECK();
+  ctx.sync();   // the partial sums are freed on return
 }
 
 }  // namespace ffmm

@@ -24,7 +24,7 @@ constexpr double C4PI = 0.07957747154594767;   // 1 / (4 pi)
 
 template <int OUT>
 __device__ __forceinline__ void accumulate(double d0, double d1, double d2, double w, double acc[4]) {
-  double ir = 1.0 / sqrt(fma(d2, d2, fma(d1, d1, d0 * d0)));
+  double ir = rsqrt_f64(fma(d2, d2, fma(d1, d1, d0 * d0)));
   if (OUT != FFMM_GRADIENT) acc[0] += (C4PI * ir) * w;
   if (OUT != FFMM_POTENTIAL) {
     double a = -C4PI * ir * ir * ir;
@@ -35,7 +35,7 @@ __device__ __forceinline__ void accumulate(double d0, double d1, double d2, doub
 }
 template <int OUT>
 __device__ __forceinline__ void accumulate_masked(double d0, double d1, double d2, double w, bool keep, double acc[4]) {
-  double ir = 1.0 / sqrt(fma(d2, d2, fma(d1, d1, d0 * d0)));
+  double ir = rsqrt_f64(fma(d2, d2, fma(d1, d1, d0 * d0)));
   if (OUT != FFMM_GRADIENT) acc[0] += keep ? (C4PI * ir) * w : 0.0;
   if (OUT != FFMM_POTENTIAL) {
     double a = -C4PI * ir * ir * ir;
@@ -80,8 +80,15 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
 #pragma unroll
   for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
   for (int32_t tb = t0; tb < t1; tb += EV_TB) {
-    const int32_t j = tb + (int32_t)threadIdx.x;
-    const bool valid = j < t1;
+    // a tile of up to 64 targets; when it has fewer, R = 64 / ntp replicas of each
+    // target split the surface points and the sources, and are summed at the end
+    const int cnt = min(EV_TB, t1 - tb);
+    int ntp = 1;
+    while (ntp < cnt) ntp <<= 1;
+    const int R = EV_TB / ntp;
+    const int tl = (int)threadIdx.x % ntp, rep = (int)threadIdx.x / ntp;
+    const int32_t j = tb + tl;
+    const bool valid = tl < cnt;
     const int32_t jj = valid ? j : t0;
     const double x = self ? a.sx[jj] : a.tx[jj];
     const double y = self ? a.sy[jj] : a.ty[jj];
@@ -89,7 +96,7 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
     const double xr = x - c0, yr = y - c1, zr = z - c2;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     // l2t over the large surface (src/operators/ki_common.jl:293-315)
-    for (int k = 0; k < a.Nl; k++)
+    for (int k = rep; k < a.Nl; k += R)
       accumulate<OUT>(xr - X[k] * sc, yr - Y[k] * sc, zr - Z[k] * sc, L[k], acc);
     // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363)
     const int32_t skip = self ? j : -1;
@@ -97,10 +104,13 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
       const int32_t nb = nbr[m];
       if (nb < 0) continue;
       const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
-      for (int32_t q = p0; q <= p1; q++)
+      for (int32_t q = p0 + rep; q <= p1; q += R)
         accumulate_masked<OUT>(x - a.sx[q], y - a.sy[q], z - a.sz[q], a.tau[q], q != skip, acc);
     }
-    if (valid) {
+    for (int off = ntp; off < EV_TB; off <<= 1)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[c] += __shfl_xor(acc[c], off, EV_TB);
+    if (valid && rep == 0) {
       const int64_t o = self ? a.outidx[j] : a.outidx[j];
       if (OUT == FFMM_POTENTIAL) {
         a.out[o] = acc[0];

@@ -240,6 +240,33 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     o.inv.upload(inv.data(), inv.size(), s);
     o.sgn.upload(sgn, (size_t)NFAR * Ns, s);
     o.embed.upload(o.embed_h.data(), o.embed_h.size(), s);
+    {
+      // the p^3-corner index of each surface point, and the far index of every
+      // (sibling a, child c of the parent's colleague k) pair: offset 2 o_k + b_c - b_a
+      const int n = 2 * o.p;
+      std::vector<int32_t> s8(Ns);
+      for (int i = 0; i < Ns; i++) {
+        int e = o.embed_h[i], i1 = e % n, i2 = (e / n) % n, i3 = e / (n * n);
+        s8[i] = i1 + o.p * (i2 + o.p * i3);
+      }
+      o.sub8.upload(s8.data(), Ns, s);
+      int far_index[343];
+      for (int u = 0, nf = 0; u < 343; u++) {
+        int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
+        far_index[u] = std::max(std::abs(x), std::max(std::abs(y), std::abs(z))) >= 2 ? nf++ : -1;
+      }
+      std::vector<int16_t> pf(NCOLL * NCHILD * NCHILD);
+      for (int k = 0; k < NCOLL; k++)
+        for (int c = 0; c < NCHILD; c++)
+          for (int a = 0; a < NCHILD; a++) {
+            int ok[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+            int u = 0;
+            for (int d = 0, m = 1; d < 3; d++, m *= 7) u += (2 * ok[d] + ((c >> d) & 1) - ((a >> d) & 1) + 3) * m;
+            pf[(k * NCHILD + c) * NCHILD + a] = (int16_t)far_index[u];
+          }
+      o.pairf.upload(pf.data(), pf.size(), s);
+      ctx.sync();
+    }
     o.has_tables = tables != nullptr;
     if (tables) {
       const int n = 2 * o.p;
@@ -303,6 +330,7 @@ int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, 
       build_lists(p);
     }
     allocate_pass_buffers(p);
+    if (p.m2l == FFMM_M2L_SPECTRAL) spectral_prepare(p);
     ctx.sync();
     *out = h.release();
   });

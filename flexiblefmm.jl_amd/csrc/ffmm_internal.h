@@ -38,6 +38,17 @@ struct Error : std::runtime_error {
   } while (0)
 #define FFMM_LAUNCH_CHECK() FFMM_HIP(hipGetLastError())
 
+// 1/sqrt(x) in FP64: the hardware estimate (v_rsq_f64) refined by two Newton steps,
+// y <- y + y (1/2 - x y^2 / 2), to within an ulp or two of the correctly rounded value;
+// +inf at x = 0 turns into NaN, which the callers mask with a select.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  return y;
+}
+
 // workgroups covering n work-items (at least one: every kernel checks its index against n)
 inline unsigned nblocks(int64_t n, int b) { return n > 0 ? (unsigned)((n + b - 1) / b) : 1u; }
 
@@ -127,6 +138,8 @@ struct OpCache {
   DBuf<double> tables;                       // 316 x nfreq complex (interleaved)
   DBuf<float> tables32;
   DBuf<int32_t> embed;                       // Ns: lattice index of each small-surface point
+  DBuf<int32_t> sub8;                        // Ns: its index in the p^3 corner of the lattice
+  DBuf<int16_t> pairf;                       // [(k * 8 + c) * 8 + a]: far index of (sibling a, child c of colleague k)
   std::vector<int32_t> embed_h;
   std::vector<int32_t> far_class_h;
 };
@@ -175,7 +188,7 @@ struct Plan {
   // spectral M2L workspace
   DBuf<double> spec_S, spec_U;
   DBuf<float> spec_S32, spec_U32;
-  DBuf<int32_t> spec_slot, spec_tgts;
+  std::vector<DBuf<int32_t>> spec_gptr, spec_cand;   // per level: sibling groups' far-source candidates
   // targets
   DBuf<double> tgt_sorted;
   DBuf<int32_t> tgt_node, tgt_leafslot, tgt_order, tgt_ptr;
@@ -211,6 +224,7 @@ void gather_inputs(Plan &plan, const double *tau, int sorted);
 void upward_pass(Plan &plan);
 void downward_pass(Plan &plan);
 void m2l_dense_level(Plan &plan, int l);
+void spectral_prepare(Plan &plan);
 void m2l_spectral_level(Plan &plan, int l);
 
 // ---- direct sums (direct.hip) ----

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(S2M_TB) k_s2m(const int32_t *__restrict__ node
       for (int m = 0; m < S2M_KP; m++) {
         double d0 = y0[m] - xr, d1 = y1[m] - yr, d2 = y2[m] - zr;
         double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
-        acc[m] += (C4PI * (1.0 / sqrt(r2))) * tj;
+        acc[m] += (C4PI * rsqrt_f64(r2)) * tj;
       }
     }
     __syncthreads();

@@ -1,13 +1,382 @@
-// The spectral M2L of the cube method on the device (placeholder until the batched
-// FFT kernels land): plans built with FFMM_M2L_SPECTRAL are rejected at run time.
+// The spectral M2L of the cube method on the device.
+//
+// The small surfaces of all boxes are p-per-edge lattices of one spacing, so M2L at a
+// far offset is a convolution of the multipole (embedded in the (2p)^3 lattice) with
+// a kernel table: u_t = IFFT( sum_pairs T_f * FFT(m_s) ).  Per level:
+//   1. forward: every sourced box's multipole into its half spectrum (p+1)(2p)(2p);
+//      the multipole lives in the [0, p)^3 corner of the padded lattice, so the three
+//      passes are small DFTs over p nonzero inputs (one workgroup per box, in LDS);
+//   2. Hadamard: per sibling group (8 targets sharing a parent) and frequency block,
+//      the far pairs are the sourced children of the parent's 27 colleagues; each
+//      source spectrum is read once for the 8 targets, the 316 tables of the block sit
+//      in LDS; one lane per (frequency, target), accumulators in registers, no atomics;
+//   3. inverse: per target, the three inverse DFTs with only the p^3 corner of the
+//      output formed, the surface entries scaled and added into the partial locals.
+// Follows _spectral_m2l_level! and _hadamard_block! (src/operators/spectral_m2l.jl:
+// 260-286, 328-451); the pair enumeration is that of the interaction lists
+// (src/tree/lists.jl:96-124).  p = 8 (n = 16) is fixed here.
 #include "ffmm_internal.h"
+
+#include <cmath>
 
 namespace ffmm {
 
+constexpr int SP_P = 8;                       // points per edge
+constexpr int SP_N = 2 * SP_P;                // padded lattice
+constexpr int SP_H = SP_P + 1;                // half spectrum along the first axis
+constexpr int SP_NF = SP_H * SP_N * SP_N;     // 2304 frequencies
+constexpr int SP_TB = 256;
+
+template <class T> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+
+__constant__ double c_w16[2 * SP_N];   // exp(-2 pi i j / 16), j = 0..15: (cos, -sin)
+
+template <class C, class T>
+__device__ __forceinline__ void cmac(C &acc, C a, T br, T bi) {   // acc += a * (br + i bi)
+  acc.x = fma(a.x, br, fma(-a.y, bi, acc.x));
+  acc.y = fma(a.x, bi, fma(a.y, br, acc.y));
+}
+
+// ---------------------------------------------------------------------------
+// 1. forward: S[node][k3][k2][k1] = sum_{i in [0,p)^3} x[i3][i2][i1] w^(i1 k1 + i2 k2 + i3 k3)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(SP_TB) k_spec_forward(const double *__restrict__ Mp, int Ns,
+                                                        const int32_t *__restrict__ sub, int64_t lo, int64_t nl,
+                                                        const uint8_t *__restrict__ has_source,
+                                                        typename cplx_t<T>::type *__restrict__ S) {
+  using C = typename cplx_t<T>::type;
+  __shared__ T X[SP_P * SP_P * SP_P];
+  __shared__ C A[SP_P * SP_P * SP_H];
+  __shared__ C B[SP_P * SP_N * SP_H];
+  __shared__ T wr[SP_N], wi[SP_N];
+  const int64_t q = blockIdx.x;
+  const int64_t node = lo + q;
+  if (!has_source[node]) return;
+  const int t = threadIdx.x;
+  if (t < SP_N) { wr[t] = (T)c_w16[2 * t]; wi[t] = (T)c_w16[2 * t + 1]; }
+  for (int i = t; i < SP_P * SP_P * SP_P; i += SP_TB) X[i] = (T)0;
+  __syncthreads();
+  for (int i = t; i < Ns; i += SP_TB) X[sub[i]] = (T)Mp[node * Ns + i];
+  __syncthreads();
+  // first axis (real input): A[i3][i2][k1], k1 < p + 1
+  for (int o = t; o < SP_P * SP_P * SP_H; o += SP_TB) {
+    const int k1 = o % SP_H, r = o / SP_H;
+    C acc = {(T)0, (T)0};
+#pragma unroll
+    for (int i1 = 0; i1 < SP_P; i1++) {
+      const int w = (i1 * k1) & (SP_N - 1);
+      const T x = X[r * SP_P + i1];
+      acc.x = fma(x, wr[w], acc.x);
+      acc.y = fma(x, wi[w], acc.y);
+    }
+    A[o] = acc;
+  }
+  __syncthreads();
+  // second axis: B[i3][k2][k1]
+  for (int o = t; o < SP_P * SP_N * SP_H; o += SP_TB) {
+    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, i3 = o / (SP_H * SP_N);
+    C acc = {(T)0, (T)0};
+#pragma unroll
+    for (int i2 = 0; i2 < SP_P; i2++) {
+      const int w = (i2 * k2) & (SP_N - 1);
+      cmac(acc, A[(i3 * SP_P + i2) * SP_H + k1], wr[w], wi[w]);
+    }
+    B[o] = acc;
+  }
+  __syncthreads();
+  // third axis, straight to the spectrum
+  C *__restrict__ out = S + q * SP_NF;
+  for (int o = t; o < SP_NF; o += SP_TB) {
+    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, k3 = o / (SP_H * SP_N);
+    C acc = {(T)0, (T)0};
+#pragma unroll
+    for (int i3 = 0; i3 < SP_P; i3++) {
+      const int w = (i3 * k3) & (SP_N - 1);
+      cmac(acc, B[(i3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    }
+    out[o] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Hadamard: U[t][w] = sum over the far pairs (t, s) of T_f[w] S_s[w]
+// pairf[(k * 8 + c) * 8 + a]: far index of the pair (sibling a, child c of the parent's
+// colleague k), -1 when near.
+// ---------------------------------------------------------------------------
+// The far-pair sources of every sibling group of a level, compacted: the sourced
+// children of the parent's 27 colleagues as (level-local source << 8 | 8 k + c), the
+// group's entries at cand[gptr[g] .. gptr[g + 1]) (the enumeration of the lists,
+// src/tree/lists.jl:96-124, shared by the 8 siblings); none for a ghost group.
+__global__ void k_group_cand_count(const int32_t *__restrict__ parent, const int32_t *__restrict__ child_first,
+                                   const int32_t *__restrict__ col, const uint8_t *__restrict__ ghost,
+                                   const uint8_t *__restrict__ has_source, int64_t lo, int64_t ngroups, int64_t nn,
+                                   int32_t *__restrict__ cnt) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t first = lo + NCHILD * g;
+  int32_t n = 0;
+  if (!ghost[first]) {
+    const int32_t P = parent[first];
+    for (int k = 0; k < NCOLL; k++) {
+      const int32_t pc = col[(int64_t)k * nn + P];
+      if (pc < 0) continue;
+      const int32_t cf = child_first[pc];
+      if (cf < 0) continue;
+      for (int c = 0; c < NCHILD; c++) n += has_source[cf + c];
+    }
+  }
+  cnt[g] = n;
+}
+__global__ void k_group_cand_fill(const int32_t *__restrict__ parent, const int32_t *__restrict__ child_first,
+                                  const int32_t *__restrict__ col, const uint8_t *__restrict__ ghost,
+                                  const uint8_t *__restrict__ has_source, int64_t lo, int64_t ngroups, int64_t nn,
+                                  const int32_t *__restrict__ gptr, int32_t *__restrict__ cand) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t first = lo + NCHILD * g;
+  if (ghost[first]) return;
+  const int32_t P = parent[first];
+  int32_t pos = gptr[g];
+  for (int k = 0; k < NCOLL; k++) {
+    const int32_t pc = col[(int64_t)k * nn + P];
+    if (pc < 0) continue;
+    const int32_t cf = child_first[pc];
+    if (cf < 0) continue;
+    for (int c = 0; c < NCHILD; c++)
+      if (has_source[cf + c]) cand[pos++] = (int32_t)((cf + c - lo) << 8) | (k * NCHILD + c);
+  }
+}
+
+// One wave per sibling group and frequency block of FB.  The 316 tables of the block
+// sit in LDS (rows padded by one entry against bank conflicts, plus a zero row that
+// near pairs point at, so the inner loop has no branches), the group's candidate list
+// in a per-wave LDS slice.  Each step, lane (phi, r) takes candidate j + r
+// (R = 64 / FB candidates per step), reads its spectrum once and applies it to all 8
+// siblings (accumulators for the 8 targets in registers); the next step's spectrum is
+// loaded before the current one is used.  At the end the R partial sums of each
+// (phi, target) are reduced across the wave.
+constexpr int HD_TB = 512;   // 8 waves share one copy of the tables
+
+template <class T, int FB>
+__global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T>::type *__restrict__ S,
+                                                         typename cplx_t<T>::type *__restrict__ U,
+                                                         const typename cplx_t<T>::type *__restrict__ tables,
+                                                         const int16_t *__restrict__ pairf,
+                                                         const int32_t *__restrict__ gptr,
+                                                         const int32_t *__restrict__ cand, int64_t ngroups, int gpw) {
+  using C = typename cplx_t<T>::type;
+  constexpr int R = 64 / FB;               // lanes sharing a frequency: candidates per step
+  constexpr int TS = FB + 1;               // padded table row
+  constexpr int NW = HD_TB / 64;
+  constexpr int MAXC = NCOLL * NCHILD;     // 216 candidates at most
+  __shared__ C Tt[(NFAR + 1) * TS];
+  __shared__ __attribute__((aligned(16))) int16_t F[NCOLL * NCHILD * NCHILD];
+  __shared__ int32_t Cl[NW][MAXC + 64];
+  const int f0 = blockIdx.y * FB;
+  const int t = threadIdx.x;
+  for (int i = t; i < (NFAR + 1) * TS; i += HD_TB) {
+    const int f = i / TS, ph = i % TS;
+    Tt[i] = (f < NFAR && ph < FB) ? tables[(int64_t)f * SP_NF + f0 + ph] : C{(T)0, (T)0};
+  }
+  for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HD_TB) F[i] = pairf[i] < 0 ? (int16_t)NFAR : pairf[i];
+  __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  const int phi = lane % FB, r = lane / FB;
+  int32_t *cl = Cl[wave];
+  const int64_t g0 = ((int64_t)blockIdx.x * NW + wave) * gpw;
+  for (int64_t g = g0; g < g0 + gpw && g < ngroups; g++) {
+    const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;
+    if (n == 0) continue;                    // a ghost group, or no sourced far boxes
+    for (int i = lane; i < n; i += 64) cl[i] = cand[cbeg + i];
+    __builtin_amdgcn_wave_barrier();
+    C acc[NCHILD];
+#pragma unroll
+    for (int a = 0; a < NCHILD; a++) acc[a] = {(T)0, (T)0};
+    // step 0's candidate and spectrum; past the end a lane re-reads entry 0 and adds zeros
+    int32_t e = cl[r < n ? r : 0];
+    C sv = S[(int64_t)(e >> 8) * SP_NF + f0 + phi];
+    bool live = r < n;
+    for (int j = 0; j < n; j += R) {
+      const int jn = j + R + r;
+      const int32_t en = cl[jn < n ? jn : 0];
+      const C svn = S[(int64_t)(en >> 8) * SP_NF + f0 + phi];
+      const int16_t *Fk = &F[(e & 255) * NCHILD];
+      const T sx = live ? sv.x : (T)0, sy = live ? sv.y : (T)0;
+#pragma unroll
+      for (int a = 0; a < NCHILD; a++) cmac(acc[a], Tt[Fk[a] * TS + phi], sx, sy);
+      e = en;
+      sv = svn;
+      live = jn < n;
+    }
+    // reduce over the R lanes of each frequency (lanes phi + FB j)
+#pragma unroll
+    for (int a = 0; a < NCHILD; a++) {
+#pragma unroll
+      for (int off = FB; off < 64; off <<= 1) {
+        acc[a].x += __shfl_xor(acc[a].x, off, 64);
+        acc[a].y += __shfl_xor(acc[a].y, off, 64);
+      }
+    }
+    // lane (phi, r) writes the targets a = r, r + R, ...
+    constexpr int CPL = NCHILD / R;
+    static_assert(CPL * R == NCHILD, "64 / FB must divide 8");
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+      const int a = r + m * R;
+      C v = acc[0];
+#pragma unroll
+      for (int b = 1; b < NCHILD; b++)
+        if (b == a) v = acc[b];
+      U[(NCHILD * g + a) * SP_NF + f0 + phi] = v;
+    }
+    __builtin_amdgcn_wave_barrier();          // the candidate slice is rewritten next
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. inverse: x = IFFT(U_t) on the [0, p)^3 corner, surface entries into pL
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(SP_TB) k_spec_inverse(const typename cplx_t<T>::type *__restrict__ U,
+                                                        const int32_t *__restrict__ ptr,
+                                                        const int32_t *__restrict__ sub, int Ns, double alpha,
+                                                        double *__restrict__ pL) {
+  using C = typename cplx_t<T>::type;
+  __shared__ C Cb[SP_P * SP_N * SP_H];
+  __shared__ C Db[SP_P * SP_P * SP_H];
+  __shared__ T wr[SP_N], wi[SP_N];
+  const int64_t q = blockIdx.x;
+  if (ptr[q + 1] == ptr[q]) return;          // no far pairs (ghosts included)
+  const int t = threadIdx.x;
+  if (t < SP_N) { wr[t] = (T)c_w16[2 * t]; wi[t] = -(T)c_w16[2 * t + 1]; }   // conjugate twiddles
+  __syncthreads();
+  const C *__restrict__ in = U + q * SP_NF;
+  // third axis: Cb[i3][k2][k1], i3 < p
+  for (int o = t; o < SP_P * SP_N * SP_H; o += SP_TB) {
+    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, i3 = o / (SP_H * SP_N);
+    C acc = {(T)0, (T)0};
+#pragma unroll
+    for (int k3 = 0; k3 < SP_N; k3++) {
+      const int w = (i3 * k3) & (SP_N - 1);
+      cmac(acc, in[(k3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    }
+    Cb[o] = acc;
+  }
+  __syncthreads();
+  // second axis: Db[i3][i2][k1], i2 < p
+  for (int o = t; o < SP_P * SP_P * SP_H; o += SP_TB) {
+    const int k1 = o % SP_H, i2 = (o / SP_H) % SP_P, i3 = o / (SP_H * SP_P);
+    C acc = {(T)0, (T)0};
+#pragma unroll
+    for (int k2 = 0; k2 < SP_N; k2++) {
+      const int w = (i2 * k2) & (SP_N - 1);
+      cmac(acc, Cb[(i3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    }
+    Db[o] = acc;
+  }
+  __syncthreads();
+  // first axis (real output, Hermitian): at the surface points only
+  double *__restrict__ out = pL + q * Ns;
+  for (int j = t; j < Ns; j += SP_TB) {
+    const int s8 = sub[j];
+    const int i1 = s8 % SP_P, r = s8 / SP_P;
+    const C *d = &Db[r * SP_H];
+    T x = d[0].x + ((i1 & 1) ? -d[SP_P].x : d[SP_P].x);
+    T y = (T)0;
+#pragma unroll
+    for (int k1 = 1; k1 < SP_P; k1++) {
+      const int w = (i1 * k1) & (SP_N - 1);
+      y = fma(d[k1].x, wr[w], fma(-d[k1].y, wi[w], y));
+    }
+    out[j] += alpha * (double)(x + (T)2 * y);
+  }
+}
+
+// host side -----------------------------------------------------------------
+static void upload_spectral_constants() {
+  double w[2 * SP_N];
+  for (int j = 0; j < SP_N; j++) {
+    w[2 * j] = std::cos(2 * M_PI * j / SP_N);
+    w[2 * j + 1] = -std::sin(2 * M_PI * j / SP_N);
+  }
+  FFMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w16), w, sizeof w));
+}
+
+void spectral_prepare(Plan &plan) {
+  OpCache &o = *plan.ops;
+  if (o.p != SP_P) fail(FFMM_ERR_ARGUMENT, "the spectral M2L is built for p = 8");
+  upload_spectral_constants();
+  int64_t nmax = 1;
+  for (int l = 2; l < plan.L; l++) nmax = std::max(nmax, plan.nl[l]);
+  if (plan.precision == FFMM_PREC_MIXED) {
+    plan.spec_S32.resize((size_t)2 * SP_NF * nmax);
+    plan.spec_U32.resize((size_t)2 * SP_NF * nmax);
+  } else {
+    plan.spec_S.resize((size_t)2 * SP_NF * nmax);
+    plan.spec_U.resize((size_t)2 * SP_NF * nmax);
+  }
+  // the sibling groups' candidate lists, once per plan (they follow the tree only)
+  Context &ctx = *plan.ctx;
+  hipStream_t s = ctx.stream;
+  plan.spec_gptr.clear();
+  plan.spec_cand.clear();
+  plan.spec_gptr.resize(plan.L);
+  plan.spec_cand.resize(plan.L);
+  DBuf<int32_t> cnt;
+  for (int l = 2; l < plan.L; l++) {
+    const int64_t lo = plan.lo[l], ng = plan.nl[l] / NCHILD;
+    cnt.resize(ng);
+    plan.spec_gptr[l].resize(ng + 1);
+    k_group_cand_count<<<nblocks(ng, 256), 256, 0, s>>>(plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p,
+                                                         plan.has_source.p, lo, ng, plan.nnodes, cnt.p);
+    FFMM_LAUNCH_CHECK();
+    const int64_t total = exclusive_scan_i32(ctx, cnt.p, plan.spec_gptr[l].p, ng);
+    const int32_t tot32 = (int32_t)total;
+    FFMM_HIP(hipMemcpyAsync(plan.spec_gptr[l].p + ng, &tot32, sizeof tot32, hipMemcpyHostToDevice, s));
+    plan.spec_cand[l].resize(total);
+    if (total > 0) {
+      k_group_cand_fill<<<nblocks(ng, 256), 256, 0, s>>>(plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p,
+                                                          plan.has_source.p, lo, ng, plan.nnodes, plan.spec_gptr[l].p,
+                                                          plan.spec_cand[l].p);
+      FFMM_LAUNCH_CHECK();
+    }
+    ctx.sync();   // tot32 lives on this stack frame
+  }
+}
+
+template <class T, int FB>
+static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typename cplx_t<T>::type *U,
+                           const typename cplx_t<T>::type *tables) {
+  Context &ctx = *plan.ctx;
+  OpCache &o = *plan.ops;
+  hipStream_t s = ctx.stream;
+  const int64_t lo = plan.lo[l], nl = plan.nl[l], ng = nl / NCHILD;
+  if (plan.lists[l].np == 0) return;
+  k_spec_forward<T><<<(unsigned)nl, SP_TB, 0, s>>>(plan.Mp.p, o.Ns, o.sub8.p, lo, nl, plan.has_source.p, S);
+  FFMM_LAUNCH_CHECK();
+  const int gpw = 4;
+  const int64_t gpb = (HD_TB / 64) * gpw;
+  dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
+  k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p, ng,
+                                               gpw);
+  FFMM_LAUNCH_CHECK();
+  const double alpha = (1.0 / plan.lam[l]) / (double)(SP_N * SP_N * SP_N);
+  k_spec_inverse<T><<<(unsigned)nl, SP_TB, 0, s>>>(U, plan.lists[l].ptr.p, o.sub8.p, o.Ns, alpha,
+                                                  plan.Pl.p + lo * o.Ns);
+  FFMM_LAUNCH_CHECK();
+}
+
 void m2l_spectral_level(Plan &plan, int l) {
-  (void)plan;
-  (void)l;
-  fail(FFMM_ERR_ARGUMENT, "the spectral M2L is not available in this build");
+  OpCache &o = *plan.ops;
+  if (plan.precision == FFMM_PREC_MIXED)
+    spectral_level<float, 16>(plan, l, (float2 *)plan.spec_S32.p, (float2 *)plan.spec_U32.p,
+                              (const float2 *)o.tables32.p);
+  else
+    spectral_level<double, 8>(plan, l, (double2 *)plan.spec_S.p, (double2 *)plan.spec_U.p,
+                              (const double2 *)o.tables.p);
 }
 
 }  // namespace ffmm

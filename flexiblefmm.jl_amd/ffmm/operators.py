@@ -241,7 +241,7 @@ def spectral_m2l_apply(ops: CubeOperators, f: int, x: np.ndarray, alpha: float =
     n = 2 * ops.p
     lat = np.zeros(n ** 3)
     lat[ops.embed] = x
-    u = np.fft.irfftn(np.fft.rfftn(lat.reshape(n, n, n)) * ops.tables[f], s=(n, n, n))
+    u = np.fft.irfftn(np.fft.rfftn(lat.reshape(n, n, n)) * ops.tables[f], s=(n, n, n), axes=(0, 1, 2))
     return alpha * u.reshape(-1)[ops.embed]
 
 

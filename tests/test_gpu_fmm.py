@@ -52,6 +52,27 @@ def test_matvec_f64_dense_vs_oracle_and_direct(setup):
     assert mrd(u[rows], ud) < 1e-6
 
 
+@pytest.mark.parametrize("precision,tol_oracle,tol_direct", [("f64", 1e-11, 1e-6), ("mixed", 2e-6, 2e-6)])
+def test_matvec_spectral_vs_oracle_and_direct(setup, precision, tol_oracle, tol_direct):
+    """The spectral M2L (batched DFTs + per-pair Hadamard products, the reference CPU's
+    form for the cube method, src/operators/spectral_m2l.jl:328-451) against the
+    oracle's dense class translations: equal to rounding in FP64 (spectral equals dense
+    to rtol 1e-12, test/test_cubeki.jl:43-68); FP32 spectra in the mixed form."""
+    s = setup
+    ffmm = s["ffmm"]
+    print(f"[fmm] spectral {precision}", flush=True)
+    plan = ffmm.DevicePlan(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"], m2l="spectral", precision=precision)
+    u = plan.matvec(s["tau"]).cpu().numpy()
+    assert mrd(u, s["u_ref"]) < tol_oracle
+    rows = s["rng"].choice(len(u), 400, replace=False)
+    assert mrd(u[rows], O.direct_self(s["pts"], s["tau"], rows=rows)) < tol_direct
+    # the plocals of the spectral and dense device plans
+    s["plan"].build_expansions(s["tau"])
+    _, Pd, _ = s["plan"].download_expansions()
+    _, Ps, _ = plan.download_expansions()
+    assert mrd(Ps.ravel(), Pd.ravel()) < (1e-12 if precision == "f64" else 1e-6)
+
+
 def test_expansions_vs_oracle(setup):
     """Expansions against the oracle summing in the device's point order.  The
     equivalent densities (multipoles, locals) come out of pseudo-inverses that keep
@@ -144,6 +165,23 @@ def test_host_api_mirror(setup):
         ffmm.build_expansions(f, s["pts"], s["tau"][:-1])
     with pytest.raises(ffmm.ArgumentError, match="outside"):
         ffmm.evaluate(f, st, targets + 2.0)
+
+
+def test_device_direct_sums_against_oracle():
+    """ffmm_direct (direct! / direct_self!, src/kernels.jl:442-487) against the oracle's
+    FP64 sums: potential, gradient and both, with and without the self term."""
+    import ffmm
+    rng = np.random.default_rng(21)
+    s, t = rng.random((20000, 3)), rng.random((700, 3))
+    tau = rng.standard_normal(20000)
+    K = ffmm.Laplace3D()
+    assert mrd(ffmm.direct(K, s, tau, t), O.direct(s, tau, t)) < 1e-13
+    g = ffmm.direct(K, s, tau, t, ffmm.Gradient)
+    assert mrd(g, O.direct(s, tau, t, grad=True)) < 1e-13
+    b = ffmm.direct(K, s, tau, t, ffmm.PotentialGradient)
+    assert np.allclose(b[:, 0], ffmm.direct(K, s, tau, t), rtol=1e-14, atol=0) and np.allclose(b[:, 1:], g, rtol=1e-14)
+    rows = np.arange(0, 20000, 37)
+    assert mrd(ffmm.direct_self(K, s, tau, rows=rows), O.direct_self(s, tau, rows=rows)) < 1e-13
 
 
 def test_analytic_charged_sphere():
