@@ -40,15 +40,17 @@ static void set_device(Context &ctx) { FFMM_HIP(hipSetDevice(ctx.device)); }
 
 // Stage timers record event pairs on the plan's stream without synchronising; the
 // times are read when asked for (every entry point has synchronised by then).  The
-// M2L stage keeps one pair per level.
+// per-level stages (M2L, the Hadamard kernel) keep one pair per level.
 static hipEvent_t &stage_event(Plan &p, int stage, int level, int which) {
-  if (stage != 4) return p.ev[2 * stage + which];
-  if ((int)p.m2l_ev.size() < 2 * (level + 1)) {
-    size_t old = p.m2l_ev.size();
-    p.m2l_ev.resize(2 * (level + 1), nullptr);
-    for (size_t k = old; k < p.m2l_ev.size(); k++) FFMM_HIP(hipEventCreate(&p.m2l_ev[k]));
+  const int sl = level_stage_slot(stage);
+  if (sl < 0) return p.ev[2 * stage + which];
+  std::vector<hipEvent_t> &v = p.lev_ev[sl];
+  if ((int)v.size() < 2 * (level + 1)) {
+    size_t old = v.size();
+    v.resize(2 * (level + 1), nullptr);
+    for (size_t k = old; k < v.size(); k++) FFMM_HIP(hipEventCreate(&v[k]));
   }
-  return p.m2l_ev[2 * level + which];
+  return v[2 * level + which];
 }
 StageTimer::StageTimer(Plan &p, int s, int l) : plan(p), stage(s), level(l) {
   FFMM_HIP(hipEventRecord(stage_event(plan, stage, level, 0), plan.ctx->stream));
@@ -58,9 +60,10 @@ StageTimer::~StageTimer() {
     (void)hipGetLastError();   // a failed record only loses the time: clear the error state
     return;
   }
-  if (stage == 4) {
-    if ((int)plan.m2l_recorded.size() <= level) plan.m2l_recorded.resize(level + 1, 0);
-    plan.m2l_recorded[level] = 1;
+  const int sl = level_stage_slot(stage);
+  if (sl >= 0) {
+    if ((int)plan.lev_recorded[sl].size() <= level) plan.lev_recorded[sl].resize(level + 1, 0);
+    plan.lev_recorded[sl][level] = 1;
   } else {
     plan.stage_recorded[stage] = true;
   }
@@ -343,8 +346,9 @@ int ffmm_plan_destroy(ffmm_plan plan) {
     (void)hipStreamSynchronize(plan->c.ctx->stream);
     for (auto &e : plan->c.ev)
       if (e) (void)hipEventDestroy(e);
-    for (auto &e : plan->c.m2l_ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto &v : plan->c.lev_ev)
+      for (auto &e : v)
+        if (e) (void)hipEventDestroy(e);
     delete plan;
   });
 }
@@ -447,7 +451,7 @@ int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted) {
       StageTimer tm(p, 2);
       upward_pass(p);
     }
-    std::fill(p.m2l_recorded.begin(), p.m2l_recorded.end(), 0);
+    for (auto &v : p.lev_recorded) std::fill(v.begin(), v.end(), 0);
     {
       StageTimer tm(p, 3);
       downward_pass(p);
@@ -515,12 +519,14 @@ int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n) {
     Plan &p = plan->c;
     set_device(*p.ctx);
     p.ctx->sync();
-    for (int i = 0; i < n && i < 8; i++) {
+    for (int i = 0; i < n && i < NSTAGES; i++) {
       float t = 0.f;
       double v = 0.0;
-      if (i == 4) {
-        for (size_t l = 0; l < p.m2l_recorded.size(); l++)
-          if (p.m2l_recorded[l] && hipEventElapsedTime(&t, p.m2l_ev[2 * l], p.m2l_ev[2 * l + 1]) == hipSuccess) v += t;
+      const int sl = level_stage_slot(i);
+      if (sl >= 0) {
+        const std::vector<hipEvent_t> &e = p.lev_ev[sl];
+        for (size_t l = 0; l < p.lev_recorded[sl].size(); l++)
+          if (p.lev_recorded[sl][l] && hipEventElapsedTime(&t, e[2 * l], e[2 * l + 1]) == hipSuccess) v += t;
       } else if (p.stage_recorded[i] && hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]) == hipSuccess) {
         v = t;
       }

@@ -18,6 +18,8 @@ constexpr int NCOLL = 27;
 constexpr int SELF_SLOT = 13;
 constexpr int NFAR = 316;
 constexpr int MAX_KEY_LEVELS = 21;   // 3 bits per level in a 63-bit key
+constexpr int NSTAGES = 9;           // timed stages (ffmm_plan_stage_times)
+inline int level_stage_slot(int stage) { return stage == 4 ? 0 : stage == 8 ? 1 : -1; }
 
 struct Error : std::runtime_error {
   int code;
@@ -193,10 +195,12 @@ struct Plan {
   DBuf<double> tgt_sorted;
   DBuf<int32_t> tgt_node, tgt_leafslot, tgt_order, tgt_ptr;
   DBuf<uint32_t> tgt_keys;
-  hipEvent_t ev[16] = {};
-  std::vector<hipEvent_t> m2l_ev;   // per level: start, end
-  std::vector<char> m2l_recorded;   // per level: both events recorded by the last pass
-  bool stage_recorded[8] = {};
+  // stage timers: stages 0-7 one event pair each; the per-level stages (4: M2L,
+  // 8: the Hadamard kernel of the spectral M2L) one pair per level
+  hipEvent_t ev[2 * NSTAGES] = {};
+  std::vector<hipEvent_t> lev_ev[2];     // per level: start, end
+  std::vector<char> lev_recorded[2];     // per level: both events recorded by the last pass
+  bool stage_recorded[NSTAGES] = {};
   bool expansions_ready = false;
 };
 

@@ -188,7 +188,8 @@ void build_lists(Plan &plan) {
       k_pair_fill<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, li.ptr.p, li.src.p, li.far.p);
       FFMM_LAUNCH_CHECK();
     }
-    // the class lists
+    // the class lists (only the dense class translations read them)
+    if (plan.m2l != FFMM_M2L_DENSE) continue;
     int64_t ne = (int64_t)nclass * nl;
     ccnt.resize(ne);
     cstart.resize(ne);

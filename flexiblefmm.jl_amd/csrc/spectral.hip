@@ -360,6 +360,7 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   const int gpw = 4;
   const int64_t gpb = (HD_TB / 64) * gpw;
   dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
+  StageTimer tm(plan, 8, l);
   k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p, ng,
                                                gpw);
   FFMM_LAUNCH_CHECK();

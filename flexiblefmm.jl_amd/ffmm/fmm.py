@@ -415,9 +415,9 @@ class DevicePlan:
         return M.T, Pl.T, Lc.T
 
     def stage_times(self) -> dict:
-        ms = (ctypes.c_double * 8)()
-        call("ffmm_plan_stage_times", self._h, ms, 8)
-        names = ["tree", "lists", "upward", "downward", "m2l", "eval_sources", "eval_targets", "gather"]
+        ms = (ctypes.c_double * 9)()
+        call("ffmm_plan_stage_times", self._h, ms, 9)
+        names = ["tree", "lists", "upward", "downward", "m2l", "eval_sources", "eval_targets", "gather", "hadamard"]
         return dict(zip(names, ms[:]))
 
     # -- the passes (device tensors in and out) --

@@ -148,9 +148,12 @@ int ffmm_matvec(ffmm_plan plan, const double *tau, int sorted, double *out, int 
  * order of the leaves level by level). */
 int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *plocals, double *leaf_locals);
 
-/* Per-stage device times of the last calls, in milliseconds (hipEvent pairs):
- * [0] tree, [1] lists, [2] upward, [3] downward, [4] M2L (within downward),
- * [5] evaluation at sources, [6] evaluation at targets, [7] input gather. */
+/* Per-stage device times of the last calls, in milliseconds (hipEvent pairs on the
+ * plan's stream): [0] tree, [1] lists, [2] upward, [3] downward, [4] M2L (within
+ * downward), [5] evaluation at sources, [6] evaluation at targets, [7] input gather,
+ * [8] the Hadamard kernel of the spectral M2L (within [4]).  No reference counterpart:
+ * the reference's verbose plan build prints host-side stage times
+ * (src/device_passes.jl:331-336). */
 int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n);
 
 #ifdef __cplusplus
