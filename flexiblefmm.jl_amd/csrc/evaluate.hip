@@ -45,23 +45,6 @@ __device__ __forceinline__ void accumulate_masked(double d0, double d1, double d
   }
 }
 
-struct EvalArgs {
-  const int32_t *leaf_nodes;   // per leaf slot
-  const int32_t *tptr;         // targets of leaf slot s: [tptr[s], tptr[s+1]); null: the leaf's own sources
-  const double *tx, *ty, *tz;  // targets, grouped by leaf
-  const int32_t *outidx;       // output position of each target
-  const double *Lc;            // Nl x nleaves
-  int Nl;
-  const double *surf;          // 3 x Nl SoA at the reference width
-  const double *lam;
-  const int32_t *level;
-  const double *centers;
-  int64_t nn;
-  const int32_t *col, *plo, *phi;
-  const double *sx, *sy, *sz, *tau;
-  double *out;
-};
-
 template <int OUT>
 __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
   const int s = blockIdx.x;
@@ -131,6 +114,10 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
 static void launch_eval(Plan &plan, EvalArgs &a, int output) {
   const int64_t nleaves = plan.leaf_off[plan.L];
   if (nleaves == 0) return;
+  if (plan.ops->method == METHOD_HARMONIC) {
+    launch_eval_harmonic(plan, a, output);
+    return;
+  }
   hipStream_t s = plan.ctx->stream;
   if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
   else if (output == FFMM_GRADIENT) k_eval<FFMM_GRADIENT><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
@@ -156,6 +143,14 @@ static EvalArgs base_args(Plan &plan, double *out) {
   a.sz = plan.sz.p;
   a.tau = plan.tau.p;
   a.out = out;
+  if (plan.ops->method == METHOD_HARMONIC) {
+    a.tb = plan.ops->tab();
+    a.p = plan.ops->p;
+    a.nd = plan.nd;
+    a.hscale = plan.hscale.p;
+    a.khat = plan.khat.p;
+    a.kappa = plan.ops->kappa;
+  }
   return a;
 }
 

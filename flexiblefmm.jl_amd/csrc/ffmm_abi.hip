@@ -91,7 +91,7 @@ using namespace ffmm;
 extern "C" {
 
 const char *ffmm_last_error(void) { return g_last_error.c_str(); }
-int ffmm_abi_version(void) { return 1; }
+int ffmm_abi_version(void) { return 2; }
 
 int ffmm_context_create(int device, void *stream, ffmm_context *out) {
   return guard([&] {
@@ -196,7 +196,8 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     if (!(w_ref > 0)) fail(FFMM_ERR_ARGUMENT, "the reference width must be positive");
     const int Ns = o.Ns, Nl = o.Nl;
     if (r_up < 1 || r_up > Ns || r_dn < 1 || r_dn > Ns) fail(FFMM_ERR_ARGUMENT, "pseudo-inverse ranks out of range");
-    if (nclass < 1 || nclass > 32) fail(FFMM_ERR_ARGUMENT, "nclass out of range");
+    if (o.method != METHOD_CUBE) fail(FFMM_ERR_ARGUMENT, "the cache belongs to a harmonic method");
+    if (nclass < 1 || nclass > MAX_CLASSES) fail(FFMM_ERR_ARGUMENT, "nclass out of range");
     if (!small || !large || !Uh_up || !invS_up || !V_up || !Uh_dn || !invS_dn || !V_dn || !M2M || !L2L ||
         !class_ops || !far_class || !idx || !sgn)
       fail(FFMM_ERR_ARGUMENT, "null operator");
@@ -242,6 +243,7 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     o.idx.upload(idx, (size_t)NFAR * Ns, s);
     o.inv.upload(inv.data(), inv.size(), s);
     o.sgn.upload(sgn, (size_t)NFAR * Ns, s);
+    o.sgn_t.upload(sgn, (size_t)NFAR * Ns, s);   // the cube lattices: one permutation for both sides
     o.embed.upload(o.embed_h.data(), o.embed_h.size(), s);
     {
       // the p^3-corner index of each surface point, and the far index of every
@@ -283,6 +285,165 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
   });
 }
 
+int ffmm_opcache_create_harmonic(ffmm_context ctx, int kernel, int p, double kappa, double w_ref, ffmm_opcache *out) {
+  return guard([&] {
+    if (!ctx || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (kernel != FFMM_KERNEL_LAPLACE && kernel != FFMM_KERNEL_YUKAWA && kernel != FFMM_KERNEL_STOKESLET)
+      fail(FFMM_ERR_ARGUMENT, "unknown kernel");
+    if (p < 1 || p > 60) fail(FFMM_ERR_ARGUMENT, "p must lie in 1:60");
+    if (!(w_ref > 0) || !std::isfinite(w_ref)) fail(FFMM_ERR_ARGUMENT, "the reference width must be positive");
+    if (kernel == FFMM_KERNEL_YUKAWA && !(kappa >= 0 && std::isfinite(kappa)))
+      fail(FFMM_ERR_ARGUMENT, "kappa must be finite and non-negative");
+    set_device(ctx->c);
+    auto h = std::make_unique<ffmm_opcache_s>();
+    OpCache &o = h->c;
+    o.ctx = &ctx->c;
+    o.method = METHOD_HARMONIC;
+    o.kernel = kernel;
+    o.p = p;
+    o.hN = (p + 1) * (p + 1);
+    o.nb = kernel == FFMM_KERNEL_STOKESLET ? 4 : 1;
+    o.Ns = o.Nl = o.nb * o.hN;
+    o.kappa = kernel == FFMM_KERNEL_YUKAWA ? kappa : 0.0;
+    o.w_ref = w_ref;
+    // HarmonicTables (src/operators/harmonics.jl:163-187): a, b, cz, cm, cp at the
+    // triangular indices, e_m = sqrt((2m - 1) / 2m)
+    const int nt = (p + 1) * (p + 2) / 2;
+    o.ntri = nt;
+    std::vector<double> tab(5 * (size_t)nt + p, 0.0);
+    double *a = tab.data(), *b = a + nt, *cz = b + nt, *cm = cz + nt, *cp = cm + nt, *e = cp + nt;
+    for (int n = 0; n <= p; n++)
+      for (int m = 0; m <= n; m++) {
+        const int t = n * (n + 1) / 2 + m;
+        const double c = std::sqrt((double)((n - m) * (n + m)));
+        cz[t] = c;
+        if (n > m) {
+          a[t] = (2 * n - 1) / c;
+          b[t] = std::sqrt((double)((n - m - 1) * (n + m - 1))) / c;
+        }
+        cm[t] = n + m >= 2 ? std::sqrt((double)((n + m) * (n + m - 1))) : 0.0;
+        cp[t] = n - m >= 2 ? std::sqrt((double)((n - m) * (n - m - 1))) : 0.0;
+      }
+    for (int m = 1; m <= p; m++) e[m - 1] = std::sqrt((2.0 * m - 1) / (2.0 * m));
+    o.htab.upload(tab.data(), tab.size(), ctx->c.stream);
+    // the far offsets in far-index order (far_offsets, src/util.jl:87-97), for the Stokes coupling
+    std::vector<double> offs;
+    for (int u = 0; u < 343; u++) {
+      int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
+      if (std::max(std::abs(x), std::max(std::abs(y), std::abs(z))) >= 2) {
+        offs.push_back(x);
+        offs.push_back(y);
+        offs.push_back(z);
+      }
+    }
+    o.offs.upload(offs.data(), offs.size(), ctx->c.stream);
+    ctx->c.sync();
+    *out = h.release();
+  });
+}
+
+int ffmm_opcache_put_harmonic_classes(ffmm_opcache cache, int nclass, const int32_t *far_class, const int32_t *idx,
+                                      const int8_t *sgn, const int32_t *idx_t, const int8_t *sgn_t) {
+  return guard([&] {
+    if (!cache) fail(FFMM_ERR_ARGUMENT, "null cache");
+    OpCache &o = cache->c;
+    if (o.method != METHOD_HARMONIC) fail(FFMM_ERR_ARGUMENT, "the cache belongs to the cube method");
+    if (!far_class || !idx || !sgn || !idx_t || !sgn_t) fail(FFMM_ERR_ARGUMENT, "null argument");
+    if (nclass < 1 || nclass > MAX_CLASSES) fail(FFMM_ERR_ARGUMENT, "nclass out of range");
+    Context &ctx = *o.ctx;
+    set_device(ctx);
+    const int N = o.hN;
+    for (int f = 0; f < NFAR; f++)
+      if (far_class[f] < 0 || far_class[f] >= nclass) fail(FFMM_ERR_ARGUMENT, "far offset class out of range");
+    std::vector<int32_t> inv((size_t)NFAR * N, -1);
+    std::vector<char> seen((size_t)NFAR * N, 0);
+    for (int f = 0; f < NFAR; f++)
+      for (int i = 0; i < N; i++) {
+        int32_t k = idx[(size_t)f * N + i], kt = idx_t[(size_t)f * N + i];
+        if (k < 0 || k >= N || seen[(size_t)f * N + k]) fail(FFMM_ERR_ARGUMENT, "idx is not a permutation");
+        seen[(size_t)f * N + k] = 1;
+        if (kt < 0 || kt >= N || inv[(size_t)f * N + kt] >= 0) fail(FFMM_ERR_ARGUMENT, "idx_t is not a permutation");
+        inv[(size_t)f * N + kt] = i;
+        int8_t s1 = sgn[(size_t)f * N + i], s2 = sgn_t[(size_t)f * N + i];
+        if ((s1 != 1 && s1 != -1) || (s2 != 1 && s2 != -1)) fail(FFMM_ERR_ARGUMENT, "signs must be +1 or -1");
+      }
+    hipStream_t s = ctx.stream;
+    o.nclass = nclass;
+    o.far_class_h.assign(far_class, far_class + NFAR);
+    o.far_class.upload(far_class, NFAR, s);
+    o.idx.upload(idx, (size_t)NFAR * N, s);
+    o.sgn.upload(sgn, (size_t)NFAR * N, s);
+    o.inv.upload(inv.data(), inv.size(), s);
+    o.sgn_t.upload(sgn_t, (size_t)NFAR * N, s);
+    ctx.sync();
+    o.has_classes = true;
+    o.ready = true;
+  });
+}
+
+int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const double *M2M, const double *L2L,
+                                    const double *reps) {
+  return guard([&] {
+    if (!cache) fail(FFMM_ERR_ARGUMENT, "null cache");
+    OpCache &o = cache->c;
+    if (o.method != METHOD_HARMONIC) fail(FFMM_ERR_ARGUMENT, "the cache belongs to the cube method");
+    if (!o.has_classes) fail(FFMM_ERR_STATE, "put the classes first");
+    if (!M2M || !L2L || !reps) fail(FFMM_ERR_ARGUMENT, "null operator");
+    if (!(width > 0) || !std::isfinite(width)) fail(FFMM_ERR_ARGUMENT, "the width must be positive");
+    Context &ctx = *o.ctx;
+    set_device(ctx);
+    hipStream_t s = ctx.stream;
+    const int N = o.hN, NB = o.Ns;
+    auto lv = std::make_unique<HarmonicLevel>();
+    lv->w = width;
+    if (o.nb == 1) {
+      lv->M2M.upload(M2M, (size_t)N * NCHILD * N, s);
+      lv->L2L.upload(L2L, (size_t)NCHILD * N * N, s);
+    } else {
+      // _stokes_dense_operators (src/operators/harmonics_stokes.jl:112-131): for child j
+      // with centre offset d (in parent widths), blocks a = 1..3 of the parent get A_j M_a
+      // and the fourth A_j (M_4 / 2 + d . M); L2L mirrors it with B_j (2 L_4 - 2 d . L)
+      std::vector<double> Md((size_t)NB * NCHILD * NB, 0.0), Ld((size_t)NCHILD * NB * NB, 0.0);
+      const int ldm = NB, ldl = NCHILD * NB;
+      for (int j = 0; j < NCHILD; j++) {
+        const double dh[3] = {((j & 1) - 0.5) / 2, (((j >> 1) & 1) - 0.5) / 2, (((j >> 2) & 1) - 0.5) / 2};
+        const int cj = j * NB;
+        for (int c = 0; c < N; c++)
+          for (int r = 0; r < N; r++) {
+            const double A = M2M[(size_t)(j * N + c) * N + r];           // M2M[r, jN + c]
+            const double B = L2L[(size_t)c * (NCHILD * N) + j * N + r];  // L2L[jN + r, c]
+            for (int a = 0; a < 3; a++) {
+              Md[(size_t)(cj + a * N + c) * ldm + a * N + r] = A;
+              Ld[(size_t)(a * N + c) * ldl + cj + a * N + r] = B;
+              Md[(size_t)(cj + a * N + c) * ldm + 3 * N + r] = dh[a] * A;
+              Ld[(size_t)(a * N + c) * ldl + cj + 3 * N + r] = (-2 * dh[a]) * B;
+            }
+            Md[(size_t)(cj + 3 * N + c) * ldm + 3 * N + r] = A / 2;
+            Ld[(size_t)(3 * N + c) * ldl + cj + 3 * N + r] = 2 * B;
+          }
+      }
+      lv->M2M.upload(Md.data(), Md.size(), s);
+      lv->L2L.upload(Ld.data(), Ld.size(), s);
+    }
+    lv->reps.upload(reps, (size_t)o.nclass * N * N, s);
+    ctx.sync();
+    for (auto &h : o.hlev)
+      if (h->w == width) {
+        h = std::move(lv);
+        return;
+      }
+    o.hlev.push_back(std::move(lv));
+  });
+}
+
+int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has) {
+  return guard([&] {
+    if (!cache || !has) fail(FFMM_ERR_ARGUMENT, "null argument");
+    *has = cache->c.find_width(width) != nullptr;
+  });
+}
+
 int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, const double origin[3], double width,
                      int e0, int max_levels, int precision, int m2l, ffmm_plan *out) {
   return guard([&] {
@@ -296,6 +457,12 @@ int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, 
     if (!(width > 0)) fail(FFMM_ERR_ARGUMENT, "box width must be positive");
     if (precision != FFMM_PREC_F64 && precision != FFMM_PREC_MIXED) fail(FFMM_ERR_ARGUMENT, "unknown precision");
     if (m2l != FFMM_M2L_DENSE && m2l != FFMM_M2L_SPECTRAL) fail(FFMM_ERR_ARGUMENT, "unknown M2L form");
+    if (o.method == METHOD_HARMONIC) {
+      if (m2l != FFMM_M2L_DENSE) fail(FFMM_ERR_ARGUMENT, "the harmonic methods take the dense class M2L");
+      if (precision != FFMM_PREC_F64) fail(FFMM_ERR_ARGUMENT, "the harmonic methods run in FP64");
+      if (o.kernel != FFMM_KERNEL_YUKAWA && !o.find_width(o.w_ref))
+        fail(FFMM_ERR_STATE, "the cache holds no operators of the reference width");
+    }
     if (m2l == FFMM_M2L_SPECTRAL && !o.has_tables) fail(FFMM_ERR_ARGUMENT, "the cache holds no spectral tables");
     if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
     Context &ctx = *o.ctx;
@@ -320,6 +487,25 @@ int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, 
     p.lam.resize(p.L);
     for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, e0 - l);
     p.lam_d.upload(p.lam.data(), p.L, ctx.stream);
+    if (o.method == METHOD_HARMONIC) {
+      // per level: 1/w_l and kappa w_l (the extras' scale and kappa-hat, src/operators/harmonics.jl:327-336),
+      // the M2L factor (rescale, src/operators/harmonics.jl:393-398; 1 for Yukawa, whose
+      // operators are built per width), and the operators of the level's width
+      std::vector<double> hs(p.L), kh(p.L);
+      p.alpha.assign(p.L, 1.0);
+      p.hl.assign(p.L, nullptr);
+      for (int l = 0; l < p.L; l++) {
+        const double w = std::ldexp(width, -l);
+        hs[l] = 1.0 / w;
+        kh[l] = o.kappa * w;
+        if (o.kernel != FFMM_KERNEL_YUKAWA) {
+          p.alpha[l] = 1.0 / p.lam[l];
+          p.hl[l] = o.find_width(o.w_ref);
+        }
+      }
+      p.hscale.upload(hs.data(), p.L, ctx.stream);
+      p.khat.upload(kh.data(), p.L, ctx.stream);
+    }
     {
       // node -> leaf slot
       std::vector<int32_t> slot(p.nnodes, -1), leaves(p.leaf_off[p.L]);
@@ -439,10 +625,30 @@ int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *s
 }
 
 int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted) {
+  return ffmm_build_expansions_nd(plan, tau, 1, sorted);
+}
+
+int ffmm_build_expansions_nd(ffmm_plan plan, const double *tau, int nd, int sorted) {
   return guard([&] {
     if (!plan || !tau) fail(FFMM_ERR_ARGUMENT, "null argument");
     Plan &p = plan->c;
+    OpCache &o = *p.ops;
+    const bool stokes = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET;
+    if (stokes && nd != 3 && nd != 9)
+      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
+    if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
     set_device(*p.ctx);
+    if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {
+      // the operators of every level with a far field or a translation (2 .. L-1)
+      for (int l = 2; l < p.L; l++) {
+        const double w = std::ldexp(p.W0, -l);
+        p.hl[l] = o.find_width(w);
+        if (!p.hl[l])
+          fail(FFMM_ERR_STATE, "the cache holds no operators for boxes of width " + std::to_string(w) + " (level " +
+                                   std::to_string(l) + ")");
+      }
+    }
+    p.nd = nd;
     {
       StageTimer tm(p, 7);
       gather_inputs(p, tau, sorted);
@@ -467,6 +673,8 @@ int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output) {
     Plan &p = plan->c;
     if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
     if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
+      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
     set_device(*p.ctx);
     {
       StageTimer tm(p, 5);
@@ -482,6 +690,8 @@ int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out
     Plan &p = plan->c;
     if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
     if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
+      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
     if (nt < 0) fail(FFMM_ERR_DIMENSION, "negative target count");
     if (nt == 0) return;
     if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");

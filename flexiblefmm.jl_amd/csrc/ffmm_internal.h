@@ -5,6 +5,7 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -17,6 +18,7 @@ constexpr int NCHILD = 8;
 constexpr int NCOLL = 27;
 constexpr int SELF_SLOT = 13;
 constexpr int NFAR = 316;
+constexpr int MAX_CLASSES = 64;      // symmetry classes of an M2L (16 cube, 34 harmonic)
 constexpr int MAX_KEY_LEVELS = 21;   // 3 bits per level in a 63-bit key
 constexpr int NSTAGES = 9;           // timed stages (ffmm_plan_stage_times)
 inline int level_stage_slot(int stage) { return stage == 4 ? 0 : stage == 8 ? 1 : -1; }
@@ -123,9 +125,44 @@ struct Context {
   void readback(int64_t *host, const int64_t *dev, int n);
 };
 
+// the methods of an operator cache
+constexpr int METHOD_CUBE = 0;       // CubeKernelIndependent{3}(Laplace3D())
+constexpr int METHOD_HARMONIC = 1;   // SphericalHarmonics (Laplace3D, Yukawa3D), StokesHarmonics
+
+// the operators of the harmonic methods for boxes of one width
+struct HarmonicLevel {
+  double w = 0.0;
+  DBuf<double> M2M, L2L;   // NB x 8NB, 8NB x NB (NB = nb N; the dense Stokes forms for nb = 4)
+  DBuf<double> reps;       // nclass x (N x N)
+};
+
+// device pointers of the recurrence tables (HarmonicTables, src/operators/harmonics.jl:163-187)
+struct HTab {
+  const double *a, *b, *cz, *cm, *cp, *e;
+};
+
 struct OpCache {
   Context *ctx = nullptr;
+  int method = METHOD_CUBE;
   int p = 8, pc = 16, Ns = 0, Nl = 0, nclass = 0, r_up = 0, r_dn = 0;
+  // harmonic methods: kernel, degree, scalar block length N = (p+1)^2 and blocks per
+  // expansion nb (Ns = Nl = nb N)
+  int kernel = FFMM_KERNEL_LAPLACE, hN = 0, nb = 1, ntri = 0;
+  double kappa = 0.0;
+  bool has_classes = false;
+  DBuf<double> htab;                         // a, b, cz, cm, cp (ntri each), e (p)
+  DBuf<int8_t> sgn_t;                        // target-side signs [f * N + i] (the cube: = sgn)
+  DBuf<double> offs;                         // 316 x 3 far offsets (the Stokes coupling)
+  std::vector<std::unique_ptr<HarmonicLevel>> hlev;
+  const HarmonicLevel *find_width(double w) const {
+    for (auto &h : hlev)
+      if (h->w == w) return h.get();
+    return nullptr;
+  }
+  HTab tab() const {
+    const double *t = htab.p;
+    return HTab{t, t + ntri, t + 2 * ntri, t + 3 * ntri, t + 4 * ntri, t + 5 * ntri};
+  }
   double d = 0.1, w_ref = 0.0;
   bool ready = false;
   bool has_tables = false;
@@ -167,6 +204,12 @@ struct Plan {
   std::vector<int64_t> lo, nl;
   std::vector<double> lam;   // level width / reference width
   DBuf<double> lam_d;
+  // harmonic methods: the operators of each level, the M2L factor of each level, and
+  // per level 1/w_l and kappa w_l on the device; density components per point
+  std::vector<const HarmonicLevel *> hl;
+  std::vector<double> alpha;
+  DBuf<double> hscale, khat;
+  int nd = 1;
   DBuf<int32_t> node_leafslot;   // node -> position in leaf_nodes, -1 if not a real leaf
   // the flat tree
   DBuf<int32_t> cell;        // 3 x n SoA
@@ -238,6 +281,34 @@ void direct(Context &ctx, const double *src, const double *tau, int64_t ns, cons
 // ---- evaluation (evaluate.hip) ----
 void evaluate_at_sources(Plan &plan, double *out, int output);
 void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output);
+
+// the arguments of the evaluation kernels: per leaf slot, its targets (or its own
+// sources), the far field of its leaf and the near field of its colleagues
+struct EvalArgs {
+  const int32_t *leaf_nodes;   // per leaf slot
+  const int32_t *tptr;         // targets of leaf slot s: [tptr[s], tptr[s+1]); null: the leaf's own sources
+  const double *tx, *ty, *tz;  // targets, grouped by leaf
+  const int32_t *outidx;       // output position of each target
+  const double *Lc;            // Nl x nleaves
+  int Nl;
+  const double *surf;          // cube: 3 x Nl SoA at the reference width
+  const double *lam;
+  const int32_t *level;
+  const double *centers;
+  int64_t nn;
+  const int32_t *col, *plo, *phi;
+  const double *sx, *sy, *sz, *tau;
+  double *out;
+  // harmonic methods
+  HTab tb;
+  int p, nd;
+  const double *hscale, *khat;
+  double kappa;
+};
+
+// ---- harmonic methods (harmonic.hip) ----
+void harmonic_s2m_level(Plan &plan, int l);
+void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output);
 
 // ---- timing ----
 struct StageTimer {

@@ -1,0 +1,531 @@
+// The harmonic methods on the device: the multipoles of the childless nodes (S2M)
+// and the evaluation (local expansion of the leaf plus the near field of its
+// colleagues) for SphericalHarmonics with Laplace3D or Yukawa3D charges and for
+// StokesHarmonics with stokeslets and stresslets.
+//
+// S2M: one wave per node; lane (m, r) runs the order-m chain of the Schmidt
+// recurrences (_fold_chain, src/operators/harmonics.jl:681-704) for the sources
+// r, r + R, ... of the node (R = 64 / next_pow2(p + 1) replicas) and owns the entries
+// of order m in replica r's column in LDS; the replicas are summed at the end
+// (_s2m_harmonic_kernel!, src/device_passes.jl:343-373, which runs one work-item per
+// node and order).
+//
+// Evaluation: one wave per leaf; as in the cube evaluation, lanes take targets and,
+// when a leaf has fewer than 64, R replicas of each target split the orders of the
+// far field and the colleagues' sources, summed by xor shuffles
+// (_evaluate_harmonic_kernel!, src/device.jl:186-206).
+#include "ffmm_internal.h"
+
+namespace ffmm {
+
+constexpr int HB = 64;
+constexpr double HC4PI = 0.07957747154594767;    // 1 / (4 pi)
+constexpr double HC8PI = 0.039788735772973836;   // 1 / (8 pi)
+
+__device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
+
+// f(n, Re, Im) of the Schmidt regular harmonics of order m, n = m..nmax, at (x, y, z);
+// the sectoral start is the product of m factors
+template <class F>
+__device__ __forceinline__ void chain(const HTab &tb, double x, double y, double z, int m, int nmax, F &&f) {
+  if (m > nmax) return;
+  const double r2 = x * x + y * y + z * z;
+  double sr = 1.0, si = 0.0;
+  for (int k = 1; k <= m; k++) {
+    const double e = tb.e[k - 1];
+    const double tr = -e * (x * sr - y * si), ti = -e * (x * si + y * sr);
+    sr = tr;
+    si = ti;
+  }
+  f(m, sr, si);
+  double pr = 0.0, pi = 0.0, cr = sr, ci = si;
+  for (int n = m + 1; n <= nmax; n++) {
+    const int t = tri(n, m);
+    const double az = tb.a[t] * z, br = tb.b[t] * r2;
+    const double nr = az * cr - br * pr, ni = az * ci - br * pi;
+    f(n, nr, ni);
+    pr = cr;
+    pi = ci;
+    cr = nr;
+    ci = ni;
+  }
+}
+
+// u_n(z) for n = 0..nmax from z^2: Miller's backward recurrence normalised with
+// u_0 = sinh z / z (src/operators/harmonics_yukawa.jl:43-63, 275-297)
+__device__ void yukawa_u(double z2, int nmax, double *u) {
+  if (z2 < 1e-8) {
+    for (int n = 0; n <= nmax; n++) u[n] = 1 + z2 / (2 * (2 * n + 3));
+    return;
+  }
+  const double z = sqrt(z2);
+  const int cz = (int)ceil(z);
+  const int N = (nmax > cz ? nmax : cz) + 25;
+  double a = 0.0, b = 1.0;
+  for (int n = N - 1; n >= 0; n--) {
+    const double c = b + z2 * a / ((2.0 * n + 3) * (2.0 * n + 5));
+    if (n <= nmax) u[n] = c;
+    a = b;
+    b = c;
+  }
+  const double s = (sinh(z) / z) / b;
+  for (int n = 0; n <= nmax; n++) u[n] *= s;
+}
+
+__device__ __forceinline__ void acc_entry(double *col, int n, int m, double re, double im) {
+  if (m == 0) {
+    col[n * n] += re;
+  } else {
+    col[n * n + 2 * m - 1] += re;
+    col[n * n + 2 * m] += im;
+  }
+}
+
+// the entries of order m of a dipole d at x (folds to p - 1 of the chains m - 1, m,
+// m + 1): _accumulate_order! (src/operators/harmonics.jl:718-752)
+template <int ND>
+__device__ __forceinline__ void dipole_order(const HTab &tb, double x, double y, double z, int p, int m,
+                                             const double (&d)[ND][3], double *col, int N) {
+  if (m >= 1)
+    chain(tb, x, y, z, m - 1, p - 1, [&](int n1, double ur, double ui) {
+      const int n = n1 + 1;
+      const double c = tb.cm[tri(n, m)] / 2;
+#pragma unroll
+      for (int k = 0; k < ND; k++)
+        acc_entry(col + k * N, n, m, -c * (d[k][0] * ur - d[k][1] * ui), c * (d[k][0] * ui + d[k][1] * ur));
+    });
+  if (m <= p - 1)
+    chain(tb, x, y, z, m, p - 1, [&](int n1, double ur, double ui) {
+      const int n = n1 + 1;
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        const double c = tb.cz[tri(n, m)] * d[k][2];
+        acc_entry(col + k * N, n, m, c * ur, -c * ui);
+      }
+    });
+  if (m + 1 <= p - 1)
+    chain(tb, x, y, z, m + 1, p - 1, [&](int n1, double ur, double ui) {
+      const int n = n1 + 1;
+      if (m >= 1) {
+        const double c = tb.cp[tri(n, m)] / 2;
+#pragma unroll
+        for (int k = 0; k < ND; k++)
+          acc_entry(col + k * N, n, m, c * (d[k][0] * ur + d[k][1] * ui), c * (d[k][1] * ur - d[k][0] * ui));
+      } else {
+        const double c = tb.cp[tri(n, 0)];
+#pragma unroll
+        for (int k = 0; k < ND; k++) acc_entry(col + k * N, n, 0, c * (d[k][0] * ur + d[k][1] * ui), 0.0);
+      }
+    });
+}
+
+// KIND: 0 Laplace3D charges, 1 Yukawa3D charges, 2 Stokeslet3D (nd = 3 or 9)
+template <int KIND>
+__global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nodes, const double *__restrict__ centers,
+                                                 int64_t nn, const int32_t *__restrict__ level,
+                                                 const double *__restrict__ hscale, const double *__restrict__ khat,
+                                                 const int32_t *__restrict__ plo, const int32_t *__restrict__ phi,
+                                                 const double *__restrict__ sx, const double *__restrict__ sy,
+                                                 const double *__restrict__ sz, const double *__restrict__ dens, int nd,
+                                                 HTab tb, int p, int MP, double *__restrict__ Mp) {
+  extern __shared__ double sm[];
+  const int N = (p + 1) * (p + 1);
+  constexpr int nb = KIND == 2 ? 4 : 1;
+  const int NB = nb * N;
+  const int R = HB / MP;
+  double *acc = sm;               // R columns of NB
+  double *uw = sm + R * NB;       // R x (p + 2) radial weights (Yukawa)
+  const int t = threadIdx.x, m = t % MP, r = t / MP;
+  const int node = nodes[blockIdx.x];
+  for (int i = t; i < R * NB; i += HB) acc[i] = 0.0;
+  const double c0 = centers[node], c1 = centers[nn + node], c2 = centers[2 * nn + node];
+  const int lv = level[node];
+  const double s = hscale[lv];
+  const int32_t j0 = plo[node], j1 = phi[node];
+  double *col = acc + r * NB;
+  __syncthreads();
+  for (int32_t jb = j0; jb <= j1; jb += R) {
+    const int32_t j = jb + r;
+    const bool valid = j <= j1;
+    const double x = valid ? (sx[j] - c0) * s : 0.0;
+    const double y = valid ? (sy[j] - c1) * s : 0.0;
+    const double z = valid ? (sz[j] - c2) * s : 0.0;
+    if (KIND == 1) {
+      if (valid && m == 0) {
+        const double kh = khat[lv];
+        yukawa_u(kh * kh * (x * x + y * y + z * z), p + 1, uw + r * (p + 2));
+      }
+      __syncthreads();
+    }
+    if (valid && m <= p) {
+      if (KIND == 0) {
+        const double q = dens[j];
+        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) { acc_entry(col, n, m, q * vr, -q * vi); });
+      } else if (KIND == 1) {
+        const double q = dens[j];
+        const double *u = uw + r * (p + 2);
+        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) {
+          const double qq = q * u[n];
+          acc_entry(col, n, m, qq * vr, -qq * vi);
+        });
+      } else {
+        const double *f = dens + (int64_t)nd * j;
+        const double f0 = f[0], f1 = f[1], f2 = f[2];
+        const double fy = x * f0 + y * f1 + z * f2;
+        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) {
+          acc_entry(col, n, m, f0 * vr, -f0 * vi);
+          acc_entry(col + N, n, m, f1 * vr, -f1 * vi);
+          acc_entry(col + 2 * N, n, m, f2 * vr, -f2 * vi);
+          acc_entry(col + 3 * N, n, m, fy * vr, -fy * vi);
+        });
+        if (nd == 9) {
+          // the stresslet (g, nu): Phi_k from Dipole(-(g_k nu + nu_k g) s), Psi from
+          // Dipole(-((nu.y) g + (g.y) nu) s) (src/operators/harmonics_stokes.jl:204-211)
+          const double g[3] = {f[3], f[4], f[5]}, nu[3] = {f[6], f[7], f[8]};
+          double d[4][3];
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) d[k][c] = -(g[k] * nu[c] + nu[k] * g[c]) * s;
+          const double nuy = nu[0] * x + nu[1] * y + nu[2] * z;
+          const double gy = g[0] * x + g[1] * y + g[2] * z;
+#pragma unroll
+          for (int c = 0; c < 3; c++) d[3][c] = -(nuy * g[c] + gy * nu[c]) * s;
+          dipole_order<4>(tb, x, y, z, p, m, d, col, N);
+        }
+      }
+    }
+    if (KIND == 1) __syncthreads();
+  }
+  __syncthreads();
+  for (int i = t; i < NB; i += HB) {
+    double v = 0.0;
+    for (int rr = 0; rr < R; rr++) v += acc[rr * NB + i];
+    Mp[(int64_t)node * NB + i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// evaluation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void get_entry(const double *L, int n, int m, double &re, double &im) {
+  if (m > n) {
+    re = 0.0;
+    im = 0.0;
+  } else if (m == 0) {
+    re = L[n * n];
+    im = 0.0;
+  } else {
+    re = L[n * n + 2 * m - 1];
+    im = L[n * n + 2 * m];
+  }
+}
+// complex coefficient (n, m) of a block, any sign of m, zero outside the triangle
+__device__ __forceinline__ void cget(const double *L, int p, int n, int m, double &re, double &im) {
+  if (n > p || abs(m) > n) {
+    re = 0.0;
+    im = 0.0;
+  } else if (m == 0) {
+    re = L[n * n];
+    im = 0.0;
+  } else if (m > 0) {
+    re = L[n * n + 2 * m - 1];
+    im = L[n * n + 2 * m];
+  } else {
+    const int k = -m;
+    const double sg = (k & 1) ? -1.0 : 1.0;
+    re = sg * L[n * n + 2 * k - 1];
+    im = -sg * L[n * n + 2 * k];
+  }
+}
+__device__ __forceinline__ double hcz(const HTab &tb, int n, int m) { return abs(m) > n ? 0.0 : tb.cz[tri(n, abs(m))]; }
+__device__ __forceinline__ double hcm(const HTab &tb, int n, int m) {
+  if (abs(m) > n) return 0.0;
+  return m >= 0 ? tb.cm[tri(n, m)] : tb.cp[tri(n, -m)];
+}
+__device__ __forceinline__ double hcp(const HTab &tb, int n, int m) {
+  if (abs(m) > n) return 0.0;
+  return m >= 0 ? tb.cp[tri(n, m)] : tb.cm[tri(n, -m)];
+}
+// the scaled-gradient coefficient sets at (n1, m1) of a block: _dcoef
+// (src/operators/harmonics_stokes.jl:243-251); out = (x re, x im, y re, y im, z re, z im)
+__device__ __forceinline__ void dcoef(const double *L, int p, const HTab &tb, int n1, int m1, double o[6]) {
+  const int n = n1 + 1;
+  double ar, ai, br, bi, zr, zi;
+  cget(L, p, n, m1 + 1, ar, ai);
+  cget(L, p, n, m1 - 1, br, bi);
+  cget(L, p, n, m1, zr, zi);
+  const double cmv = -hcm(tb, n, m1 + 1), cpv = hcp(tb, n, m1 - 1), czv = hcz(tb, n, m1);
+  const double dmr = cmv * ar, dmi = cmv * ai, dpr = cpv * br, dpi = cpv * bi;
+  o[0] = (dmr + dpr) / 2;
+  o[1] = (dmi + dpi) / 2;
+  o[2] = -(dpi - dmi) / 2;
+  o[3] = (dpr - dmr) / 2;
+  o[4] = czv * zr;
+  o[5] = czv * zi;
+}
+
+// near field of one source j at d = x - s_j (self excluded by the select)
+template <int KIND, int OUT>
+__device__ __forceinline__ void near_term(double d0, double d1, double d2, const double *__restrict__ dens, int nd,
+                                          int32_t j, bool keep, double kappa, double acc[4]) {
+  const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+  const double ir = rsqrt_f64(r2);
+  if (KIND == 2) {
+    const double *f = dens + (int64_t)nd * j;
+    const double a = HC8PI * ir;
+    const double b = a * ir * ir * (d0 * f[0] + d1 * f[1] + d2 * f[2]);
+    double u0 = a * f[0] + b * d0, u1 = a * f[1] + b * d1, u2 = a * f[2] + b * d2;
+    if (nd == 9) {
+      const double ir2 = ir * ir;
+      const double e = -(6 * HC8PI) * (d0 * f[3] + d1 * f[4] + d2 * f[5]) * (d0 * f[6] + d1 * f[7] + d2 * f[8]) *
+                       (ir2 * ir2 * ir);
+      u0 += e * d0;
+      u1 += e * d1;
+      u2 += e * d2;
+    }
+    acc[0] += keep ? u0 : 0.0;
+    acc[1] += keep ? u1 : 0.0;
+    acc[2] += keep ? u2 : 0.0;
+    return;
+  }
+  const double q = dens[j];
+  double v, dv;
+  if (KIND == 0) {
+    v = HC4PI * ir;
+    dv = -HC4PI * ir * ir * ir;
+  } else {
+    const double e = exp(-kappa * (r2 * ir));
+    v = HC4PI * e * ir;
+    dv = -HC4PI * e * (kappa * ir + ir * ir) * ir;
+  }
+  if (OUT == FFMM_POTENTIAL) {
+    acc[0] += keep ? v * q : 0.0;
+  } else if (OUT == FFMM_GRADIENT) {
+    acc[0] += keep ? dv * d0 * q : 0.0;
+    acc[1] += keep ? dv * d1 * q : 0.0;
+    acc[2] += keep ? dv * d2 * q : 0.0;
+  } else {
+    acc[0] += keep ? v * q : 0.0;
+    acc[1] += keep ? dv * d0 * q : 0.0;
+    acc[2] += keep ? dv * d1 * q : 0.0;
+    acc[3] += keep ? dv * d2 * q : 0.0;
+  }
+}
+
+template <int KIND, int OUT>
+__global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
+  extern __shared__ double sm[];
+  const int s = blockIdx.x;
+  const int i = a.leaf_nodes[s];
+  const bool self = a.tptr == nullptr;
+  const int32_t t0 = self ? a.plo[i] : a.tptr[s];
+  const int32_t t1 = self ? a.phi[i] + 1 : a.tptr[s + 1];
+  if (t0 >= t1) return;
+  const int p = a.p, N = (p + 1) * (p + 1);
+  constexpr int nb = KIND == 2 ? 4 : 1;
+  const int NB = nb * N;
+  const HTab tb = a.tb;
+  double *L = sm;                       // the leaf's local expansion
+  double *uw = sm + NB;                 // per lane: radial weights to p (Yukawa potential)
+  double *ug = uw + HB * (p + 2);       // per lane: radial weights to p + 1 (Yukawa gradient)
+  const double *Lg = a.Lc + (int64_t)s * NB;
+  for (int k = threadIdx.x; k < NB; k += HB) L[k] = Lg[k];
+  __syncthreads();
+  const double c0 = a.centers[i], c1 = a.centers[a.nn + i], c2 = a.centers[2 * a.nn + i];
+  const int lv = a.level[i];
+  const double sc = a.hscale[lv];
+  const double kh = KIND == 1 ? a.khat[lv] : 0.0;
+  int32_t nbr[NCOLL];
+#pragma unroll
+  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  for (int32_t tbk = t0; tbk < t1; tbk += HB) {
+    const int cnt = min(HB, t1 - tbk);
+    int ntp = 1;
+    while (ntp < cnt) ntp <<= 1;
+    const int R = HB / ntp;
+    const int tl = (int)threadIdx.x % ntp, rep = (int)threadIdx.x / ntp;
+    const int32_t j = tbk + tl;
+    const bool valid = tl < cnt;
+    const int32_t jj = valid ? j : t0;
+    const double x = self ? a.sx[jj] : a.tx[jj];
+    const double y = self ? a.sy[jj] : a.ty[jj];
+    const double z = self ? a.sz[jj] : a.tz[jj];
+    const double xh = (x - c0) * sc, yh = (y - c1) * sc, zh = (z - c2) * sc;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (KIND == 2) {
+      // velocity 1/2 [Phi - G x + grad Psi] (_stokes_fold1 / _stokes_output,
+      // src/operators/harmonics_stokes.jl:253-301, 321-325), orders split over replicas
+      double Phi[3] = {0, 0, 0}, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, gP[3] = {0, 0, 0};
+      for (int m = rep; m <= p; m += R)
+        chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
+          const double wt = m == 0 ? 1.0 : 2.0;
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            double cr, ci;
+            cget(L + k * N, p, n, m, cr, ci);
+            Phi[k] += wt * (cr * vr + ci * vi);
+          }
+          if (n < p) {
+            double o[6];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+              dcoef(L + k * N, p, tb, n, m, o);
+#pragma unroll
+              for (int c = 0; c < 3; c++) G[c][k] += wt * (o[2 * c] * vr + o[2 * c + 1] * vi);
+            }
+            dcoef(L + 3 * N, p, tb, n, m, o);
+#pragma unroll
+            for (int c = 0; c < 3; c++) gP[c] += wt * (o[2 * c] * vr + o[2 * c + 1] * vi);
+          }
+        });
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double gx = G[c][0] * xh + G[c][1] * yh + G[c][2] * zh;
+        acc[c] = (Phi[c] - gx + gP[c]) / 2;
+      }
+    } else {
+      double *up = uw + threadIdx.x * (p + 2), *ugr = ug + threadIdx.x * (p + 2);
+      if (KIND == 1) {
+        const double z2 = kh * kh * (xh * xh + yh * yh + zh * zh);
+        if (OUT != FFMM_GRADIENT) yukawa_u(z2, p, up);
+        if (OUT != FFMM_POTENTIAL) yukawa_u(z2, p + 1, ugr);
+      }
+      if (OUT != FFMM_GRADIENT) {
+        // _harmonic_potential (src/operators/harmonics.jl:462-468)
+        double pot = 0.0;
+        for (int m = rep; m <= p; m += R)
+          chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
+            double lr, li;
+            get_entry(L, n, m, lr, li);
+            const double wn = KIND == 1 ? up[n] : 1.0;
+            pot += m == 0 ? wn * (lr * vr) : wn * (2 * (lr * vr + li * vi));
+          });
+        acc[0] = pot;
+      }
+      if (OUT != FFMM_POTENTIAL) {
+        // _harmonic_gradient (src/operators/harmonics.jl:469-491); Yukawa adds
+        // x * (potential with the radial weights) (src/operators/harmonics_yukawa.jl:318-327)
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0, pr = 0.0;
+        for (int m1 = rep; m1 <= p - 1; m1 += R)
+          chain(tb, xh, yh, zh, m1, p - 1, [&](int n1, double vr, double vi) {
+            const int n = n1 + 1;
+            const double wn = KIND == 1 ? ugr[n] : 1.0;
+            if (m1 == 0) {
+              double lr, li;
+              get_entry(L, n, 1, lr, li);
+              const double c = tb.cm[tri(n, 1)];
+              const double ax = -c * lr, ay = -c * li, az = tb.cz[tri(n, 0)] * L[n * n];
+              g0 += wn * (ax * vr);
+              g1 += wn * (ay * vr);
+              g2 += wn * (az * vr);
+            } else {
+              double ur, ui, dr, di, zr, zi;
+              get_entry(L, n, m1 + 1, ur, ui);
+              get_entry(L, n, m1 - 1, dr, di);
+              const double cu = tb.cm[tri(n, m1 + 1)] / 2, cd = tb.cp[tri(n, m1 - 1)] / 2;
+              const double axr = -cu * ur + cd * dr, axi = -cu * ui + cd * di;
+              const double ayr = -(cu * ui + cd * di), ayi = cu * ur + cd * dr;
+              const double czv = tb.cz[tri(n, m1)];
+              get_entry(L, n, m1, zr, zi);
+              g0 += (2 * wn) * (axr * vr + axi * vi);
+              g1 += (2 * wn) * (ayr * vr + ayi * vi);
+              g2 += (2 * wn) * (czv * (zr * vr + zi * vi));
+            }
+          });
+        if (KIND == 1) {
+          for (int m = rep; m <= p; m += R)
+            chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
+              double lr, li;
+              get_entry(L, n, m, lr, li);
+              const double wn = kh * kh * ugr[n + 1] / (2 * n + 3);
+              pr += m == 0 ? wn * (lr * vr) : wn * (2 * (lr * vr + li * vi));
+            });
+        }
+        const int o = OUT == FFMM_GRADIENT ? 0 : 1;
+        acc[o] = (g0 + xh * pr) * sc;
+        acc[o + 1] = (g1 + yh * pr) * sc;
+        acc[o + 2] = (g2 + zh * pr) * sc;
+      }
+    }
+    // the near field of the 27 colleagues (src/device.jl:89-100)
+    const int32_t skip = self ? j : -1;
+    for (int m = 0; m < NCOLL; m++) {
+      const int32_t nb2 = nbr[m];
+      if (nb2 < 0) continue;
+      const int32_t p0 = a.plo[nb2], p1 = a.phi[nb2];
+      for (int32_t q = p0 + rep; q <= p1; q += R)
+        near_term<KIND, OUT>(x - a.sx[q], y - a.sy[q], z - a.sz[q], a.tau, a.nd, q, q != skip, a.kappa, acc);
+    }
+    for (int off = ntp; off < HB; off <<= 1)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[c] += __shfl_xor(acc[c], off, HB);
+    if (valid && rep == 0) {
+      const int64_t o = a.outidx[j];
+      if (KIND == 2) {
+        a.out[3 * o] = acc[0];
+        a.out[3 * o + 1] = acc[1];
+        a.out[3 * o + 2] = acc[2];
+      } else if (OUT == FFMM_POTENTIAL) {
+        a.out[o] = acc[0];
+      } else if (OUT == FFMM_GRADIENT) {
+        a.out[3 * o] = acc[0];
+        a.out[3 * o + 1] = acc[1];
+        a.out[3 * o + 2] = acc[2];
+      } else {
+        a.out[4 * o] = acc[0];
+        a.out[4 * o + 1] = acc[1];
+        a.out[4 * o + 2] = acc[2];
+        a.out[4 * o + 3] = acc[3];
+      }
+    }
+  }
+}
+
+static int next_pow2(int v) {
+  int m = 1;
+  while (m < v) m <<= 1;
+  return m;
+}
+
+void harmonic_s2m_level(Plan &plan, int l) {
+  OpCache &o = *plan.ops;
+  const int64_t nq = plan.s2m_off[l + 1] - plan.s2m_off[l];
+  if (nq == 0) return;
+  const int p = o.p, MP = next_pow2(p + 1), R = HB / MP;
+  const int NB = o.Ns;
+  const size_t lds = sizeof(double) * ((size_t)R * NB + (size_t)R * (p + 2));
+  const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];
+  hipStream_t s = plan.ctx->stream;
+#define FFMM_S2M_ARGS                                                                                              \
+  nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.hscale.p, plan.khat.p, plan.plo.p, plan.phi.p, plan.sx.p, \
+      plan.sy.p, plan.sz.p, plan.tau.p, plan.nd, o.tab(), p, MP, plan.Mp.p
+  if (o.kernel == FFMM_KERNEL_LAPLACE) k_s2m_harm<0><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
+  else if (o.kernel == FFMM_KERNEL_YUKAWA) k_s2m_harm<1><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
+  else k_s2m_harm<2><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
+#undef FFMM_S2M_ARGS
+  FFMM_LAUNCH_CHECK();
+}
+
+void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output) {
+  OpCache &o = *plan.ops;
+  const int64_t nleaves = plan.leaf_off[plan.L];
+  if (nleaves == 0) return;
+  const size_t lds = sizeof(double) * ((size_t)o.Ns + (o.kernel == FFMM_KERNEL_YUKAWA ? 2 * (size_t)HB * (o.p + 2) : 0));
+  hipStream_t s = plan.ctx->stream;
+  const unsigned g = (unsigned)nleaves;
+  if (o.kernel == FFMM_KERNEL_STOKESLET) {
+    k_eval_harm<2, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+  } else if (o.kernel == FFMM_KERNEL_LAPLACE) {
+    if (output == FFMM_POTENTIAL) k_eval_harm<0, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    else if (output == FFMM_GRADIENT) k_eval_harm<0, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<0, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
+  } else {
+    if (output == FFMM_POTENTIAL) k_eval_harm<1, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    else if (output == FFMM_GRADIENT) k_eval_harm<1, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<1, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
+  }
+  FFMM_LAUNCH_CHECK();
+}
+
+}  // namespace ffmm

@@ -17,7 +17,6 @@ namespace ffmm {
 constexpr int TB = 256;
 
 __constant__ int32_t c_far_index[343];   // far_index_table (src/util.jl:87-97), -1: near
-__constant__ int32_t c_far_class[NFAR];
 
 struct TreePtrs {
   const int32_t *cell, *parent, *child_first, *col;
@@ -80,26 +79,27 @@ __global__ void k_pair_fill(TreePtrs t, int64_t lo, int64_t nl, const int32_t *_
 
 // _class_count! (src/device_lists.jl:314-324), class-major: cnt[c * nl + q]
 __global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__restrict__ ptr, int64_t nl, int nclass,
-                              int32_t *__restrict__ cnt) {
+                              const int32_t *__restrict__ far_class, int32_t *__restrict__ cnt) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nl) return;
-  int32_t c16[32];
-  for (int c = 0; c < nclass; c++) c16[c] = 0;
-  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) c16[c_far_class[far[p]]]++;
-  for (int c = 0; c < nclass; c++) cnt[(int64_t)c * nl + q] = c16[c];
+  int32_t cc[MAX_CLASSES];
+  for (int c = 0; c < nclass; c++) cc[c] = 0;
+  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) cc[far_class[far[p]]]++;
+  for (int c = 0; c < nclass; c++) cnt[(int64_t)c * nl + q] = cc[c];
 }
 
 // _class_fill! (src/device_lists.jl:325-334)
 __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__restrict__ far,
                              const int32_t *__restrict__ ptr, const int32_t *__restrict__ cstart, int64_t nl,
-                             int nclass, int32_t *__restrict__ csrc, int32_t *__restrict__ cfar) {
+                             int nclass, const int32_t *__restrict__ far_class, int32_t *__restrict__ csrc,
+                             int32_t *__restrict__ cfar) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nl) return;
-  int32_t cur[32];
+  int32_t cur[MAX_CLASSES];
   for (int c = 0; c < nclass; c++) cur[c] = cstart[(int64_t)c * nl + q];
   for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) {
     int32_t f = far[p];
-    int c = c_far_class[f];
+    int c = far_class[f];
     int32_t pos = cur[c]++;
     csrc[pos] = src[p];
     cfar[pos] = f;
@@ -153,12 +153,10 @@ void build_lists(Plan &plan) {
       fi[u] = m >= 2 ? nf++ : -1;
     }
     FFMM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_far_index), fi, sizeof fi, 0, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_far_class), ops.far_class_h.data(), sizeof(int32_t) * NFAR, 0,
-                                    hipMemcpyHostToDevice, s));
     ctx.sync();
   }
   const int nclass = ops.nclass;
-  if (nclass > 32) fail(FFMM_ERR_ARGUMENT, "at most 32 symmetry classes");
+  if (nclass > MAX_CLASSES) fail(FFMM_ERR_ARGUMENT, "too many symmetry classes");
   TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};
   plan.lists.clear();
   plan.lists.resize(plan.L);
@@ -193,14 +191,14 @@ void build_lists(Plan &plan) {
     int64_t ne = (int64_t)nclass * nl;
     ccnt.resize(ne);
     cstart.resize(ne);
-    k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ccnt.p);
+    k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ops.far_class.p, ccnt.p);
     FFMM_LAUNCH_CHECK();
     exclusive_scan_i32(ctx, ccnt.p, cstart.p, ne);
     li.csrc.resize(np);
     li.cfar.resize(np);
     if (np > 0) {
-      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, cstart.p, nl, nclass, li.csrc.p,
-                                                  li.cfar.p);
+      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, cstart.p, nl, nclass,
+                                                  ops.far_class.p, li.csrc.p, li.cfar.p);
       FFMM_LAUNCH_CHECK();
     }
     // the runs

@@ -126,40 +126,60 @@ __global__ void k_gather_perm(double *__restrict__ dst, const double *__restrict
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) dst[j] = src[perm[j]];
 }
+// nd components per point, point major
+__global__ void k_gather_perm_nd(double *__restrict__ dst, const double *__restrict__ src,
+                                 const int32_t *__restrict__ perm, int64_t n, int nd) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nd) return;
+  int64_t j = g / nd;
+  int c = (int)(g % nd);
+  dst[g] = src[(int64_t)perm[j] * nd + c];
+}
 
 // ---------------------------------------------------------------------------
 // dense M2L by symmetry class: _class_gather! / _class_scatter! (src/device_passes.jl:426-473)
-// Gc[i, q] = sgn[f, i] Mp[idx[f, i], src[q]]  (P_f^T m)
+// An expansion column holds nb blocks of N entries (Ns = nb N); the class operator acts
+// on each block (nb columns per pair, as the reference lays out the Stokes blocks).
+// Gc[i, q nb + b] = sgn[f, i] Mp[b N + idx[f, i], src[q]]  (P_f^T m, block b)
 // ---------------------------------------------------------------------------
-__global__ void k_class_gather(double *__restrict__ Gc, const double *__restrict__ Mp, int Ns,
+__global__ void k_class_gather(double *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
                                const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int64_t npk,
                                const int32_t *__restrict__ idx, const int8_t *__restrict__ sgn) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)Ns * npk) return;
-  int64_t q = g / Ns;
-  int i = (int)(g % Ns);
+  if (g >= (int64_t)N * nb * npk) return;
+  int64_t c = g / N;
+  int i = (int)(g % N);
+  int64_t q = c / nb;
+  int b = (int)(c % nb);
   int64_t f = cfar[q];
-  Gc[g] = (double)sgn[f * Ns + i] * Mp[(int64_t)csrc[q] * Ns + idx[f * Ns + i]];
+  Gc[g] = (double)sgn[f * N + i] * Mp[(int64_t)csrc[q] * Ns + b * N + idx[f * N + i]];
 }
-// target-owned scatter of P_f y over the runs of a class that fall in the piece
-// [a, a + npk): pL[j, rtgt[r]] += alpha sum_p sgn[f, i] Yc[i, p - a], i = inv[f, j]
-__global__ void k_class_scatter(double *__restrict__ pL, const double *__restrict__ Yc, int Ns,
+// target-owned scatter of P_t y over the runs of a class that fall in the piece
+// [a, a + npk): pL[b N + j, rtgt[r]] += alpha sum_p sgn_t[f, i] Y_b[i, p - a], i = inv[f, j];
+// for Stokes (nb = 4) the fourth block adds the coupling Psi += o_f . Phi with the
+// pair's own offset (src/device_passes.jl:449-473, src/operators/harmonics.jl:604-628)
+__global__ void k_class_scatter(double *__restrict__ pL, const double *__restrict__ Yc, int Ns, int N, int nb,
                                 const int32_t *__restrict__ rtgt, const int32_t *__restrict__ rptr, int64_t r0,
                                 int64_t nr, const int32_t *__restrict__ cfar, int64_t a, int64_t npk,
-                                const int32_t *__restrict__ inv, const int8_t *__restrict__ sgn, double alpha) {
+                                const int32_t *__restrict__ inv, const int8_t *__restrict__ sgn_t,
+                                const double *__restrict__ offs, double alpha) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)Ns * nr) return;
   int64_t r = r0 + g / Ns;
-  int j = (int)(g % Ns);
+  int jb = (int)(g % Ns);
+  int b = jb / N, j = jb % N;
   int64_t p0 = max((int64_t)rptr[r], a), p1 = min((int64_t)rptr[r + 1], a + npk);
   if (p0 >= p1) return;
   double acc = 0.0;
   for (int64_t p = p0; p < p1; p++) {
     int64_t f = cfar[p];
-    int i = inv[f * Ns + j];
-    acc += (double)sgn[f * Ns + i] * Yc[(p - a) * Ns + i];
+    int i = inv[f * N + j];
+    const double *Y = Yc + (p - a) * nb * N;
+    double y = Y[b * N + i];
+    if (nb == 4 && b == 3) y = y + offs[3 * f] * Y[i] + offs[3 * f + 1] * Y[N + i] + offs[3 * f + 2] * Y[2 * N + i];
+    acc += (double)sgn_t[f * N + i] * y;
   }
-  pL[(int64_t)rtgt[r] * Ns + j] += alpha * acc;
+  pL[(int64_t)rtgt[r] * Ns + jb] += alpha * acc;
 }
 
 static void gemm(Context &ctx, int m, int n, int k, double alpha, const double *A, int lda, const double *B, int ldb,
@@ -185,10 +205,12 @@ void allocate_pass_buffers(Plan &plan) {
   plan.Mp.resize((size_t)Ns * plan.nnodes);
   plan.Pl.resize((size_t)Ns * plan.nnodes);
   plan.Lc.resize((size_t)Nl * std::max<int64_t>(nleaves, 1));
-  plan.tau.resize(plan.N);
-  plan.pM.resize((size_t)Nl * nq);
-  plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);
-  plan.Mup.resize((size_t)Ns * nq);
+  plan.tau.resize((size_t)plan.N * (o.method == METHOD_HARMONIC && o.nb == 4 ? 9 : 1));
+  if (o.method == METHOD_CUBE) {
+    plan.pM.resize((size_t)Nl * nq);
+    plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);
+    plan.Mup.resize((size_t)Ns * nq);
+  }
   plan.tM.resize((size_t)Ns * ng);
   plan.G.resize((size_t)Ns * std::max(nq, ng));
   int64_t pmax = 1;
@@ -201,10 +223,13 @@ void allocate_pass_buffers(Plan &plan) {
 
 void gather_inputs(Plan &plan, const double *tau, int sorted) {
   Context &ctx = *plan.ctx;
+  const int nd = plan.nd;
   if (sorted)
-    FFMM_HIP(hipMemcpyAsync(plan.tau.p, tau, sizeof(double) * plan.N, hipMemcpyDeviceToDevice, ctx.stream));
-  else
+    FFMM_HIP(hipMemcpyAsync(plan.tau.p, tau, sizeof(double) * plan.N * nd, hipMemcpyDeviceToDevice, ctx.stream));
+  else if (nd == 1)
     k_gather_perm<<<nblocks(plan.N, TB), TB, 0, ctx.stream>>>(plan.tau.p, tau, plan.perm.p, plan.N);
+  else
+    k_gather_perm_nd<<<nblocks(plan.N * nd, TB), TB, 0, ctx.stream>>>(plan.tau.p, tau, plan.perm.p, plan.N, nd);
   FFMM_LAUNCH_CHECK();
 }
 
@@ -220,7 +245,10 @@ void upward_pass(Plan &plan) {
   for (int l = plan.L - 1; l >= 2; l--) {
     const int64_t nq = plan.s2m_off[l + 1] - plan.s2m_off[l];
     const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];
-    if (nq > 0) {
+    if (o.method == METHOD_HARMONIC) {
+      // the multipoles directly (partial_to_multipole! is a copy, src/operators/harmonics.jl:441)
+      harmonic_s2m_level(plan, l);
+    } else if (nq > 0) {
       k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p, o.large.p,
                                             Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p, plan.tau.p,
                                             plan.pM.p);
@@ -234,7 +262,8 @@ void upward_pass(Plan &plan) {
     if (l < plan.L - 1) {
       const int64_t ng = plan.nl[l + 1] / NCHILD;
       const double *B = plan.Mp.p + plan.lo[l + 1] * Ns;
-      gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, o.M2M.p, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
+      const double *M2M = o.method == METHOD_HARMONIC ? plan.hl[l]->M2M.p : o.M2M.p;
+      gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, M2M, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
       k_scatter_parents<<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM.p, plan.parent.p,
                                                                      plan.lo[l + 1], Ns, ng, 1.0);
       FFMM_LAUNCH_CHECK();
@@ -247,20 +276,25 @@ void m2l_dense_level(Plan &plan, int l) {
   OpCache &o = *plan.ops;
   hipStream_t s = ctx.stream;
   const int Ns = o.Ns;
+  const bool harm = o.method == METHOD_HARMONIC;
+  const int N = harm ? o.hN : Ns, nb = harm ? o.nb : 1;
   LevelLists &li = plan.lists[l];
   double *pL = plan.Pl.p + plan.lo[l] * Ns;
-  const double alpha = 1.0 / plan.lam[l];   // m2l_scale of the level: (w_l / w_ref)^(-1)
+  // m2l_scale of the level: (w_l / w_ref)^(-1) for the homogeneous kernels, 1 for Yukawa
+  const double alpha = harm ? plan.alpha[l] : 1.0 / plan.lam[l];
+  const double *reps = harm ? plan.hl[l]->reps.p : o.class_ops.p;
   for (int c = 0; c < o.nclass; c++) {
     const int64_t pa = li.class_pair[c], pb = li.class_pair[c + 1];
     const int64_t r0 = li.class_run[c], nr = li.class_run[c + 1] - r0;
     for (int64_t a = pa; a < pb; a += PIECE) {
       const int64_t npk = std::min(PIECE, pb - a);
-      k_class_gather<<<nblocks((int64_t)Ns * npk, TB), TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, li.csrc.p + a,
-                                                                   li.cfar.p + a, npk, o.idx.p, o.sgn.p);
+      k_class_gather<<<nblocks((int64_t)N * nb * npk, TB), TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a,
+                                                                       li.cfar.p + a, npk, o.idx.p, o.sgn.p);
       FFMM_LAUNCH_CHECK();
-      gemm(ctx, Ns, (int)npk, Ns, 1.0, o.class_ops.p + (size_t)c * Ns * Ns, Ns, plan.Gc.p, Ns, 0.0, plan.Yc.p, Ns);
-      k_class_scatter<<<nblocks((int64_t)Ns * nr, TB), TB, 0, s>>>(pL, plan.Yc.p, Ns, li.rtgt.p, li.rptr.p, r0, nr,
-                                                                   li.cfar.p, a, npk, o.inv.p, o.sgn.p, alpha);
+      gemm(ctx, N, (int)(nb * npk), N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p, N, 0.0, plan.Yc.p, N);
+      k_class_scatter<<<nblocks((int64_t)Ns * nr, TB), TB, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0,
+                                                                   nr, li.cfar.p, a, npk, o.inv.p, o.sgn_t.p,
+                                                                   o.offs.p, alpha);
       FFMM_LAUNCH_CHECK();
     }
   }
@@ -286,7 +320,8 @@ void downward_pass(Plan &plan) {
                                                                     Ns, ng);
       FFMM_LAUNCH_CHECK();
       // the stacked children output of L2L is the level's block of partial locals
-      gemm(ctx, NCHILD * Ns, (int)ng, Ns, 1.0, o.L2L.p, NCHILD * Ns, plan.G.p, Ns, 0.0, pL, NCHILD * Ns);
+      const double *L2L = o.method == METHOD_HARMONIC ? plan.hl[l - 1]->L2L.p : o.L2L.p;
+      gemm(ctx, NCHILD * Ns, (int)ng, Ns, 1.0, L2L, NCHILD * Ns, plan.G.p, Ns, 0.0, pL, NCHILD * Ns);
     } else {
       k_fill<<<nblocks((int64_t)Ns * nl, TB), TB, 0, s>>>(pL, (int64_t)Ns * nl, 0.0);
       FFMM_LAUNCH_CHECK();
@@ -299,6 +334,13 @@ void downward_pass(Plan &plan) {
   for (int l = 2; l < plan.L; l++) {
     const int64_t nq = plan.leaf_off[l + 1] - plan.leaf_off[l];
     if (nq == 0) continue;
+    if (o.method == METHOD_HARMONIC) {
+      // partial_to_local! is a copy (src/operators/harmonics.jl:442)
+      k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Lc.p + plan.leaf_off[l] * Nl, plan.Pl.p,
+                                                                 plan.leaf_nodes.p + plan.leaf_off[l], Ns, nq);
+      FFMM_LAUNCH_CHECK();
+      continue;
+    }
     k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, plan.leaf_nodes.p + plan.leaf_off[l],
                                                                Ns, nq);
     FFMM_LAUNCH_CHECK();

@@ -15,6 +15,7 @@ FFMM_OK, FFMM_ERR_ARGUMENT, FFMM_ERR_DIMENSION, FFMM_ERR_DEVICE, FFMM_ERR_STATE,
 PREC_F64, PREC_MIXED = 0, 1
 M2L_DENSE, M2L_SPECTRAL = 0, 1
 POTENTIAL, GRADIENT, POTENTIAL_GRADIENT = 0, 1, 2
+KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_STOKESLET = 0, 1, 2
 
 
 class ArgumentError(ValueError):
@@ -46,6 +47,10 @@ SIGNATURES = {
     "ffmm_opcache_destroy": (_i32, [_vp]),
     "ffmm_opcache_put_cube": (_i32, [_vp, _d, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32,
                                      _vp, _vp, _vp, _vp, _vp]),
+    "ffmm_opcache_create_harmonic": (_i32, [_vp, _i32, _i32, _d, _d, ctypes.POINTER(_vp)]),
+    "ffmm_opcache_put_harmonic_classes": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "ffmm_opcache_put_harmonic_level": (_i32, [_vp, _d, _vp, _vp, _vp]),
+    "ffmm_opcache_has_width": (_i32, [_vp, _d, ctypes.POINTER(_i32)]),
     "ffmm_plan_create": (_i32, [_vp, _vp, _i64, _i32, _vp, _d, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)]),
     "ffmm_plan_destroy": (_i32, [_vp]),
     "ffmm_plan_sizes": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i64), _vp, _vp, _vp,
@@ -53,6 +58,7 @@ SIGNATURES = {
     "ffmm_plan_download_tree": (_i32, [_vp] + [_vp] * 14),
     "ffmm_plan_download_lists": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "ffmm_build_expansions": (_i32, [_vp, _vp, _i32]),
+    "ffmm_build_expansions_nd": (_i32, [_vp, _vp, _i32, _i32]),
     "ffmm_evaluate_at_sources": (_i32, [_vp, _vp, _i32]),
     "ffmm_evaluate": (_i32, [_vp, _vp, _i64, _vp, _i32]),
     "ffmm_matvec": (_i32, [_vp, _vp, _i32, _vp, _i32]),

@@ -50,9 +50,31 @@ class Laplace3D:
         return "Laplace3D()"
 
 
+class Yukawa3D:
+    """Modified Helmholtz (Yukawa) Green's function e^{-k r}/(4 pi r) (src/kernels.jl:220-240)."""
+    homogeneity = None
+
+    def __init__(self, k: float):
+        self.k = float(k)
+
+    def __repr__(self):
+        return f"Yukawa3D({self.k})"
+
+
+class Stokeslet3D:
+    """The Stokeslet (1/8 pi)(I + r r^T / r^2)/r with unit viscosity (src/kernels.jl:241-272)."""
+    homogeneity = -1
+
+    def __repr__(self):
+        return "Stokeslet3D()"
+
+
 class Potential:
     code = _lib.POTENTIAL
     ncomp = 1
+
+
+Velocity = Potential     # the reference's alias for the Stokes kernels (src/sources.jl:31-44)
 
 
 class Gradient:
@@ -102,6 +124,68 @@ class CubeKernelIndependent:
 
     def __repr__(self):
         return f"CubeKernelIndependent{{3}}({self.kernel!r}; p={self.p}, pc={self.pc}, d={self.d})"
+
+    @property
+    def expansion_length(self) -> int:
+        return self.nsurface
+
+
+class SphericalHarmonics:
+    """``SphericalHarmonics(kernel; p=16, tol=0)`` for Laplace3D or Yukawa3D charges
+    (src/operators/harmonics.jl:313-322, src/operators/harmonics_yukawa.jl:27-33):
+    solid-harmonic expansions of degree p, dense class M2L (tol = 0)."""
+
+    def __init__(self, kernel, p: int = 16, tol: float = 0.0):
+        if not isinstance(kernel, (Laplace3D, Yukawa3D)):
+            raise ArgumentError("SphericalHarmonics takes Laplace3D or Yukawa3D")
+        if isinstance(kernel, Yukawa3D):
+            if not 1 <= p <= 60:
+                raise ArgumentError("p must lie in 1:60")
+        elif p < 1:
+            raise ArgumentError("p must be at least 1")
+        if not 0 <= tol < 1:
+            raise ArgumentError("tol must lie in [0, 1)")
+        if tol > 0:
+            raise ArgumentError("the device passes of this build take the dense M2L (tol = 0)")
+        if p > 60:
+            raise ArgumentError("the device passes of this build take p <= 60")
+        self.kernel, self.p, self.tol = kernel, int(p), float(tol)
+
+    nblocks = 1
+
+    @property
+    def expansion_length(self) -> int:
+        return self.nblocks * (self.p + 1) ** 2
+
+    def __repr__(self):
+        return f"SphericalHarmonics({self.kernel!r}; p={self.p})"
+
+
+class StokesHarmonics(SphericalHarmonics):
+    """``StokesHarmonics(Stokeslet3D(); p=16, tol=0)`` (src/operators/harmonics_stokes.jl:34-40):
+    four Laplace expansions [Phi_1; Phi_2; Phi_3; Psi] per box; stokeslet and stresslet
+    densities, velocity output."""
+    nblocks = 4
+
+    def __init__(self, kernel, p: int = 16, tol: float = 0.0):
+        if not isinstance(kernel, Stokeslet3D):
+            raise ArgumentError("StokesHarmonics takes Stokeslet3D")
+        if p < 1:
+            raise ArgumentError("p must be at least 1")
+        if not 0 <= tol < 1:
+            raise ArgumentError("tol must lie in [0, 1)")
+        if tol > 0:
+            raise ArgumentError("the device passes of this build take the dense M2L (tol = 0)")
+        if p > 60:
+            raise ArgumentError("the device passes of this build take p <= 60")
+        self.kernel, self.p, self.tol = kernel, int(p), float(tol)
+
+    def __repr__(self):
+        return f"StokesHarmonics({self.kernel!r}; p={self.p})"
+
+
+def _is_harmonic(m) -> bool:
+    return isinstance(m, SphericalHarmonics)
 
 
 # ---------------------------------------------------------------------------
@@ -191,11 +275,13 @@ class OperatorCache:
     device copies are built once per device and shared by every plan made with this
     cache (OperatorCache.device, src/device_passes.jl:136-146)."""
 
-    def __init__(self, method: CubeKernelIndependent):
+    def __init__(self, method):
         self.method = method
         self.W0 = math.nan
         self._host = None
         self._device = {}
+        self._widths = {}          # harmonic methods: width -> HarmonicOperators (Yukawa: one per width)
+        self._put = {}             # device -> widths uploaded
 
     def has_base_width(self) -> bool:
         return not math.isnan(self.W0)
@@ -226,16 +312,76 @@ class OperatorCache:
             e += 1
         raise ArgumentError("could not fit a box")
 
-    def host_operators(self) -> _ops.CubeOperators:
+    def host_operators(self):
         if not self.has_base_width():
             raise ArgumentError("fit a box before requesting operators")
         if self._host is None:
             m = self.method
-            self._host = _ops.level_operators(self.W0, p=m.p, pc=m.pc, d=m.d, rtol=m.pinv_rtol)
+            if _is_harmonic(m):
+                self._host = self.harmonic_operators(self.W0)
+            else:
+                self._host = _ops.level_operators(self.W0, p=m.p, pc=m.pc, d=m.d, rtol=m.pinv_rtol)
         return self._host
+
+    def harmonic_operators(self, w: float):
+        """``get_operators!`` of a harmonic method for boxes of width ``w``
+        (src/operators/cache.jl:224-248): Laplace and Stokes take the reference set of
+        width W0 for every width (homogeneous), Yukawa one set per width."""
+        from . import harmonics as _h
+        m = self.method
+        if not isinstance(m.kernel, Yukawa3D):
+            w = self.W0
+        ops = self._widths.get(w)
+        if ops is None:
+            if isinstance(m.kernel, Yukawa3D):
+                ops = _h.yukawa_level_operators(w, m.p, m.kernel.k)
+            else:
+                ops = _h.laplace_level_operators(w, m.p)
+            self._widths[w] = ops
+        return ops
+
+    def _put_width(self, device: int, w: float):
+        ops = self.harmonic_operators(w)
+        put = self._put.setdefault(device, set())
+        if ops.width in put:
+            return
+        F = lambda a: np.asfortranarray(a, dtype=np.float64)   # noqa: E731
+        M2M, L2L = F(ops.M2M), F(ops.L2L)
+        reps = np.ascontiguousarray(np.stack([F(r).T for r in ops.reps]))   # each N x N column major
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        call("ffmm_opcache_put_harmonic_level", self._device[device], ops.width, P(M2M), P(L2L), P(reps))
+        put.add(ops.width)
+
+    def _device_harmonic(self, device: int):
+        from . import harmonics as _h
+        m = self.method
+        kern = (_lib.KERNEL_STOKESLET if isinstance(m, StokesHarmonics) else
+                _lib.KERNEL_YUKAWA if isinstance(m.kernel, Yukawa3D) else _lib.KERNEL_LAPLACE)
+        kappa = m.kernel.k if isinstance(m.kernel, Yukawa3D) else 0.0
+        h = ctypes.c_void_p()
+        call("ffmm_opcache_create_harmonic", context(device), kern, m.p, kappa, self.W0, ctypes.byref(h))
+        self._device[device] = h
+        sc = _h.harmonic_symmetry_classes(m.p)
+        keep = [np.ascontiguousarray(sc.klass, dtype=np.int32), np.ascontiguousarray(sc.idx, dtype=np.int32),
+                np.ascontiguousarray(sc.sgn, dtype=np.int8), np.ascontiguousarray(sc.idx_t, dtype=np.int32),
+                np.ascontiguousarray(sc.sgn_t, dtype=np.int8)]
+        call("ffmm_opcache_put_harmonic_classes", h, len(sc.reps), *[a.ctypes.data_as(ctypes.c_void_p) for a in keep])
+        if not isinstance(m.kernel, Yukawa3D):
+            self._put_width(device, self.W0)
+        return h
+
+    def ensure_widths(self, device: int, widths):
+        """Upload the operators of these box widths (Yukawa: per width)."""
+        self.device(device)
+        for w in widths:
+            self._put_width(device, float(w))
 
     def device(self, device: int = 0):
         h = self._device.get(device)
+        if h is None and _is_harmonic(self.method):
+            if not self.has_base_width():
+                raise ArgumentError("fit a box before requesting operators")
+            return self._device_harmonic(device)
         if h is None:
             o = self.host_operators()
             m = self.method
@@ -326,7 +472,7 @@ class DevicePlan:
     (src/device_passes.jl:98-134).  ``precision`` is ``"f64"`` or ``"mixed"``; ``m2l`` is
     ``"dense"`` (class GEMMs) or ``"spectral"`` (batched FFT)."""
 
-    def __init__(self, method: CubeKernelIndependent, xs, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
+    def __init__(self, method, xs, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
                  max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
         cache = OperatorCache(method) if cache is None else cache
         if cache.method is not method:
@@ -360,6 +506,9 @@ class DevicePlan:
         self.N = n
         self.ppl = int(ppl)
         self._read_sizes()
+        if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
+            # the per-width Yukawa operators of the levels with translations (2 .. L-1)
+            cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -404,8 +553,11 @@ class DevicePlan:
         return ptr, src, far
 
     def download_expansions(self):
-        o = self.cache.host_operators()
-        Ns, Nl = o.nsmall, o.nlarge
+        if _is_harmonic(self.method):
+            Ns = Nl = self.method.expansion_length
+        else:
+            o = self.cache.host_operators()
+            Ns, Nl = o.nsmall, o.nlarge
         nleaves = int(self.level_leaves.sum())
         M = np.empty((self.nnodes, Ns))
         Pl = np.empty((self.nnodes, Ns))
@@ -422,16 +574,23 @@ class DevicePlan:
 
     # -- the passes (device tensors in and out) --
     def build_expansions(self, tau, sorted: bool = False):
+        """Densities: N charges, or for Stokes (N, 3) stokeslets or (N, 9) rows
+        (f, g, nu) of a stokeslet f plus a stresslet (g, nu)."""
         t = _dev_f64(tau, self.device)
-        if t.ndim != 1 or t.shape[0] != self.N:
+        if t.shape[0] != self.N or t.ndim > 2:
             raise DimensionMismatch(f"expected {self.N} entries in group 1")
-        call("ffmm_build_expansions", self._h, _ptr(t), int(bool(sorted)))
+        nd = 1 if t.ndim == 1 else t.shape[1]
+        call("ffmm_build_expansions_nd", self._h, _ptr(t), nd, int(bool(sorted)))
         return self
+
+    def _ncomp(self, o) -> int:
+        return 3 if isinstance(self.method, StokesHarmonics) else o.ncomp
 
     def evaluate_at_sources(self, output=Potential, out=None):
         o = _output(output)
         torch = _torch()
-        shape = (self.N,) if o.ncomp == 1 else (self.N, o.ncomp)
+        nc = self._ncomp(o)
+        shape = (self.N,) if nc == 1 else (self.N, nc)
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
         elif tuple(out.shape) != shape:
@@ -444,7 +603,8 @@ class DevicePlan:
         torch = _torch()
         t = _dev_f64(targets, self.device, cols=3)
         nt = t.shape[0]
-        shape = (nt,) if o.ncomp == 1 else (nt, o.ncomp)
+        nc = self._ncomp(o)
+        shape = (nt,) if nc == 1 else (nt, nc)
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
         elif tuple(out.shape) != shape:
@@ -493,7 +653,7 @@ class FMM:
         return self.plan.nlevels
 
     def __repr__(self):
-        return (f"FMM(CubeKernelIndependent, Tree{{3,Float64}}: {self.plan.N} points, {self.plan.nnodes} nodes on "
+        return (f"FMM({type(self.method).__name__}, Tree{{3,Float64}}: {self.plan.N} points, {self.plan.nnodes} nodes on "
                 f"{self.plan.nlevels} levels)")
 
 

@@ -96,6 +96,43 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
                           const double *M2M, const double *L2L, int nclass, const double *class_ops,
                           const int32_t *far_class, const int32_t *idx, const int8_t *sgn, const double *tables);
 
+/* kernels of the harmonic methods */
+#define FFMM_KERNEL_LAPLACE 0    /* Laplace3D(): 1/(4 pi r) (src/kernels.jl:179-200) */
+#define FFMM_KERNEL_YUKAWA 1     /* Yukawa3D(kappa): e^{-kappa r}/(4 pi r) (src/kernels.jl:220-240) */
+#define FFMM_KERNEL_STOKESLET 2  /* Stokeslet3D(): (1/8pi)(I + r r^T/r^2)/r (src/kernels.jl:241-272) */
+
+/* The device copies of a harmonic method's operators: SphericalHarmonics(Laplace3D(); p)
+ * (src/operators/harmonics.jl:313-322), SphericalHarmonics(Yukawa3D(kappa); p)
+ * (src/operators/harmonics_yukawa.jl:27-33) or StokesHarmonics(Stokeslet3D(); p)
+ * (src/operators/harmonics_stokes.jl:34-40), for boxes of width w_ref * 2^e.  The
+ * recurrence tables (HarmonicTables, src/operators/harmonics.jl:163-187) are built here
+ * from p; expansions hold (p+1)^2 reals per scalar block (4 blocks for Stokes). */
+int ffmm_opcache_create_harmonic(ffmm_context ctx, int kernel, int p, double kappa, double w_ref, ffmm_opcache *out);
+
+/* The class structure of the M2L: the class of each of the 316 far offsets and per
+ * offset the source-side (idx, sgn) and target-side (idx_t, sgn_t) signed permutations
+ * of the real harmonic storage, (N x 316 each, 0-based), with
+ * M2L_f m = P_t (A_rep (P_s^T m)), (P_s^T m)[i] = sgn[i] m[idx[i]], (P_t y)[idx_t[i]] = sgn_t[i] y[i]:
+ * harmonic_symmetry_classes (src/operators/symmetry.jl:290-308). */
+int ffmm_opcache_put_harmonic_classes(ffmm_opcache cache, int nclass, const int32_t *far_class, const int32_t *idx,
+                                      const int8_t *sgn, const int32_t *idx_t, const int8_t *sgn_t);
+
+/* The operators of boxes of one width (host FP64, column major): the scalar folded M2M
+ * (N x 8N), L2L (8N x N) and the nclass M2L representatives (N x N each, consecutive,
+ * c_K / width included): level_operators(method, width) (src/operators/harmonics.jl:357-385,
+ * src/operators/harmonics_yukawa.jl:219-231).  For Stokes they are the scalar Laplace
+ * operators (src/operators/harmonics_stokes.jl:74-82); the dense 4N x 32N M2M and L2L with
+ * the coupling of the fourth potential (_stokes_dense_operators,
+ * src/operators/harmonics_stokes.jl:112-131) are formed here.  Laplace and Stokes take
+ * one width, w_ref, for every level (homogeneous: the M2L of level l is scaled by
+ * w_ref / w_l, rescale, src/operators/harmonics.jl:393-398); Yukawa takes the width of
+ * every level of a plan before the plan's first pass. */
+int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const double *M2M, const double *L2L,
+                                    const double *reps);
+
+/* has = 1 when the cache holds the operators of boxes of this width (else 0). */
+int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has);
+
 /* The plan from device points xyz (3 x n AoS, FP64, any order) in the box (origin, width):
  * DevicePlan(m, xs; ppl, bbox, cache, max_levels) (src/device_passes.jl:113-134) with the
  * device tree device_tree (src/tree/device_builder.jl:398-474), its lists
@@ -131,8 +168,15 @@ int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *s
  * device_build_expansions!(plan, ds; sorted) (src/device_passes.jl:511-597). */
 int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted);
 
+/* The same from densities of nd components per point (dens: nd x N, point major):
+ * nd = 1 charges (any method); for Stokes nd = 3 stokeslets f, or nd = 9 a stokeslet f
+ * plus a stresslet (f, nu) at every point (the tuple densities of the reference,
+ * (SVector{3}, Stresslet{3}), src/sources.jl:55-66; src/operators/harmonics_stokes.jl:197-217). */
+int ffmm_build_expansions_nd(ffmm_plan plan, const double *dens, int nd, int sorted);
+
 /* Output at every source, in the user's order, from all other sources:
- * evaluate_at_sources! (src/device.jl:153-163).  out: N (potential) or 3 x N AoS (gradient). */
+ * evaluate_at_sources! (src/device.jl:153-163).  out: N (potential) or 3 x N AoS (gradient).
+ * Stokes plans: FFMM_POTENTIAL is the velocity (3 x N AoS, src/sources.jl:31-43). */
 int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output);
 
 /* Output at nt device targets (3 x nt AoS), located on the device: evaluate!

@@ -72,6 +72,15 @@ def lib():
         L.om2l_gather.argtypes = [f64p, ctypes.c_int, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p]
         L.om2l_scatter.argtypes = [f64p, ctypes.c_int, i64p, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p,
                                    ctypes.c_double]
+        tabs = [f64p] * 6
+        L.oh_s2m.argtypes = [ctypes.c_int, ctypes.c_int, *tabs, i32p, ctypes.c_int64, f64p, i32p, f64p, f64p, i32p,
+                             i32p, f64p, f64p, ctypes.c_int, f64p]
+        L.oh_l2t.argtypes = [ctypes.c_int, ctypes.c_int, *tabs, f64p, i32p, ctypes.c_int64, f64p, f64p, i32p, f64p,
+                             f64p, ctypes.c_int, f64p]
+        L.oh_near.argtypes = [ctypes.c_int, ctypes.c_double, f64p, i32p, i32p, ctypes.c_int64, i32p, i32p, i32p,
+                              f64p, f64p, ctypes.c_int, ctypes.c_int, f64p]
+        L.oh_direct.argtypes = [ctypes.c_int, ctypes.c_double, f64p, f64p, ctypes.c_int, ctypes.c_int64, f64p,
+                                ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, f64p]
         _LIB = L
     return _LIB
 

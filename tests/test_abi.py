@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), f"{n} is declared in include/ffmm.h but not exported"
     assert set(names) == set(_lib.SIGNATURES), "the ctypes binding and the header disagree"
-    assert _lib.lib().ffmm_abi_version() == 1
+    assert _lib.lib().ffmm_abi_version() == 2
 
 
 def test_status_codes_and_messages_without_a_device():

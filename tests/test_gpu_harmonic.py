@@ -1,0 +1,121 @@
+"""The harmonic methods on the GPU through the C ABI: SphericalHarmonics (Laplace3D,
+Yukawa3D) and StokesHarmonics against the harmonic oracle (same tree, same operators,
+FP64 throughout: agreement to rounding) and against FP64 direct sums (the method's
+truncation error)."""
+import numpy as np
+import pytest
+
+import harmonic as HO
+from pointsets import mixed, uniform
+
+pytestmark = pytest.mark.gpu
+
+BOX = (np.zeros(3), 1.0)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_for(method, cache, pts, ppl, perm):
+    from ffmm import Yukawa3D, StokesHarmonics
+    kind = "stokes" if isinstance(method, StokesHarmonics) else "yukawa" if isinstance(method.kernel, Yukawa3D) else "laplace"
+    kappa = method.kernel.k if kind == "yukawa" else 0.0
+    W0 = cache.W0
+
+    def level_ops(l, w):
+        ops = cache.harmonic_operators(w)
+        return ops, (1.0 if kind == "yukawa" else W0 / w)
+
+    return HO.HarmonicFMM(kind, level_ops, pts, ppl, *BOX, kappa=kappa, perm=perm), kind, kappa
+
+
+@pytest.mark.parametrize("kernel", ["laplace", "yukawa"])
+def test_spherical_harmonics_vs_oracle_and_direct(kernel):
+    import ffmm
+    rng = np.random.default_rng(11)
+    n = 5000
+    pts = mixed(rng, n)
+    tau = rng.standard_normal(n)
+    tgt = uniform(rng, 400)
+    K = ffmm.Laplace3D() if kernel == "laplace" else ffmm.Yukawa3D(5.0)
+    m = ffmm.SphericalHarmonics(K, p=10)
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] {kernel} n={n} p=10", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    u = plan.matvec(tau, ffmm.PotentialGradient).cpu().numpy()
+    v = plan.evaluate(tgt, ffmm.PotentialGradient).cpu().numpy()
+    up = plan.evaluate_at_sources(ffmm.Potential).cpu().numpy()
+    ug = plan.evaluate_at_sources(ffmm.Gradient).cpu().numpy()
+    tree = plan.download_tree()
+    f, kind, kappa = oracle_for(m, cache, pts, 40, tree.perm)
+    assert f.tree.nnodes == plan.nnodes
+    uo, vo = f.matvec(tau, output=2, targets=tgt)
+    # device and oracle: the same algorithm in FP64
+    assert rel(u, uo) < 1e-11 and rel(v, vo) < 1e-11
+    assert np.allclose(up, u[:, 0], rtol=1e-13, atol=1e-13 * np.max(np.abs(up)))
+    assert np.allclose(ug, u[:, 1:], rtol=1e-13, atol=1e-13 * np.max(np.abs(ug)))
+    # multipoles and the leaves' locals against the oracle's
+    M, Pl, Lc = plan.download_expansions()
+    assert np.max(np.abs(M.T - f.multipoles)) <= 1e-11 * np.max(np.abs(f.multipoles))
+    leaves = np.nonzero(tree.leaf)[0]
+    lv = tree.level[leaves]
+    order = leaves[np.argsort(lv, kind="stable")]
+    assert np.max(np.abs(Lc.T - f.locals[order])) <= 1e-11 * np.max(np.abs(f.locals))
+    # the truncation error of p = 10
+    ud = HO.direct(kind, pts, tau, pts, 2, kappa=kappa, skip=np.arange(n))
+    vd = HO.direct(kind, pts, tau, tgt, 2, kappa=kappa)
+    assert rel(u[:, 0], ud[:, 0]) < 2e-5 and rel(v[:, 0], vd[:, 0]) < 2e-5
+    assert rel(u[:, 1:], ud[:, 1:]) < 1e-4 and rel(v[:, 1:], vd[:, 1:]) < 1e-4
+
+
+@pytest.mark.parametrize("nd", [3, 9])
+def test_stokes_harmonics_vs_oracle_and_direct(nd):
+    import ffmm
+    rng = np.random.default_rng(12)
+    n = 4000
+    pts = mixed(rng, n)
+    dens = rng.standard_normal((n, nd))
+    if nd == 9:
+        dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+    tgt = uniform(rng, 300)
+    m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] stokes nd={nd} n={n} p=8", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    u = plan.matvec(dens, ffmm.Velocity).cpu().numpy()
+    v = plan.evaluate(tgt, ffmm.Velocity).cpu().numpy()
+    assert u.shape == (n, 3) and v.shape == (300, 3)
+    f, kind, _ = oracle_for(m, cache, pts, 40, plan.download_tree().perm)
+    uo, vo = f.matvec(dens, targets=tgt)
+    assert rel(u, uo) < 1e-11 and rel(v, vo) < 1e-11
+    assert rel(u, HO.direct("stokes", pts, dens, pts, skip=np.arange(n))) < 1e-3
+    assert rel(v, HO.direct("stokes", pts, dens, tgt)) < 1e-3
+
+
+def test_harmonic_sorted_densities_linearity_and_errors():
+    import ffmm
+    rng = np.random.default_rng(13)
+    n = 3000
+    pts = uniform(rng, n)
+    m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=6)
+    plan = ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX)
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    ua = plan.matvec(a).cpu().numpy()
+    ub = plan.matvec(b).cpu().numpy()
+    uab = plan.matvec(2 * a - 3 * b).cpu().numpy()
+    assert np.allclose(uab, 2 * ua - 3 * ub, rtol=1e-12, atol=1e-12 * np.max(np.abs(uab)))
+    perm = plan.download_tree().perm
+    us = plan.matvec(a[perm], sorted=True).cpu().numpy()
+    assert np.array_equal(us, ua)
+    with pytest.raises(ffmm.ArgumentError, match="dense class M2L"):
+        ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX, m2l="spectral")
+    with pytest.raises(ffmm.ArgumentError, match="FP64"):
+        ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX, precision="mixed")
+    ms = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=4)
+    ps = ffmm.DevicePlan(ms, pts, ppl=30, bbox=BOX)
+    with pytest.raises(ffmm.ArgumentError, match="stokeslets"):
+        ps.build_expansions(a)
+    ps.build_expansions(rng.standard_normal((n, 3)))
+    with pytest.raises(ffmm.ArgumentError, match="velocity"):
+        ps.evaluate_at_sources(ffmm.Gradient)
