@@ -68,6 +68,107 @@ def cpu_baseline(n_sample: int, ppl: int):
                       f"potential at the sources, FP64, {dt:.1f} s"}
 
 
+def serving_stream(nreq: int, nmin: int, nmax: int, rate: float, seed: int = 5):
+    """The served mixed-size stream: Poisson arrivals at ``rate`` per second, point
+    counts log-uniform in [nmin, nmax], 70% Laplace3D cube p = 8 / 30% Yukawa3D(5)
+    harmonics, uniform or Gaussian-mixture layouts, charges N(0, 1), 40% repeats of an
+    earlier geometry with new charges, half in the 1e-6 class (cube p = 8 mixed
+    spectral, Yukawa p = 10) and half in the 1e-3 class (cube p = 4, Yukawa p = 4)."""
+    from ffmm import workloads as W
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs, geoms = [], []
+    for k in range(nreq):
+        lap = rng.random() < 0.7
+        fine = rng.random() < 0.5
+        if geoms and rng.random() < 0.4:
+            g = geoms[rng.integers(len(geoms))]
+            lap, fine, pts, gid = g["lap"], g["fine"], g["pts"], g["gid"]
+        else:
+            n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+            pts = W.gaussian_mixture(n, seed=int(rng.integers(1 << 30))) if rng.random() < 0.5 else \
+                W.uniform(n, seed=int(rng.integers(1 << 30)))
+            gid = len(geoms)
+            geoms.append(dict(lap=lap, fine=fine, pts=pts, gid=gid))
+        reqs.append(dict(t=float(t[k]), lap=lap, fine=fine, pts=pts, gid=gid,
+                         tau=rng.standard_normal(len(pts))))
+    return reqs
+
+
+def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
+    """Serve the stream through the server (arrivals replayed in real time by a
+    submitting thread) and solve the same requests one after another with plain plans
+    (a new DevicePlan per request, no reuse); report p50/p99 latency and throughput."""
+    import threading
+    import ffmm
+    from ffmm.serving import Server
+    box = (np.zeros(3), 1.0)
+    methods = {(True, True): ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8),
+               (True, False): ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=4),
+               (False, True): ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10),
+               (False, False): ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=4)}
+    caches = {k: ffmm.OperatorCache(m) for k, m in methods.items()}
+    form = {(True, True): ("mixed", "spectral"), (True, False): ("f64", "dense"),
+            (False, True): ("f64", "dense"), (False, False): ("f64", "dense")}
+    reqs = serving_stream(nreq, nmin, nmax, rate)
+    total = float(sum(len(r["pts"]) for r in reqs))
+    log(f"[bench] serving: {nreq} requests, {total:.3e} points, {nslots} slots")
+    srv = Server(nslots=nslots, hbm_budget=96 << 30)
+    # warm the operator caches (host precompute and device copies, Yukawa widths of 14 levels)
+    for k, c in caches.items():
+        c.fit_box(np.zeros(3), np.ones(3), box)
+        srv._prepare(c, np.zeros(3), np.ones(3), box)
+    tickets = [None] * nreq
+    t_sub = [0.0] * nreq
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            dt = r["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            key = (r["lap"], r["fine"])
+            prec, m2l = form[key]
+            t_sub[i] = time.perf_counter()
+            tickets[i] = srv.submit(caches[key], r["pts"], r["tau"], geometry=r["gid"], ppl=64, bbox=box,
+                                    precision=prec, m2l=m2l)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = []
+    for i in range(nreq):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        _, _, ms = srv.wait(tickets[i])
+        lat.append(ms)
+    t_served = time.perf_counter() - t0
+    th.join()
+    st = srv.stats()
+    srv.close()
+    # the same requests one after another, plain plans, no reuse and no arrival gaps
+    torch = __import__("torch")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for r in reqs:
+        key = (r["lap"], r["fine"])
+        prec, m2l = form[key]
+        plan = ffmm.DevicePlan(methods[key], r["pts"], ppl=64, bbox=box, cache=caches[key], precision=prec, m2l=m2l)
+        plan.matvec(r["tau"]).cpu()
+        del plan
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t1
+    lat = np.array(lat)
+    return {"requests": nreq, "points": total, "slots": nslots, "arrival_rate_per_s": rate,
+            "n_range": [nmin, nmax], "p50_latency_ms": float(np.percentile(lat, 50)),
+            "p99_latency_ms": float(np.percentile(lat, 99)), "served_points_per_s": total / t_served,
+            "served_wall_s": t_served, "sequential_points_per_s": total / t_seq, "sequential_wall_s": t_seq,
+            "speedup_vs_sequential": t_seq / t_served, "plans_built": st["plans_built"],
+            "plans_reused": st["plans_reused"], "peak_device_bytes": st["peak_bytes"],
+            "note": "arrivals replayed in real time (the served wall time includes the arrival gaps); "
+                    "the sequential leg solves back to back with a new plan per request"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +182,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nerr", type=int, default=10_000)
+    ap.add_argument("--serve-requests", type=int, default=60, help="served-stream requests (0: skip the leg)")
+    ap.add_argument("--serve-nmin", type=int, default=10_000)
+    ap.add_argument("--serve-nmax", type=int, default=200_000)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus N with N > 1 is not supported "
@@ -192,6 +298,12 @@ def main():
                 "eval_sources": {"flops": flops_eval, "ms": stages["eval_sources"],
                                  "achieved_tflops": flops_eval / (stages["eval_sources"] * 1e-3) / 1e12}}
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_bench(args.serve_requests, args.serve_nmin, args.serve_nmax, args.serve_rate,
+                                args.serve_slots)
+        log("[bench] serving: " + json.dumps(serving))
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle on {args.cpu_sample} points")
@@ -210,7 +322,7 @@ def main():
             "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
             "tree": {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
                      "leaves": int(plan.level_leaves.sum())},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "serving": serving}
     print(json.dumps(line), flush=True)
 
 

@@ -3,10 +3,10 @@
 // colleagues, the target's own source excluded by index (a masked select, never a
 // multiply).
 //
-// One wavefront per real leaf, one lane per target: the surface, the leaf's local
-// density and the colleagues' sources are read at wave-uniform addresses, so every
-// lane of the wave consumes the same stream (scalar loads through the constant cache)
-// and the 64 lanes only differ in their target.  Targets at arbitrary points are
+// One wavefront per real leaf, one lane per target: the surface with the leaf's local
+// density, then the colleagues' sources, pass through LDS tiles that the wave loads
+// with coalesced reads and every lane then reads at the same address (a broadcast),
+// so the 64 lanes only differ in their target.  Targets at arbitrary points are
 // located by a device walk with the builder's child-slot rule and bucketed by leaf
 // with a stable sort, so a leaf's targets are contiguous.
 //
@@ -20,6 +20,7 @@ namespace ffmm {
 
 constexpr int TB = 256;
 constexpr int EV_TB = 64;
+constexpr int EV_CK = 256;   // entries of an LDS tile: surface points with densities, or sources with charges
 constexpr double C4PI = 0.07957747154594767;   // 1 / (4 pi)
 
 template <int OUT>
@@ -47,6 +48,7 @@ __device__ __forceinline__ void accumulate_masked(double d0, double d1, double d
 
 template <int OUT>
 __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
+  __shared__ double4 tile[EV_CK];
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
   const bool self = a.tptr == nullptr;
@@ -78,17 +80,39 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
     const double z = self ? a.sz[jj] : a.tz[jj];
     const double xr = x - c0, yr = y - c1, zr = z - c2;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    // l2t over the large surface (src/operators/ki_common.jl:293-315)
-    for (int k = rep; k < a.Nl; k += R)
-      accumulate<OUT>(xr - X[k] * sc, yr - Y[k] * sc, zr - Z[k] * sc, L[k], acc);
-    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363)
+    // l2t over the large surface (src/operators/ki_common.jl:293-315): tiles of the
+    // scaled surface points and the leaf's densities in LDS, read by every lane
+    for (int k0 = 0; k0 < a.Nl; k0 += EV_CK) {
+      const int nk = min(EV_CK, a.Nl - k0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < nk; k += EV_TB)
+        tile[k] = make_double4(X[k0 + k] * sc, Y[k0 + k] * sc, Z[k0 + k] * sc, L[k0 + k]);
+      __syncthreads();
+#pragma unroll 4
+      for (int k = rep; k < nk; k += R) {
+        const double4 v = tile[k];
+        accumulate<OUT>(xr - v.x, yr - v.y, zr - v.z, v.w, acc);
+      }
+    }
+    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363), their
+    // sources staged through the same tiles
     const int32_t skip = self ? j : -1;
     for (int m = 0; m < NCOLL; m++) {
       const int32_t nb = nbr[m];
       if (nb < 0) continue;
       const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
-      for (int32_t q = p0 + rep; q <= p1; q += R)
-        accumulate_masked<OUT>(x - a.sx[q], y - a.sy[q], z - a.sz[q], a.tau[q], q != skip, acc);
+      for (int32_t q0 = p0; q0 <= p1; q0 += EV_CK) {
+        const int nq = min(EV_CK, p1 - q0 + 1);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nq; k += EV_TB)
+          tile[k] = make_double4(a.sx[q0 + k], a.sy[q0 + k], a.sz[q0 + k], a.tau[q0 + k]);
+        __syncthreads();
+#pragma unroll 4
+        for (int k = rep; k < nq; k += R) {
+          const double4 v = tile[k];
+          accumulate_masked<OUT>(x - v.x, y - v.y, z - v.z, v.w, q0 + k != skip, acc);
+        }
+      }
     }
     for (int off = ntp; off < EV_TB; off <<= 1)
 #pragma unroll

@@ -8,35 +8,54 @@
 #include <cstring>
 #include <memory>
 
-struct ffmm_context_s { ffmm::Context c; };
-struct ffmm_opcache_s { ffmm::OpCache c; };
-struct ffmm_plan_s { ffmm::Plan c; };
 
 namespace ffmm {
 
-static thread_local std::string g_last_error;
+thread_local std::string g_last_error;
 
 void fail(int code, const std::string &msg) { throw Error(code, msg); }
 
-template <class F>
-static int guard(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return FFMM_OK;
-  } catch (const Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return FFMM_ERR_MEMORY;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return FFMM_ERR_STATE;
-  }
+void once_on_device(int device, int key, const std::function<void()> &upload) {
+  static std::mutex mu;
+  static std::vector<std::pair<int, int>> done;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto &d : done)
+    if (d.first == device && d.second == key) return;
+  upload();
+  done.emplace_back(device, key);
 }
 
 static void set_device(Context &ctx) { FFMM_HIP(hipSetDevice(ctx.device)); }
+
+void context_init(Context &c, int device, void *stream) {
+  int ndev = 0;
+  FFMM_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) fail(FFMM_ERR_ARGUMENT, "device " + std::to_string(device) + " of " + std::to_string(ndev));
+  c.device = device;
+  set_device(c);
+  if (stream) {
+    c.stream = (hipStream_t)stream;
+  } else {
+    FFMM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.own_stream = true;
+  }
+  FFMM_BLAS(rocblas_create_handle(&c.blas));
+  FFMM_BLAS(rocblas_set_stream(c.blas, c.stream));
+  FFMM_BLAS(rocblas_set_pointer_mode(c.blas, rocblas_pointer_mode_host));
+  FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+}
+
+void context_release(Context &c) {
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  c.tmp.release();
+  if (c.pinned) (void)hipHostFree(c.pinned);
+  if (c.blas) rocblas_destroy_handle(c.blas);
+  if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+  c.pinned = nullptr;
+  c.blas = nullptr;
+  c.stream = nullptr;
+}
 
 // Stage timers record event pairs on the plan's stream without synchronising; the
 // times are read when asked for (every entry point has synchronised by then).  The
@@ -97,24 +116,8 @@ int ffmm_context_create(int device, void *stream, ffmm_context *out) {
   return guard([&] {
     if (!out) fail(FFMM_ERR_ARGUMENT, "null output handle");
     *out = nullptr;
-    int ndev = 0;
-    FFMM_HIP(hipGetDeviceCount(&ndev));
-    if (device < 0 || device >= ndev)
-      fail(FFMM_ERR_ARGUMENT, "device " + std::to_string(device) + " of " + std::to_string(ndev));
     auto h = std::make_unique<ffmm_context_s>();
-    Context &c = h->c;
-    c.device = device;
-    set_device(c);
-    if (stream) {
-      c.stream = (hipStream_t)stream;
-    } else {
-      FFMM_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-      c.own_stream = true;
-    }
-    FFMM_BLAS(rocblas_create_handle(&c.blas));
-    FFMM_BLAS(rocblas_set_stream(c.blas, c.stream));
-    FFMM_BLAS(rocblas_set_pointer_mode(c.blas, rocblas_pointer_mode_host));
-    FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+    context_init(h->c, device, stream);
     *out = h.release();
   });
 }
@@ -122,13 +125,7 @@ int ffmm_context_create(int device, void *stream, ffmm_context *out) {
 int ffmm_context_destroy(ffmm_context ctx) {
   return guard([&] {
     if (!ctx) return;
-    Context &c = ctx->c;
-    (void)hipSetDevice(c.device);
-    (void)hipStreamSynchronize(c.stream);
-    c.tmp.release();
-    if (c.pinned) (void)hipHostFree(c.pinned);
-    if (c.blas) rocblas_destroy_handle(c.blas);
-    if (c.own_stream) (void)hipStreamDestroy(c.stream);
+    context_release(ctx->c);
     delete ctx;
   });
 }
@@ -428,11 +425,9 @@ int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const doub
     }
     lv->reps.upload(reps, (size_t)o.nclass * N * N, s);
     ctx.sync();
+    std::lock_guard<std::mutex> lk(o.hmu);
     for (auto &h : o.hlev)
-      if (h->w == width) {
-        h = std::move(lv);
-        return;
-      }
+      if (h->w == width) return;   // the operators of a width are fixed once put
     o.hlev.push_back(std::move(lv));
   });
 }
@@ -444,83 +439,176 @@ int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has) {
   });
 }
 
+}  // extern "C"
+
+// ---- the plan's life cycle (shared by the C entry points and the server) ----
+namespace ffmm {
+
+void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
+               double width, int e0, int max_levels, int precision, int m2l) {
+  if (!o.ready) fail(FFMM_ERR_STATE, "the operator cache holds no operators");
+  if (n <= 0) fail(FFMM_ERR_ARGUMENT, "no points");
+  if (!xyz) fail(FFMM_ERR_ARGUMENT, "null points");
+  if (ppl < 1) fail(FFMM_ERR_ARGUMENT, "ppl must be positive");
+  if (!(width > 0)) fail(FFMM_ERR_ARGUMENT, "box width must be positive");
+  if (precision != FFMM_PREC_F64 && precision != FFMM_PREC_MIXED) fail(FFMM_ERR_ARGUMENT, "unknown precision");
+  if (m2l != FFMM_M2L_DENSE && m2l != FFMM_M2L_SPECTRAL) fail(FFMM_ERR_ARGUMENT, "unknown M2L form");
+  if (o.method == METHOD_HARMONIC) {
+    if (m2l != FFMM_M2L_DENSE) fail(FFMM_ERR_ARGUMENT, "the harmonic methods take the dense class M2L");
+    if (precision != FFMM_PREC_F64) fail(FFMM_ERR_ARGUMENT, "the harmonic methods run in FP64");
+    if (o.kernel != FFMM_KERNEL_YUKAWA && !o.find_width(o.w_ref))
+      fail(FFMM_ERR_STATE, "the cache holds no operators of the reference width");
+  }
+  if (m2l == FFMM_M2L_SPECTRAL && !o.has_tables) fail(FFMM_ERR_ARGUMENT, "the cache holds no spectral tables");
+  if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
+  if (ctx.device != o.ctx->device) fail(FFMM_ERR_ARGUMENT, "the cache lives on another device");
+  set_device(ctx);
+  p.ctx = &ctx;
+  p.ops = &o;
+  p.N = n;
+  p.ppl = ppl;
+  p.max_levels = max_levels;
+  p.precision = precision;
+  p.m2l = m2l;
+  for (int d = 0; d < 3; d++) p.origin[d] = origin[d];
+  p.W0 = width;
+  p.e0 = e0;
+  for (auto &e : p.ev) FFMM_HIP(hipEventCreate(&e));
+  {
+    StageTimer tm(p, 0);
+    build_tree(p, xyz);
+  }
+  p.lam.resize(p.L);
+  for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, e0 - l);
+  p.lam_d.upload(p.lam.data(), p.L, ctx.stream);
+  if (o.method == METHOD_HARMONIC) {
+    // per level: 1/w_l and kappa w_l (the extras' scale and kappa-hat, src/operators/harmonics.jl:327-336),
+    // the M2L factor (rescale, src/operators/harmonics.jl:393-398; 1 for Yukawa, whose
+    // operators are built per width), and the operators of the level's width
+    std::vector<double> hs(p.L), kh(p.L);
+    p.alpha.assign(p.L, 1.0);
+    p.hl.assign(p.L, nullptr);
+    for (int l = 0; l < p.L; l++) {
+      const double w = std::ldexp(width, -l);
+      hs[l] = 1.0 / w;
+      kh[l] = o.kappa * w;
+      if (o.kernel != FFMM_KERNEL_YUKAWA) {
+        p.alpha[l] = 1.0 / p.lam[l];
+        p.hl[l] = o.find_width(o.w_ref);
+      }
+    }
+    p.hscale.upload(hs.data(), p.L, ctx.stream);
+    p.khat.upload(kh.data(), p.L, ctx.stream);
+  }
+  {
+    // node -> leaf slot
+    std::vector<int32_t> slot(p.nnodes, -1), leaves(p.leaf_off[p.L]);
+    p.leaf_nodes.download(leaves.data(), leaves.size(), ctx.stream);
+    ctx.sync();
+    for (size_t s = 0; s < leaves.size(); s++) slot[leaves[s]] = (int32_t)s;
+    p.node_leafslot.upload(slot.data(), slot.size(), ctx.stream);
+  }
+  {
+    StageTimer tm(p, 1);
+    build_lists(p);
+  }
+  allocate_pass_buffers(p);
+  if (p.m2l == FFMM_M2L_SPECTRAL) spectral_prepare(p);
+  ctx.sync();
+}
+
+void plan_release(Plan &p) {
+  if (!p.ctx) return;
+  (void)hipSetDevice(p.ctx->device);
+  (void)hipStreamSynchronize(p.ctx->stream);
+  for (auto &e : p.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &v : p.lev_ev)
+    for (auto &e : v)
+      if (e) (void)hipEventDestroy(e);
+  for (auto &e : p.ev) e = nullptr;
+  p.lev_ev[0].clear();
+  p.lev_ev[1].clear();
+}
+
+void plan_build(Plan &p, const double *tau, int nd, int sorted) {
+  OpCache &o = *p.ops;
+  const bool stokes = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET;
+  if (stokes && nd != 3 && nd != 9)
+    fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
+  if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
+  set_device(*p.ctx);
+  if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {
+    // the operators of every level with a far field or a translation (2 .. L-1)
+    for (int l = 2; l < p.L; l++) {
+      const double w = std::ldexp(p.W0, -l);
+      p.hl[l] = o.find_width(w);
+      if (!p.hl[l])
+        fail(FFMM_ERR_STATE, "the cache holds no operators for boxes of width " + std::to_string(w) + " (level " +
+                                 std::to_string(l) + ")");
+    }
+  }
+  p.nd = nd;
+  p.expansions_ready = false;
+  {
+    StageTimer tm(p, 7);
+    gather_inputs(p, tau, sorted);
+  }
+  {
+    StageTimer tm(p, 2);
+    upward_pass(p);
+  }
+  for (auto &v : p.lev_recorded) std::fill(v.begin(), v.end(), 0);
+  {
+    StageTimer tm(p, 3);
+    downward_pass(p);
+  }
+  p.ctx->sync();
+  p.expansions_ready = true;
+}
+
+static void check_output(Plan &p, int output) {
+  if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
+  if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+  if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
+    fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
+}
+
+void plan_eval_sources(Plan &p, double *out, int output) {
+  check_output(p, output);
+  set_device(*p.ctx);
+  {
+    StageTimer tm(p, 5);
+    evaluate_at_sources(p, out, output);
+  }
+  p.ctx->sync();
+}
+
+void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output) {
+  check_output(p, output);
+  if (nt < 0) fail(FFMM_ERR_DIMENSION, "negative target count");
+  if (nt == 0) return;
+  if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+  set_device(*p.ctx);
+  {
+    StageTimer tm(p, 6);
+    evaluate_targets(p, targets, nt, out, output);
+  }
+  p.ctx->sync();
+}
+
+}  // namespace ffmm
+
+extern "C" {
+
 int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, const double origin[3], double width,
                      int e0, int max_levels, int precision, int m2l, ffmm_plan *out) {
   return guard([&] {
     if (!cache || !out || !origin) fail(FFMM_ERR_ARGUMENT, "null argument");
     *out = nullptr;
     OpCache &o = cache->c;
-    if (!o.ready) fail(FFMM_ERR_STATE, "the operator cache holds no operators");
-    if (n <= 0) fail(FFMM_ERR_ARGUMENT, "no points");
-    if (!xyz) fail(FFMM_ERR_ARGUMENT, "null points");
-    if (ppl < 1) fail(FFMM_ERR_ARGUMENT, "ppl must be positive");
-    if (!(width > 0)) fail(FFMM_ERR_ARGUMENT, "box width must be positive");
-    if (precision != FFMM_PREC_F64 && precision != FFMM_PREC_MIXED) fail(FFMM_ERR_ARGUMENT, "unknown precision");
-    if (m2l != FFMM_M2L_DENSE && m2l != FFMM_M2L_SPECTRAL) fail(FFMM_ERR_ARGUMENT, "unknown M2L form");
-    if (o.method == METHOD_HARMONIC) {
-      if (m2l != FFMM_M2L_DENSE) fail(FFMM_ERR_ARGUMENT, "the harmonic methods take the dense class M2L");
-      if (precision != FFMM_PREC_F64) fail(FFMM_ERR_ARGUMENT, "the harmonic methods run in FP64");
-      if (o.kernel != FFMM_KERNEL_YUKAWA && !o.find_width(o.w_ref))
-        fail(FFMM_ERR_STATE, "the cache holds no operators of the reference width");
-    }
-    if (m2l == FFMM_M2L_SPECTRAL && !o.has_tables) fail(FFMM_ERR_ARGUMENT, "the cache holds no spectral tables");
-    if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
-    Context &ctx = *o.ctx;
-    set_device(ctx);
     auto h = std::make_unique<ffmm_plan_s>();
-    Plan &p = h->c;
-    p.ctx = &ctx;
-    p.ops = &o;
-    p.N = n;
-    p.ppl = ppl;
-    p.max_levels = max_levels;
-    p.precision = precision;
-    p.m2l = m2l;
-    for (int d = 0; d < 3; d++) p.origin[d] = origin[d];
-    p.W0 = width;
-    p.e0 = e0;
-    for (auto &e : p.ev) FFMM_HIP(hipEventCreate(&e));
-    {
-      StageTimer tm(p, 0);
-      build_tree(p, xyz);
-    }
-    p.lam.resize(p.L);
-    for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, e0 - l);
-    p.lam_d.upload(p.lam.data(), p.L, ctx.stream);
-    if (o.method == METHOD_HARMONIC) {
-      // per level: 1/w_l and kappa w_l (the extras' scale and kappa-hat, src/operators/harmonics.jl:327-336),
-      // the M2L factor (rescale, src/operators/harmonics.jl:393-398; 1 for Yukawa, whose
-      // operators are built per width), and the operators of the level's width
-      std::vector<double> hs(p.L), kh(p.L);
-      p.alpha.assign(p.L, 1.0);
-      p.hl.assign(p.L, nullptr);
-      for (int l = 0; l < p.L; l++) {
-        const double w = std::ldexp(width, -l);
-        hs[l] = 1.0 / w;
-        kh[l] = o.kappa * w;
-        if (o.kernel != FFMM_KERNEL_YUKAWA) {
-          p.alpha[l] = 1.0 / p.lam[l];
-          p.hl[l] = o.find_width(o.w_ref);
-        }
-      }
-      p.hscale.upload(hs.data(), p.L, ctx.stream);
-      p.khat.upload(kh.data(), p.L, ctx.stream);
-    }
-    {
-      // node -> leaf slot
-      std::vector<int32_t> slot(p.nnodes, -1), leaves(p.leaf_off[p.L]);
-      p.leaf_nodes.download(leaves.data(), leaves.size(), ctx.stream);
-      ctx.sync();
-      for (size_t s = 0; s < leaves.size(); s++) slot[leaves[s]] = (int32_t)s;
-      p.node_leafslot.upload(slot.data(), slot.size(), ctx.stream);
-    }
-    {
-      StageTimer tm(p, 1);
-      build_lists(p);
-    }
-    allocate_pass_buffers(p);
-    if (p.m2l == FFMM_M2L_SPECTRAL) spectral_prepare(p);
-    ctx.sync();
+    plan_init(h->c, *o.ctx, o, xyz, n, ppl, origin, width, e0, max_levels, precision, m2l);
     *out = h.release();
   });
 }
@@ -528,13 +616,7 @@ int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, 
 int ffmm_plan_destroy(ffmm_plan plan) {
   return guard([&] {
     if (!plan) return;
-    (void)hipSetDevice(plan->c.ctx->device);
-    (void)hipStreamSynchronize(plan->c.ctx->stream);
-    for (auto &e : plan->c.ev)
-      if (e) (void)hipEventDestroy(e);
-    for (auto &v : plan->c.lev_ev)
-      for (auto &e : v)
-        if (e) (void)hipEventDestroy(e);
+    plan_release(plan->c);
     delete plan;
   });
 }
@@ -631,76 +713,21 @@ int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted) {
 int ffmm_build_expansions_nd(ffmm_plan plan, const double *tau, int nd, int sorted) {
   return guard([&] {
     if (!plan || !tau) fail(FFMM_ERR_ARGUMENT, "null argument");
-    Plan &p = plan->c;
-    OpCache &o = *p.ops;
-    const bool stokes = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET;
-    if (stokes && nd != 3 && nd != 9)
-      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
-    if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
-    set_device(*p.ctx);
-    if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {
-      // the operators of every level with a far field or a translation (2 .. L-1)
-      for (int l = 2; l < p.L; l++) {
-        const double w = std::ldexp(p.W0, -l);
-        p.hl[l] = o.find_width(w);
-        if (!p.hl[l])
-          fail(FFMM_ERR_STATE, "the cache holds no operators for boxes of width " + std::to_string(w) + " (level " +
-                                   std::to_string(l) + ")");
-      }
-    }
-    p.nd = nd;
-    {
-      StageTimer tm(p, 7);
-      gather_inputs(p, tau, sorted);
-    }
-    {
-      StageTimer tm(p, 2);
-      upward_pass(p);
-    }
-    for (auto &v : p.lev_recorded) std::fill(v.begin(), v.end(), 0);
-    {
-      StageTimer tm(p, 3);
-      downward_pass(p);
-    }
-    p.ctx->sync();
-    p.expansions_ready = true;
+    plan_build(plan->c, tau, nd, sorted);
   });
 }
 
 int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output) {
   return guard([&] {
     if (!plan || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
-    Plan &p = plan->c;
-    if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
-    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
-    if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
-      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
-    set_device(*p.ctx);
-    {
-      StageTimer tm(p, 5);
-      evaluate_at_sources(p, out, output);
-    }
-    p.ctx->sync();
+    plan_eval_sources(plan->c, out, output);
   });
 }
 
 int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out, int output) {
   return guard([&] {
     if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
-    Plan &p = plan->c;
-    if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
-    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
-    if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
-      fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
-    if (nt < 0) fail(FFMM_ERR_DIMENSION, "negative target count");
-    if (nt == 0) return;
-    if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
-    set_device(*p.ctx);
-    {
-      StageTimer tm(p, 6);
-      evaluate_targets(p, targets, nt, out, output);
-    }
-    p.ctx->sync();
+    plan_eval_targets(plan->c, targets, nt, out, output);
   });
 }
 

@@ -5,7 +5,9 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -28,6 +30,28 @@ struct Error : std::runtime_error {
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 [[noreturn]] void fail(int code, const std::string &msg);
+
+// the message of the last failure on this thread (ffmm_last_error)
+extern thread_local std::string g_last_error;
+
+// runs f, turning the library's exceptions into a status code and a message
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return FFMM_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return FFMM_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return FFMM_ERR_STATE;
+  }
+}
 
 #define FFMM_HIP(call)                                                                          \
   do {                                                                                          \
@@ -52,6 +76,11 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
   y = fma(y, fma(-hx * y, y, 0.5), y);
   return y;
 }
+
+// runs upload() once per device and key, under a lock, so that no caller returns
+// before the upload is done (the constant tables, which concurrent plans on other
+// streams read)
+void once_on_device(int device, int key, const std::function<void()> &upload);
 
 // workgroups covering n work-items (at least one: every kernel checks its index against n)
 inline unsigned nblocks(int64_t n, int b) { return n > 0 ? (unsigned)((n + b - 1) / b) : 1u; }
@@ -153,8 +182,13 @@ struct OpCache {
   DBuf<double> htab;                         // a, b, cz, cm, cp (ntri each), e (p)
   DBuf<int8_t> sgn_t;                        // target-side signs [f * N + i] (the cube: = sgn)
   DBuf<double> offs;                         // 316 x 3 far offsets (the Stokes coupling)
+  // the operators by width; entries are added, never replaced or removed while the
+  // cache lives, so plans may hold pointers to them (guarded: a server's slots look
+  // widths up while the host adds new ones)
   std::vector<std::unique_ptr<HarmonicLevel>> hlev;
+  mutable std::mutex hmu;
   const HarmonicLevel *find_width(double w) const {
+    std::lock_guard<std::mutex> lk(hmu);
     for (auto &h : hlev)
       if (h->w == w) return h.get();
     return nullptr;
@@ -306,6 +340,18 @@ struct EvalArgs {
   double kappa;
 };
 
+// ---- the life cycles (ffmm_abi.hip), shared by the C entry points and the server ----
+void context_init(Context &c, int device, void *stream);
+void context_release(Context &c);
+void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
+               double width, int e0, int max_levels, int precision, int m2l);
+void plan_release(Plan &p);
+void plan_build(Plan &p, const double *dens, int nd, int sorted);
+void plan_eval_sources(Plan &p, double *out, int output);
+void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output);
+// device bytes held by a plan
+size_t plan_bytes(const Plan &p);
+
 // ---- harmonic methods (harmonic.hip) ----
 void harmonic_s2m_level(Plan &plan, int l);
 void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output);
@@ -319,3 +365,14 @@ struct StageTimer {
 };
 
 }  // namespace ffmm
+
+// the opaque handles of include/ffmm.h
+struct ffmm_context_s {
+  ffmm::Context c;
+};
+struct ffmm_opcache_s {
+  ffmm::OpCache c;
+};
+struct ffmm_plan_s {
+  ffmm::Plan c;
+};

@@ -145,7 +145,7 @@ void build_lists(Plan &plan) {
   Context &ctx = *plan.ctx;
   OpCache &ops = *plan.ops;
   hipStream_t s = ctx.stream;
-  {
+  once_on_device(ctx.device, 1, [&] {
     int32_t fi[343];
     for (int u = 0, nf = 0; u < 343; u++) {
       int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
@@ -154,7 +154,7 @@ void build_lists(Plan &plan) {
     }
     FFMM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_far_index), fi, sizeof fi, 0, hipMemcpyHostToDevice, s));
     ctx.sync();
-  }
+  });
   const int nclass = ops.nclass;
   if (nclass > MAX_CLASSES) fail(FFMM_ERR_ARGUMENT, "too many symmetry classes");
   TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};

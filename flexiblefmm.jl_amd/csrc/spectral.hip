@@ -308,7 +308,7 @@ static void upload_spectral_constants() {
 void spectral_prepare(Plan &plan) {
   OpCache &o = *plan.ops;
   if (o.p != SP_P) fail(FFMM_ERR_ARGUMENT, "the spectral M2L is built for p = 8");
-  upload_spectral_constants();
+  once_on_device(plan.ctx->device, 2, upload_spectral_constants);
   int64_t nmax = 1;
   for (int l = 2; l < plan.L; l++) nmax = std::max(nmax, plan.nl[l]);
   if (plan.precision == FFMM_PREC_MIXED) {

@@ -200,6 +200,54 @@ int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *pl
  * (src/device_passes.jl:331-336). */
 int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n);
 
+/* ---- serving: continuous batching of solves on one device ----
+ * No reference counterpart.  The nearest pieces of the reference are the plan and
+ * operator reuse of DevicePlan/OperatorCache (src/device_passes.jl:136-146,
+ * src/operators/cache.jl:5-46) and build_expansions! on a fixed geometry
+ * (src/fmm.jl:264-298).  A server runs nslots worker slots, each with its own stream;
+ * a free slot admits the first queued request whose device-memory estimate fits
+ * hbm_budget beside the running solves and the resident plans (least recently used
+ * resident plans are evicted to make room; a request waiting longer than 2 s stops
+ * later requests from overtaking it).  Requests with the same cache and geometry key
+ * (>= 0) share one resident plan: after the first, they skip the tree and list build.
+ * Host buffers of a request must stay valid until ffmm_server_wait returns for it.
+ * The caches must not be modified while a server uses them (Yukawa: put the widths
+ * of every level the requests can reach before submitting). */
+typedef struct ffmm_server_s *ffmm_server;
+
+typedef struct {
+  ffmm_opcache cache;     /* the method's operators, on the server's device */
+  int64_t geometry;       /* reuse key (-1: none) */
+  const double *points;   /* host, 3 x n AoS */
+  int64_t n;
+  int ppl;
+  double origin[3];       /* the box, already snapped: width = w_ref * 2^e0 of the cache */
+  double width;
+  int e0;
+  int max_levels;         /* 0: 30 */
+  int precision;          /* FFMM_PREC_* */
+  int m2l;                /* FFMM_M2L_* */
+  const double *dens;     /* host, nd x n */
+  int nd;
+  const double *targets;  /* host, 3 x nt AoS, or NULL */
+  int64_t nt;
+  int output;             /* FFMM_POTENTIAL, FFMM_GRADIENT, FFMM_POTENTIAL_GRADIENT */
+  double *out_sources;    /* host, ncomp x n (NULL: none) */
+  double *out_targets;    /* host, ncomp x nt */
+} ffmm_request;
+
+int ffmm_server_create(int device, int nslots, int64_t hbm_budget, ffmm_server *out);
+/* Queue a request; *ticket identifies it. */
+int ffmm_server_submit(ffmm_server server, const ffmm_request *req, int64_t *ticket);
+/* Block until the request is done; returns its status (its message: ffmm_last_error)
+ * and its latency from submission to the last output copy. */
+int ffmm_server_wait(ffmm_server server, int64_t ticket, double *latency_ms);
+/* [0] completed, [1] plans built, [2] resident plans reused, [3] resident plans,
+ * [4] device bytes held or reserved, [5] their peak, [6] evictions, [7] queued. */
+int ffmm_server_stats(ffmm_server server, int64_t *stats, int n);
+/* Finish every queued request, then release the slots and the resident plans. */
+int ffmm_server_destroy(ffmm_server server);
+
 #ifdef __cplusplus
 }
 #endif
