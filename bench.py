@@ -181,12 +181,16 @@ def main():
     ap.add_argument("--m2l", default="spectral", choices=["dense", "spectral"])
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 dense step")
     ap.add_argument("--nerr", type=int, default=10_000)
     ap.add_argument("--serve-requests", type=int, default=60, help="served-stream requests (0: skip the leg)")
     ap.add_argument("--serve-nmin", type=int, default=10_000)
     ap.add_argument("--serve-nmax", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="before the steps, copy a buffer of this size once on the device (a kernel of known "
+                         "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus N with N > 1 is not supported "
@@ -211,6 +215,15 @@ def main():
     m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
     cache = ffmm.OperatorCache(m)
     box = (np.zeros(3), 1.0)
+    if args.copy_probe_gib > 0:
+        nb = int(args.copy_probe_gib * (1 << 30)) // 8
+        a_ = torch.ones(nb, dtype=torch.float64, device=dev)
+        b_ = torch.empty_like(a_)
+        torch.cuda.synchronize()
+        torch.mul(a_, 2.0, out=b_)          # one elementwise kernel: reads a_, writes b_
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {8 * nb} bytes read and {8 * nb} written by one elementwise (mul) kernel")
+        del a_, b_
     torch.cuda.synchronize()
 
     def step():
@@ -248,6 +261,31 @@ def main():
         plan.evaluate_at_sources(ffmm.Potential, out=u_d)
     torch.cuda.synchronize()
     matvec_ms = 1e3 * (time.perf_counter() - t) / args.steps
+
+    # the plain FP64 form of the reference device plan (dense class GEMMs) on the same
+    # workload: one warm-up and one timed step, for the mixed-over-plain ratio
+    plain = None
+    if not args.no_plain and (args.precision, args.m2l) != ("f64", "dense"):
+        def plain_step():
+            p = ffmm.DevicePlan(m, xs_d, ppl=args.ppl, bbox=box, cache=cache, precision="f64", m2l="dense")
+            p.build_expansions(tau_d)
+            p.evaluate_at_sources(ffmm.Potential, out=u_d)
+            p.evaluate(tgt_d, ffmm.PotentialGradient, out=v_d)
+            return p
+        plan = None
+        p0 = plain_step()
+        del p0
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        p0 = plain_step()
+        torch.cuda.synchronize()
+        plain_ms = 1e3 * (time.perf_counter() - t)
+        del p0
+        plan = step()   # the measured form's plan again, for the checks below
+        torch.cuda.synchronize()
+        plain = {"form": "f64 dense (class GEMMs, the reference device plan's form)", "ms_per_step": plain_ms,
+                 "speedup_of_measured_form": plain_ms / ms}
+        log(f"[bench] plain FP64 dense step: {plain_ms:.1f} ms ({plain_ms / ms:.2f}x the measured form)")
 
     # accuracy on sampled points against the FP64 direct sum
     rng = np.random.default_rng(123)
@@ -322,7 +360,7 @@ def main():
             "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
             "tree": {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
                      "leaves": int(plan.level_leaves.sum())},
-            "roofline": roofline, "cpu_baseline": cpu, "serving": serving}
+            "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "plain_f64": plain}
     print(json.dumps(line), flush=True)
 
 

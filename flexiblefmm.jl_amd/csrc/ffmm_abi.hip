@@ -12,6 +12,44 @@
 namespace ffmm {
 
 thread_local std::string g_last_error;
+thread_local hipStream_t g_alloc_stream = nullptr;
+
+hipStream_t device_alloc(void **p, size_t bytes) {
+  hipStream_t s = g_alloc_stream;
+  hipError_t e;
+  if (s) {
+    int dev = 0;
+    FFMM_HIP(hipGetDevice(&dev));
+    once_on_device(dev, 3, [&] {
+      // keep freed blocks in the default pool for reuse
+      hipMemPool_t pool;
+      FFMM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+      uint64_t keep = UINT64_MAX;
+      FFMM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    });
+    e = hipMallocAsync(p, bytes, s);
+  } else {
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    fail(FFMM_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  return s;
+}
+
+void device_free(void *p, hipStream_t owner) {
+  if (!p) return;
+  if (owner) {
+    // ordered after the work on the current scope's stream, or on the stream it was
+    // allocated on
+    hipStream_t s = g_alloc_stream ? g_alloc_stream : owner;
+    (void)hipFreeAsync(p, s);
+  } else {
+    (void)hipFree(p);
+  }
+}
 
 void fail(int code, const std::string &msg) { throw Error(code, msg); }
 
@@ -463,6 +501,7 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
   if (ctx.device != o.ctx->device) fail(FFMM_ERR_ARGUMENT, "the cache lives on another device");
   set_device(ctx);
+  AllocScope scope(ctx.stream);
   p.ctx = &ctx;
   p.ops = &o;
   p.N = n;
@@ -538,6 +577,7 @@ void plan_build(Plan &p, const double *tau, int nd, int sorted) {
     fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
   if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
   set_device(*p.ctx);
+  AllocScope scope(p.ctx->stream);
   if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {
     // the operators of every level with a far field or a translation (2 .. L-1)
     for (int l = 2; l < p.L; l++) {
@@ -577,6 +617,7 @@ static void check_output(Plan &p, int output) {
 void plan_eval_sources(Plan &p, double *out, int output) {
   check_output(p, output);
   set_device(*p.ctx);
+  AllocScope scope(p.ctx->stream);
   {
     StageTimer tm(p, 5);
     evaluate_at_sources(p, out, output);
@@ -590,6 +631,7 @@ void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, 
   if (nt == 0) return;
   if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
   set_device(*p.ctx);
+  AllocScope scope(p.ctx->stream);
   {
     StageTimer tm(p, 6);
     evaluate_targets(p, targets, nt, out, output);

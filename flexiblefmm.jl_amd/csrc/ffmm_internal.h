@@ -85,34 +85,50 @@ void once_on_device(int device, int key, const std::function<void()> &upload);
 // workgroups covering n work-items (at least one: every kernel checks its index against n)
 inline unsigned nblocks(int64_t n, int b) { return n > 0 ? (unsigned)((n + b - 1) / b) : 1u; }
 
+// Device allocations.  Inside an AllocScope they are stream ordered on the scope's
+// stream (hipMallocAsync / hipFreeAsync from the device's default pool, which keeps
+// freed memory for reuse instead of returning it: plans of the same size come and go
+// without synchronising the device); outside one they are plain hipMalloc / hipFree.
+extern thread_local hipStream_t g_alloc_stream;
+struct AllocScope {
+  hipStream_t prev;
+  explicit AllocScope(hipStream_t s) : prev(g_alloc_stream) { g_alloc_stream = s; }
+  ~AllocScope() { g_alloc_stream = prev; }
+  AllocScope(const AllocScope &) = delete;
+  AllocScope &operator=(const AllocScope &) = delete;
+};
+// returns the stream the memory is ordered on (null: synchronous)
+hipStream_t device_alloc(void **p, size_t bytes);
+void device_free(void *p, hipStream_t owner);
+
 // A device array; resize() discards the contents, grow() keeps them.
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0, cap = 0;
+  hipStream_t owner = nullptr;   // the stream its allocation is ordered on (null: synchronous)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap) { o.p = nullptr; o.n = o.cap = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap), owner(o.owner) { o.p = nullptr; o.n = o.cap = 0; }
   DBuf &operator=(DBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; cap = o.cap; o.p = nullptr; o.n = o.cap = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; cap = o.cap; owner = o.owner;
+      o.p = nullptr; o.n = o.cap = 0;
+    }
     return *this;
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) device_free(p, owner);
     p = nullptr;
     n = cap = 0;
   }
   void resize(size_t m) {
     if (m > cap) {
       release();
-      size_t bytes = (m ? m : 1) * sizeof(T);
-      hipError_t e = hipMalloc(&p, bytes);
-      if (e != hipSuccess) {
-        p = nullptr;
-        fail(FFMM_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-      }
+      owner = device_alloc((void **)&p, (m ? m : 1) * sizeof(T));
       cap = m;
     }
     n = m;
@@ -122,12 +138,12 @@ struct DBuf {
     size_t c = cap ? cap : 16;
     while (c < m) c *= 2;
     T *q = nullptr;
-    hipError_t e = hipMalloc(&q, c * sizeof(T));
-    if (e != hipSuccess) fail(FFMM_ERR_MEMORY, "device allocation failed while growing a buffer");
+    hipStream_t qo = device_alloc((void **)&q, c * sizeof(T));
     if (n) FFMM_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     FFMM_HIP(hipStreamSynchronize(s));
-    if (p) (void)hipFree(p);
+    if (p) device_free(p, owner);
     p = q;
+    owner = qo;
     cap = c;
     n = m;
   }

@@ -19,6 +19,7 @@
 namespace ffmm {
 
 constexpr int HB = 64;
+constexpr int HT_CK = 128;   // sources per LDS tile of the near field
 constexpr double HC4PI = 0.07957747154594767;    // 1 / (4 pi)
 constexpr double HC8PI = 0.039788735772973836;   // 1 / (8 pi)
 
@@ -267,12 +268,11 @@ __device__ __forceinline__ void dcoef(const double *L, int p, const HTab &tb, in
 
 // near field of one source j at d = x - s_j (self excluded by the select)
 template <int KIND, int OUT>
-__device__ __forceinline__ void near_term(double d0, double d1, double d2, const double *__restrict__ dens, int nd,
-                                          int32_t j, bool keep, double kappa, double acc[4]) {
+__device__ __forceinline__ void near_term(double d0, double d1, double d2, const double *f, int nd, bool keep,
+                                          double kappa, double acc[4]) {
   const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
   const double ir = rsqrt_f64(r2);
   if (KIND == 2) {
-    const double *f = dens + (int64_t)nd * j;
     const double a = HC8PI * ir;
     const double b = a * ir * ir * (d0 * f[0] + d1 * f[1] + d2 * f[2]);
     double u0 = a * f[0] + b * d0, u1 = a * f[1] + b * d1, u2 = a * f[2] + b * d2;
@@ -289,7 +289,7 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
     acc[2] += keep ? u2 : 0.0;
     return;
   }
-  const double q = dens[j];
+  const double q = f[0];
   double v, dv;
   if (KIND == 0) {
     v = HC4PI * ir;
@@ -316,6 +316,8 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
 template <int KIND, int OUT>
 __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   extern __shared__ double sm[];
+  constexpr int TW = KIND == 2 ? 12 : 4;   // doubles per tile entry: position and density components
+  __shared__ double tile[HT_CK * TW];
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
   const bool self = a.tptr == nullptr;
@@ -448,14 +450,29 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
         acc[o + 2] = (g2 + zh * pr) * sc;
       }
     }
-    // the near field of the 27 colleagues (src/device.jl:89-100)
+    // the near field of the 27 colleagues (src/device.jl:89-100), their sources and
+    // densities staged through LDS tiles (broadcast reads)
     const int32_t skip = self ? j : -1;
     for (int m = 0; m < NCOLL; m++) {
       const int32_t nb2 = nbr[m];
       if (nb2 < 0) continue;
       const int32_t p0 = a.plo[nb2], p1 = a.phi[nb2];
-      for (int32_t q = p0 + rep; q <= p1; q += R)
-        near_term<KIND, OUT>(x - a.sx[q], y - a.sy[q], z - a.sz[q], a.tau, a.nd, q, q != skip, a.kappa, acc);
+      for (int32_t q0 = p0; q0 <= p1; q0 += HT_CK) {
+        const int nq = min(HT_CK, p1 - q0 + 1);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nq; k += HB) {
+          double *e = tile + k * TW;
+          e[0] = a.sx[q0 + k];
+          e[1] = a.sy[q0 + k];
+          e[2] = a.sz[q0 + k];
+          for (int c = 0; c < a.nd; c++) e[3 + c] = a.tau[(int64_t)(q0 + k) * a.nd + c];
+        }
+        __syncthreads();
+        for (int k = rep; k < nq; k += R) {
+          const double *e = tile + k * TW;
+          near_term<KIND, OUT>(x - e[0], y - e[1], z - e[2], e + 3, a.nd, q0 + k != skip, a.kappa, acc);
+        }
+      }
     }
     for (int off = ntp; off < HB; off <<= 1)
 #pragma unroll

@@ -142,7 +142,8 @@ __global__ void k_gather_perm_nd(double *__restrict__ dst, const double *__restr
 // on each block (nb columns per pair, as the reference lays out the Stokes blocks).
 // Gc[i, q nb + b] = sgn[f, i] Mp[b N + idx[f, i], src[q]]  (P_f^T m, block b)
 // ---------------------------------------------------------------------------
-__global__ void k_class_gather(double *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
+template <class T>
+__global__ void k_class_gather(T *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
                                const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int64_t npk,
                                const int32_t *__restrict__ idx, const int8_t *__restrict__ sgn) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,13 +153,14 @@ __global__ void k_class_gather(double *__restrict__ Gc, const double *__restrict
   int64_t q = c / nb;
   int b = (int)(c % nb);
   int64_t f = cfar[q];
-  Gc[g] = (double)sgn[f * N + i] * Mp[(int64_t)csrc[q] * Ns + b * N + idx[f * N + i]];
+  Gc[g] = (T)((double)sgn[f * N + i] * Mp[(int64_t)csrc[q] * Ns + b * N + idx[f * N + i]]);
 }
 // target-owned scatter of P_t y over the runs of a class that fall in the piece
 // [a, a + npk): pL[b N + j, rtgt[r]] += alpha sum_p sgn_t[f, i] Y_b[i, p - a], i = inv[f, j];
 // for Stokes (nb = 4) the fourth block adds the coupling Psi += o_f . Phi with the
 // pair's own offset (src/device_passes.jl:449-473, src/operators/harmonics.jl:604-628)
-__global__ void k_class_scatter(double *__restrict__ pL, const double *__restrict__ Yc, int Ns, int N, int nb,
+template <class T>
+__global__ void k_class_scatter(double *__restrict__ pL, const T *__restrict__ Yc, int Ns, int N, int nb,
                                 const int32_t *__restrict__ rtgt, const int32_t *__restrict__ rptr, int64_t r0,
                                 int64_t nr, const int32_t *__restrict__ cfar, int64_t a, int64_t npk,
                                 const int32_t *__restrict__ inv, const int8_t *__restrict__ sgn_t,
@@ -174,9 +176,10 @@ __global__ void k_class_scatter(double *__restrict__ pL, const double *__restric
   for (int64_t p = p0; p < p1; p++) {
     int64_t f = cfar[p];
     int i = inv[f * N + j];
-    const double *Y = Yc + (p - a) * nb * N;
-    double y = Y[b * N + i];
-    if (nb == 4 && b == 3) y = y + offs[3 * f] * Y[i] + offs[3 * f + 1] * Y[N + i] + offs[3 * f + 2] * Y[2 * N + i];
+    const T *Y = Yc + (p - a) * nb * N;
+    double y = (double)Y[b * N + i];
+    if (nb == 4 && b == 3)
+      y = y + offs[3 * f] * (double)Y[i] + offs[3 * f + 1] * (double)Y[N + i] + offs[3 * f + 2] * (double)Y[2 * N + i];
     acc += (double)sgn_t[f * N + i] * y;
   }
   pL[(int64_t)rtgt[r] * Ns + jb] += alpha * acc;
@@ -187,6 +190,14 @@ static void gemm(Context &ctx, int m, int n, int k, double alpha, const double *
   if (m == 0 || n == 0) return;
   FFMM_BLAS(rocblas_dgemm(ctx.blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb,
                           &beta, C, ldc));
+}
+
+static void gemm32(Context &ctx, int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
+                   int ldc) {
+  if (m == 0 || n == 0) return;
+  const float one = 1.0f, zero = 0.0f;
+  FFMM_BLAS(rocblas_sgemm(ctx.blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero,
+                          C, ldc));
 }
 
 static constexpr int64_t PIECE = 1 << 18;   // pairs per dense M2L GEMM
@@ -215,7 +226,12 @@ void allocate_pass_buffers(Plan &plan) {
   plan.G.resize((size_t)Ns * std::max(nq, ng));
   int64_t pmax = 1;
   for (auto &li : plan.lists) pmax = std::max(pmax, std::min<int64_t>(PIECE, li.np));
-  if (plan.m2l == FFMM_M2L_DENSE) {
+  if (plan.m2l == FFMM_M2L_DENSE && plan.precision == FFMM_PREC_MIXED) {
+    // the mixed form: the class GEMMs in FP32 (FP32 MFMA), gathered from and scattered
+    // into the FP64 expansions
+    plan.Gc32.resize((size_t)Ns * pmax);
+    plan.Yc32.resize((size_t)Ns * pmax);
+  } else if (plan.m2l == FFMM_M2L_DENSE) {
     plan.Gc.resize((size_t)Ns * pmax);
     plan.Yc.resize((size_t)Ns * pmax);
   }
@@ -283,18 +299,28 @@ void m2l_dense_level(Plan &plan, int l) {
   // m2l_scale of the level: (w_l / w_ref)^(-1) for the homogeneous kernels, 1 for Yukawa
   const double alpha = harm ? plan.alpha[l] : 1.0 / plan.lam[l];
   const double *reps = harm ? plan.hl[l]->reps.p : o.class_ops.p;
+  const bool f32 = !harm && plan.precision == FFMM_PREC_MIXED;
   for (int c = 0; c < o.nclass; c++) {
     const int64_t pa = li.class_pair[c], pb = li.class_pair[c + 1];
     const int64_t r0 = li.class_run[c], nr = li.class_run[c + 1] - r0;
     for (int64_t a = pa; a < pb; a += PIECE) {
       const int64_t npk = std::min(PIECE, pb - a);
-      k_class_gather<<<nblocks((int64_t)N * nb * npk, TB), TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a,
-                                                                       li.cfar.p + a, npk, o.idx.p, o.sgn.p);
-      FFMM_LAUNCH_CHECK();
-      gemm(ctx, N, (int)(nb * npk), N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p, N, 0.0, plan.Yc.p, N);
-      k_class_scatter<<<nblocks((int64_t)Ns * nr, TB), TB, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0,
-                                                                   nr, li.cfar.p, a, npk, o.inv.p, o.sgn_t.p,
-                                                                   o.offs.p, alpha);
+      const unsigned gg = nblocks((int64_t)N * nb * npk, TB), gs = nblocks((int64_t)Ns * nr, TB);
+      if (f32) {
+        k_class_gather<float><<<gg, TB, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a, li.cfar.p + a, npk,
+                                                o.idx.p, o.sgn.p);
+        FFMM_LAUNCH_CHECK();
+        gemm32(ctx, N, (int)(nb * npk), N, o.class_ops32.p + (size_t)c * N * N, N, plan.Gc32.p, N, plan.Yc32.p, N);
+        k_class_scatter<float><<<gs, TB, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0, nr, li.cfar.p, a,
+                                                 npk, o.inv.p, o.sgn_t.p, o.offs.p, alpha);
+      } else {
+        k_class_gather<double><<<gg, TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a, li.cfar.p + a, npk,
+                                                 o.idx.p, o.sgn.p);
+        FFMM_LAUNCH_CHECK();
+        gemm(ctx, N, (int)(nb * npk), N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p, N, 0.0, plan.Yc.p, N);
+        k_class_scatter<double><<<gs, TB, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0, nr, li.cfar.p, a,
+                                                  npk, o.inv.p, o.sgn_t.p, o.offs.p, alpha);
+      }
       FFMM_LAUNCH_CHECK();
     }
   }

@@ -112,6 +112,7 @@ struct ffmm_server_s {
   std::deque<ffmm::Request *> queue;
   std::map<int64_t, std::unique_ptr<ffmm::Request>> reqs;
   std::vector<std::unique_ptr<ffmm::Resident>> resident;
+  std::vector<std::pair<ffmm::OpCache *, int64_t>> building;   // geometries whose plan a slot is building
   int64_t in_use = 0, peak = 0;
   int64_t next_ticket = 1;
   uint64_t clock = 0;
@@ -160,12 +161,16 @@ static Request *pick(ffmm_server_s &S, Resident *&res, std::vector<std::unique_p
       S.queue.erase(it);
       return q;
     }
+    if (q->r.geometry >= 0 &&
+        std::find(S.building.begin(), S.building.end(), std::make_pair(q->cache, q->r.geometry)) != S.building.end())
+      continue;                          // its plan is being built: wait for it to become resident
     while (S.in_use + q->est > S.budget && evict_one(S, graveyard)) {
     }
     if (S.in_use + q->est <= S.budget || S.running == 0) {
       S.in_use += q->est;
       S.peak = std::max(S.peak, S.in_use);
       res = nullptr;
+      if (q->r.geometry >= 0) S.building.emplace_back(q->cache, q->r.geometry);
       S.queue.erase(it);
       return q;
     }
@@ -180,6 +185,7 @@ static void run_request(Worker &w, Request &q, Plan *plan, bool fresh) {
   OpCache &o = *q.cache;
   hipStream_t s = ctx.stream;
   FFMM_HIP(hipSetDevice(ctx.device));
+  AllocScope scope(s);
   if (fresh) {
     w.xyz.resize(3 * (size_t)r.n);
     FFMM_HIP(hipMemcpyAsync(w.xyz.p, r.points, sizeof(double) * 3 * r.n, hipMemcpyHostToDevice, s));
@@ -266,6 +272,8 @@ static void worker_loop(ffmm_server_s *S, Worker *w) {
     } else {
       S->in_use -= q->est;
       if (status == FFMM_OK) S->built++;
+      auto bi = std::find(S->building.begin(), S->building.end(), std::make_pair(q->cache, q->r.geometry));
+      if (bi != S->building.end()) S->building.erase(bi);
       // two slots may have built the same geometry at once: keep the first
       if (keep && find_resident(*S, q->cache, q->r.geometry)) keep = false;
       if (keep) {

@@ -17,6 +17,8 @@
 // (src/tree/lists.jl:96-124).  p = 8 (n = 16) is fixed here.
 #include "ffmm_internal.h"
 
+#include <type_traits>
+
 #include <cmath>
 
 namespace ffmm {
@@ -236,6 +238,87 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T
   }
 }
 
+// The FP32 form with two adjacent frequencies per lane: one 16-byte table read and one
+// 16-byte spectrum load feed twice the multiply-adds of the kernel above, and the
+// far-index table holds row offsets (far index x row pitch) so that no address is
+// multiplied in the inner loop.  FB / 2 lanes share a candidate, R = 128 / FB
+// candidates per step; lane (ph, r) writes sibling r's two frequencies.
+template <int FB>
+__global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restrict__ S, float2 *__restrict__ U,
+                                                          const float2 *__restrict__ tables,
+                                                          const int16_t *__restrict__ pairf,
+                                                          const int32_t *__restrict__ gptr,
+                                                          const int32_t *__restrict__ cand, int64_t ngroups, int gpw) {
+  constexpr int LPC = FB / 2;              // lanes per candidate
+  constexpr int R = 64 / LPC;              // candidates per step
+  constexpr int TS = FB + 2;               // padded table row, a multiple of two entries (16 bytes)
+  constexpr int NW = HD_TB / 64;
+  constexpr int MAXC = NCOLL * NCHILD;
+  static_assert(R == NCHILD, "one sibling per candidate slot at the write-out");
+  __shared__ __attribute__((aligned(16))) float2 Tt[(NFAR + 1) * TS];
+  __shared__ __attribute__((aligned(16))) int16_t F[NCOLL * NCHILD * NCHILD];
+  __shared__ int32_t Cl[NW][MAXC + 64];
+  const int f0 = blockIdx.y * FB;
+  const int t = threadIdx.x;
+  for (int i = t; i < (NFAR + 1) * TS; i += HD_TB) {
+    const int f = i / TS, ph = i % TS;
+    Tt[i] = (f < NFAR && ph < FB) ? tables[(int64_t)f * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
+  }
+  for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HD_TB)
+    F[i] = (int16_t)((pairf[i] < 0 ? NFAR : pairf[i]) * TS);
+  __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  const int ph = 2 * (lane % LPC), r = lane / LPC;
+  int32_t *cl = Cl[wave];
+  const int64_t g0 = ((int64_t)blockIdx.x * NW + wave) * gpw;
+  for (int64_t g = g0; g < g0 + gpw && g < ngroups; g++) {
+    const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;
+    if (n == 0) continue;
+    for (int i = lane; i < n; i += 64) cl[i] = cand[cbeg + i];
+    __builtin_amdgcn_wave_barrier();
+    float4 acc[NCHILD];
+#pragma unroll
+    for (int a = 0; a < NCHILD; a++) acc[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+    int32_t e = cl[r < n ? r : 0];
+    float4 sv = *reinterpret_cast<const float4 *>(&S[(int64_t)(e >> 8) * SP_NF + f0 + ph]);
+    bool live = r < n;
+    for (int j = 0; j < n; j += R) {
+      const int jn = j + R + r;
+      const int32_t en = cl[jn < n ? jn : 0];
+      const float4 svn = *reinterpret_cast<const float4 *>(&S[(int64_t)(en >> 8) * SP_NF + f0 + ph]);
+      const int16_t *Fk = &F[(e & 255) * NCHILD];
+      const float4 v = live ? sv : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int a = 0; a < NCHILD; a++) {
+        const float4 tt = *reinterpret_cast<const float4 *>(&Tt[Fk[a] + ph]);
+        acc[a].x = fmaf(tt.x, v.x, fmaf(-tt.y, v.y, acc[a].x));
+        acc[a].y = fmaf(tt.x, v.y, fmaf(tt.y, v.x, acc[a].y));
+        acc[a].z = fmaf(tt.z, v.z, fmaf(-tt.w, v.w, acc[a].z));
+        acc[a].w = fmaf(tt.z, v.w, fmaf(tt.w, v.z, acc[a].w));
+      }
+      e = en;
+      sv = svn;
+      live = jn < n;
+    }
+#pragma unroll
+    for (int a = 0; a < NCHILD; a++) {
+#pragma unroll
+      for (int off = LPC; off < 64; off <<= 1) {
+        acc[a].x += __shfl_xor(acc[a].x, off, 64);
+        acc[a].y += __shfl_xor(acc[a].y, off, 64);
+        acc[a].z += __shfl_xor(acc[a].z, off, 64);
+        acc[a].w += __shfl_xor(acc[a].w, off, 64);
+      }
+    }
+    float4 o = acc[0];
+#pragma unroll
+    for (int b = 1; b < NCHILD; b++)
+      if (b == r) o = acc[b];
+    *reinterpret_cast<float4 *>(&U[(NCHILD * g + r) * SP_NF + f0 + ph]) = o;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 3. inverse: x = IFFT(U_t) on the [0, p)^3 corner, surface entries into pL
 // ---------------------------------------------------------------------------
@@ -361,8 +444,12 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   const int64_t gpb = (HD_TB / 64) * gpw;
   dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
   StageTimer tm(plan, 8, l);
-  k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p, ng,
-                                               gpw);
+  if constexpr (std::is_same<T, float>::value)
+    k_spec_hadamard2<FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                ng, gpw);
+  else
+    k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                 ng, gpw);
   FFMM_LAUNCH_CHECK();
   const double alpha = (1.0 / plan.lam[l]) / (double)(SP_N * SP_N * SP_N);
   k_spec_inverse<T><<<(unsigned)nl, SP_TB, 0, s>>>(U, plan.lists[l].ptr.p, o.sub8.p, o.Ns, alpha,

@@ -55,7 +55,7 @@ def test_device_tree_matches_fixture(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,m2l", [("f64", "dense"), ("f64", "spectral"), ("mixed", "spectral")])
+@pytest.mark.parametrize("precision,m2l", [("f64", "dense"), ("f64", "spectral"), ("mixed", "spectral"), ("mixed", "dense")])
 def test_device_fmm_matches_direct_fixture(precision, m2l):
     import ffmm
     fx = load("fmm_mixed2000.npz")

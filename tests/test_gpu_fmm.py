@@ -73,6 +73,23 @@ def test_matvec_spectral_vs_oracle_and_direct(setup, precision, tol_oracle, tol_
     assert mrd(Ps.ravel(), Pd.ravel()) < (1e-12 if precision == "f64" else 1e-6)
 
 
+def test_matvec_dense_fp32_classes_vs_oracle_and_direct(setup):
+    """The mixed form of the dense class M2L: FP32 gather, FP32 GEMM (sgemm on FP32
+    MFMA), scatter into the FP64 partial locals; pseudo-inverses, M2M and L2L FP64."""
+    s = setup
+    ffmm = s["ffmm"]
+    print("[fmm] dense mixed", flush=True)
+    plan = ffmm.DevicePlan(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"], m2l="dense", precision="mixed")
+    u = plan.matvec(s["tau"]).cpu().numpy()
+    assert mrd(u, s["u_ref"]) < 2e-6
+    rows = s["rng"].choice(len(u), 400, replace=False)
+    assert mrd(u[rows], O.direct_self(s["pts"], s["tau"], rows=rows)) < 2e-6
+    s["plan"].build_expansions(s["tau"])
+    _, Pd, _ = s["plan"].download_expansions()
+    _, Pm, _ = plan.download_expansions()
+    assert mrd(Pm.ravel(), Pd.ravel()) < 1e-5
+
+
 def test_expansions_vs_oracle(setup):
     """Expansions against the oracle summing in the device's point order.  The
     equivalent densities (multipoles, locals) come out of pseudo-inverses that keep
