@@ -49,6 +49,17 @@ def interaction_counts(tree, nleaf_targets=None):
     return leaves, cnt[leaves], n[leaves]
 
 
+def traffic_from_profile(kname: str, args):
+    """HBM bytes per launch of the dominant kernel from the committed two-pass
+    rocprofv3 --pmc profile (FETCH_SIZE, then WRITE_SIZE, each scaled by the copy
+    probe's reading), when the profile is of this kernel at this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "hadamard2_4M_mixed_traffic.json")
+    if not os.path.exists(path) or args.n != 4_000_000 or args.ppl != 64:
+        return None
+    prof = json.load(open(path))
+    return prof["traffic_bytes_per_launch"] if kname.startswith(prof["kernel"]) else None
+
+
 def cpu_baseline(n_sample: int, ppl: int):
     """The oracle's restated reference CPU FMM on a Gaussian-mixture sample of the same
     generator: tree + passes + potential at the sources, points per second."""
@@ -320,7 +331,8 @@ def main():
         flops = float(pairs) * nfreq * 8
         t_k = stages["hadamard"] * 1e-3
         peak = PEAK_FP64_TFLOPS if args.precision == "f64" else PEAK_FP32_TFLOPS
-        kname = "k_spec_hadamard (spectral M2L Hadamard products)"
+        kname = ("k_spec_hadamard2<16>" if args.precision == "mixed" else "k_spec_hadamard<double, 8>") + \
+            " (spectral M2L Hadamard products)"
     else:
         flops = float(pairs) * 2 * m.nsurface ** 2
         t_k = stages["m2l"] * 1e-3
@@ -329,7 +341,7 @@ def main():
     achieved = flops / t_k / 1e12
     nlaunch = int(np.count_nonzero(plan.level_pairs))
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "kernel": kname, "launches": nlaunch,
+                "traffic": traffic_from_profile(kname, args), "kernel": kname, "launches": nlaunch,
                 "avg_launch_ms": 1e3 * t_k / max(nlaunch, 1), "flops_per_launch": flops / max(nlaunch, 1),
                 "note": "VALU FMA kernel: MI355X's vector FP64 and FP64-MFMA peaks are both 78.6 TFLOP/s "
                         "(vector FP32 without packed math: 78.6 TFLOP/s as well)",

@@ -46,7 +46,7 @@ def build(jobs: int = 8) -> str:
         objs = list(ex.map(compile_one, sources()))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas",
-           "-Wl,-rpath,/opt/rocm/lib"]
+           "-Wl,-rpath,/opt/rocm/lib", "-Wl,--no-undefined"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -49,6 +49,8 @@ __device__ __forceinline__ void accumulate_masked(double d0, double d1, double d
 template <int OUT>
 __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
   __shared__ double4 tile[EV_CK];
+  // blocks in leaf order, dealt round-robin to the XCDs: the leaves' work varies by
+  // level (they are stored level by level), and round-robin keeps the XCDs balanced
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
   const bool self = a.tptr == nullptr;
@@ -143,9 +145,10 @@ static void launch_eval(Plan &plan, EvalArgs &a, int output) {
     return;
   }
   hipStream_t s = plan.ctx->stream;
-  if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
-  else if (output == FFMM_GRADIENT) k_eval<FFMM_GRADIENT><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
-  else k_eval<2><<<(unsigned)nleaves, EV_TB, 0, s>>>(a);
+  const unsigned g = (unsigned)nleaves;
+  if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<g, EV_TB, 0, s>>>(a);
+  else if (output == FFMM_GRADIENT) k_eval<FFMM_GRADIENT><<<g, EV_TB, 0, s>>>(a);
+  else k_eval<2><<<g, EV_TB, 0, s>>>(a);
   FFMM_LAUNCH_CHECK();
 }
 

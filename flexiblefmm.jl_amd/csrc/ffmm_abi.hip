@@ -12,23 +12,37 @@
 namespace ffmm {
 
 thread_local std::string g_last_error;
-thread_local hipStream_t g_alloc_stream = nullptr;
+thread_local Context *g_alloc_ctx = nullptr;
 
-hipStream_t device_alloc(void **p, size_t bytes) {
-  hipStream_t s = g_alloc_stream;
-  hipError_t e;
-  if (s) {
-    int dev = 0;
-    FFMM_HIP(hipGetDevice(&dev));
-    once_on_device(dev, 3, [&] {
-      // keep freed blocks in the default pool for reuse
-      hipMemPool_t pool;
-      FFMM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-      uint64_t keep = UINT64_MAX;
-      FFMM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
-    });
-    e = hipMallocAsync(p, bytes, s);
-  } else {
+void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+static void flush_blocks(Context &c) {
+  std::lock_guard<std::mutex> lk(c.blocks->mu);
+  for (auto &kv : c.blocks->blocks) (void)hipFree(kv.second);
+  c.blocks->blocks.clear();
+  c.blocks->bytes = 0;
+}
+
+Context *device_alloc(void **p, size_t bytes, size_t *blk) {
+  Context *c = g_alloc_ctx;
+  if (c) {
+    std::lock_guard<std::mutex> lk(c->blocks->mu);
+    auto &bl = c->blocks->blocks;
+    // the smallest idle block that fits and wastes at most half of itself (or 64 MiB)
+    auto it = bl.lower_bound(bytes);
+    if (it != bl.end() && it->first <= std::max(2 * bytes, bytes + ((size_t)64 << 20))) {
+      *p = it->second;
+      *blk = it->first;
+      c->blocks->bytes -= it->first;
+      bl.erase(it);
+      return c;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && c) {
+    // the cached blocks may be what is missing: give them back and try once more
+    (void)hipGetLastError();
+    flush_blocks(*c);
     e = hipMalloc(p, bytes);
   }
   if (e != hipSuccess) {
@@ -36,22 +50,28 @@ hipStream_t device_alloc(void **p, size_t bytes) {
     (void)hipGetLastError();
     fail(FFMM_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
   }
-  return s;
+  *blk = bytes;
+  return c;
 }
 
-void device_free(void *p, hipStream_t owner) {
+void device_free(void *p, size_t blk, Context *owner) {
   if (!p) return;
   if (owner) {
-    // ordered after the work on the current scope's stream, or on the stream it was
-    // allocated on
-    hipStream_t s = g_alloc_stream ? g_alloc_stream : owner;
-    (void)hipFreeAsync(p, s);
+    std::lock_guard<std::mutex> lk(owner->blocks->mu);
+    BlockCache &bc = *owner->blocks;
+    bc.blocks.emplace(blk, p);
+    bc.bytes += blk;
+    // keep at most the limit: return the largest idle blocks first
+    while (bc.bytes > bc.limit && !bc.blocks.empty()) {
+      auto last = std::prev(bc.blocks.end());
+      (void)hipFree(last->second);
+      bc.bytes -= last->first;
+      bc.blocks.erase(last);
+    }
   } else {
     (void)hipFree(p);
   }
 }
-
-void fail(int code, const std::string &msg) { throw Error(code, msg); }
 
 void once_on_device(int device, int key, const std::function<void()> &upload) {
   static std::mutex mu;
@@ -81,12 +101,17 @@ void context_init(Context &c, int device, void *stream) {
   FFMM_BLAS(rocblas_set_stream(c.blas, c.stream));
   FFMM_BLAS(rocblas_set_pointer_mode(c.blas, rocblas_pointer_mode_host));
   FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+  // idle plan memory kept for reuse: up to half of the device's memory
+  size_t fr = 0, tot = 0;
+  FFMM_HIP(hipMemGetInfo(&fr, &tot));
+  c.blocks->limit = tot / 2;
 }
 
 void context_release(Context &c) {
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   c.tmp.release();
+  flush_blocks(c);
   if (c.pinned) (void)hipHostFree(c.pinned);
   if (c.blas) rocblas_destroy_handle(c.blas);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
@@ -501,7 +526,7 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
   if (ctx.device != o.ctx->device) fail(FFMM_ERR_ARGUMENT, "the cache lives on another device");
   set_device(ctx);
-  AllocScope scope(ctx.stream);
+  AllocScope scope(ctx);
   p.ctx = &ctx;
   p.ops = &o;
   p.N = n;
@@ -577,7 +602,7 @@ void plan_build(Plan &p, const double *tau, int nd, int sorted) {
     fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
   if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
   set_device(*p.ctx);
-  AllocScope scope(p.ctx->stream);
+  AllocScope scope(*p.ctx);
   if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {
     // the operators of every level with a far field or a translation (2 .. L-1)
     for (int l = 2; l < p.L; l++) {
@@ -617,7 +642,7 @@ static void check_output(Plan &p, int output) {
 void plan_eval_sources(Plan &p, double *out, int output) {
   check_output(p, output);
   set_device(*p.ctx);
-  AllocScope scope(p.ctx->stream);
+  AllocScope scope(*p.ctx);
   {
     StageTimer tm(p, 5);
     evaluate_at_sources(p, out, output);
@@ -631,7 +656,7 @@ void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, 
   if (nt == 0) return;
   if (!targets || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
   set_device(*p.ctx);
-  AllocScope scope(p.ctx->stream);
+  AllocScope scope(*p.ctx);
   {
     StageTimer tm(p, 6);
     evaluate_targets(p, targets, nt, out, output);

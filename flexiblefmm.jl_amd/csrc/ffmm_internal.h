@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -82,54 +83,73 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 // streams read)
 void once_on_device(int device, int key, const std::function<void()> &upload);
 
+// XCD-aware block order.  The dispatcher deals workgroups round-robin to the 8 XCDs
+// (linear workgroup id mod 8), each with its own L2; a kernel whose neighbouring
+// blocks share data takes block (L mod 8) * (total / 8) + L / 8 instead of L, so each
+// XCD runs a contiguous range.  total must be a multiple of 8 (pad the grid with
+// blocks that exit).
+constexpr int NXCD = 8;
+__device__ __forceinline__ int64_t xcd_block(int64_t L, int64_t total) {
+  return (L % NXCD) * (total / NXCD) + L / NXCD;
+}
+inline int64_t pad_xcd(int64_t n) { return (n + NXCD - 1) / NXCD * NXCD; }
+
 // workgroups covering n work-items (at least one: every kernel checks its index against n)
 inline unsigned nblocks(int64_t n, int b) { return n > 0 ? (unsigned)((n + b - 1) / b) : 1u; }
 
-// Device allocations.  Inside an AllocScope they are stream ordered on the scope's
-// stream (hipMallocAsync / hipFreeAsync from the device's default pool, which keeps
-// freed memory for reuse instead of returning it: plans of the same size come and go
-// without synchronising the device); outside one they are plain hipMalloc / hipFree.
-extern thread_local hipStream_t g_alloc_stream;
+// Device allocations.  Inside an AllocScope they belong to the scope's context: a
+// freed block goes to the context's block cache, and the next allocation of a
+// similar size on that context takes it back with no HIP call (plans of the same
+// size come and go without hipMalloc / hipFree, which synchronise the device).  Every
+// entry point synchronises its stream before it returns, and a context's allocations
+// are used on its own stream, so a cached block is idle when it is reused.  Outside a
+// scope, allocations are plain hipMalloc / hipFree.
+struct Context;
+extern thread_local Context *g_alloc_ctx;
 struct AllocScope {
-  hipStream_t prev;
-  explicit AllocScope(hipStream_t s) : prev(g_alloc_stream) { g_alloc_stream = s; }
-  ~AllocScope() { g_alloc_stream = prev; }
+  Context *prev;
+  explicit AllocScope(Context &c) : prev(g_alloc_ctx) { g_alloc_ctx = &c; }
+  ~AllocScope() { g_alloc_ctx = prev; }
   AllocScope(const AllocScope &) = delete;
   AllocScope &operator=(const AllocScope &) = delete;
 };
-// returns the stream the memory is ordered on (null: synchronous)
-hipStream_t device_alloc(void **p, size_t bytes);
-void device_free(void *p, hipStream_t owner);
+// *blk: the block's size in bytes (>= bytes); returns its owner (null: hipMalloc)
+Context *device_alloc(void **p, size_t bytes, size_t *blk);
+void device_free(void *p, size_t blk, Context *owner);
 
 // A device array; resize() discards the contents, grow() keeps them.
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0, cap = 0;
-  hipStream_t owner = nullptr;   // the stream its allocation is ordered on (null: synchronous)
+  size_t blk = 0;              // bytes of the underlying block
+  Context *owner = nullptr;    // the context whose block cache it returns to (null: hipFree)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap), owner(o.owner) { o.p = nullptr; o.n = o.cap = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap), blk(o.blk), owner(o.owner) {
+    o.p = nullptr;
+    o.n = o.cap = o.blk = 0;
+  }
   DBuf &operator=(DBuf &&o) noexcept {
     if (this != &o) {
       release();
-      p = o.p; n = o.n; cap = o.cap; owner = o.owner;
-      o.p = nullptr; o.n = o.cap = 0;
+      p = o.p; n = o.n; cap = o.cap; blk = o.blk; owner = o.owner;
+      o.p = nullptr; o.n = o.cap = o.blk = 0;
     }
     return *this;
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) device_free(p, owner);
+    if (p) device_free(p, blk, owner);
     p = nullptr;
-    n = cap = 0;
+    n = cap = blk = 0;
   }
   void resize(size_t m) {
     if (m > cap) {
       release();
-      owner = device_alloc((void **)&p, (m ? m : 1) * sizeof(T));
-      cap = m;
+      owner = device_alloc((void **)&p, (m ? m : 1) * sizeof(T), &blk);
+      cap = blk / sizeof(T);
     }
     n = m;
   }
@@ -138,13 +158,15 @@ struct DBuf {
     size_t c = cap ? cap : 16;
     while (c < m) c *= 2;
     T *q = nullptr;
-    hipStream_t qo = device_alloc((void **)&q, c * sizeof(T));
+    size_t qb = 0;
+    Context *qo = device_alloc((void **)&q, c * sizeof(T), &qb);
     if (n) FFMM_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     FFMM_HIP(hipStreamSynchronize(s));
-    if (p) device_free(p, owner);
+    if (p) device_free(p, blk, owner);
     p = q;
     owner = qo;
-    cap = c;
+    blk = qb;
+    cap = qb / sizeof(T);
     n = m;
   }
   T *data() const { return p; }
@@ -158,11 +180,20 @@ struct DBuf {
   }
 };
 
+// the idle device blocks of a context, by size
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<size_t, void *> blocks;
+  size_t bytes = 0;
+  size_t limit = 0;    // cached bytes kept at most
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   rocblas_handle blas = nullptr;
+  std::unique_ptr<BlockCache> blocks = std::make_unique<BlockCache>();
   int64_t *pinned = nullptr;   // small pinned readback area (64 entries)
   DBuf<char> tmp;               // scratch of the primitives
   void sync() { FFMM_HIP(hipStreamSynchronize(stream)); }

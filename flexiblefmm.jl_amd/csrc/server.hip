@@ -185,7 +185,7 @@ static void run_request(Worker &w, Request &q, Plan *plan, bool fresh) {
   OpCache &o = *q.cache;
   hipStream_t s = ctx.stream;
   FFMM_HIP(hipSetDevice(ctx.device));
-  AllocScope scope(s);
+  AllocScope scope(ctx);
   if (fresh) {
     w.xyz.resize(3 * (size_t)r.n);
     FFMM_HIP(hipMemcpyAsync(w.xyz.p, r.points, sizeof(double) * 3 * r.n, hipMemcpyHostToDevice, s));

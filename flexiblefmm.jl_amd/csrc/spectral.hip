@@ -248,7 +248,8 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restri
                                                           const float2 *__restrict__ tables,
                                                           const int16_t *__restrict__ pairf,
                                                           const int32_t *__restrict__ gptr,
-                                                          const int32_t *__restrict__ cand, int64_t ngroups, int gpw) {
+                                                          const int32_t *__restrict__ cand, int64_t ngroups, int gpw,
+                                                          int gx_real) {
   constexpr int LPC = FB / 2;              // lanes per candidate
   constexpr int R = 64 / LPC;              // candidates per step
   constexpr int TS = FB + 2;               // padded table row, a multiple of two entries (16 bytes)
@@ -258,7 +259,11 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restri
   __shared__ __attribute__((aligned(16))) float2 Tt[(NFAR + 1) * TS];
   __shared__ __attribute__((aligned(16))) int16_t F[NCOLL * NCHILD * NCHILD];
   __shared__ int32_t Cl[NW][MAXC + 64];
-  const int f0 = blockIdx.y * FB;
+  // XCD-aware order: each XCD takes a contiguous run of group chunks at one frequency block
+  const int64_t Lb = xcd_block((int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x, (int64_t)gridDim.x * gridDim.y);
+  const int bx = (int)(Lb % gridDim.x), by = (int)(Lb / gridDim.x);
+  if (bx >= gx_real) return;   // a padding block
+  const int f0 = by * FB;
   const int t = threadIdx.x;
   for (int i = t; i < (NFAR + 1) * TS; i += HD_TB) {
     const int f = i / TS, ph = i % TS;
@@ -270,7 +275,7 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restri
   const int wave = t >> 6, lane = t & 63;
   const int ph = 2 * (lane % LPC), r = lane / LPC;
   int32_t *cl = Cl[wave];
-  const int64_t g0 = ((int64_t)blockIdx.x * NW + wave) * gpw;
+  const int64_t g0 = ((int64_t)bx * NW + wave) * gpw;
   for (int64_t g = g0; g < g0 + gpw && g < ngroups; g++) {
     const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;
     if (n == 0) continue;
@@ -444,9 +449,12 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   const int64_t gpb = (HD_TB / 64) * gpw;
   dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
   StageTimer tm(plan, 8, l);
-  if constexpr (std::is_same<T, float>::value)
-    k_spec_hadamard2<FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
-                                                ng, gpw);
+  if constexpr (std::is_same<T, float>::value) {
+    const int gx = (int)grid.x;
+    const dim3 gpad((unsigned)pad_xcd(gx), grid.y);
+    k_spec_hadamard2<FB><<<gpad, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                ng, gpw, gx);
+  }
   else
     k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
                                                  ng, gpw);
