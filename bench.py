@@ -194,10 +194,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 dense step")
     ap.add_argument("--nerr", type=int, default=10_000)
-    ap.add_argument("--serve-requests", type=int, default=60, help="served-stream requests (0: skip the leg)")
+    ap.add_argument("--serve-requests", type=int, default=80, help="served-stream requests (0: skip the leg)")
     ap.add_argument("--serve-nmin", type=int, default=10_000)
     ap.add_argument("--serve-nmax", type=int, default=200_000)
-    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="before the steps, copy a buffer of this size once on the device (a kernel of known "

@@ -236,6 +236,10 @@ __global__ void k_gather_targets(const double *__restrict__ t, const int32_t *__
   tz[q] = t[3 * o + 2];
 }
 
+__global__ void k_last_key(const uint32_t *__restrict__ key, int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) host[0] = (int64_t)key[0];
+}
+
 void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output) {
   if (nt <= 0) return;
   if (nt >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many targets for 32-bit indices");
@@ -253,10 +257,11 @@ void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out
                                           plan.node_leafslot.p, keys.p, idx.p);
   FFMM_LAUNCH_CHECK();
   sort_pairs_u32(ctx, keys.p, skeys.p, idx.p, plan.tgt_order.p, nt, 32);
-  uint32_t last = 0;
-  FFMM_HIP(hipMemcpyAsync(&last, skeys.p + nt - 1, sizeof last, hipMemcpyDeviceToHost, s));
+  // the largest key (outside the box: all ones), read from pinned memory
+  k_last_key<<<1, 64, 0, s>>>(skeys.p + nt - 1, ctx.pinned_dev);
+  FFMM_LAUNCH_CHECK();
   ctx.sync();
-  if (last == 0xFFFFFFFFu) fail(FFMM_ERR_ARGUMENT, "a target lies outside the tree's box");
+  if ((uint32_t)ctx.pinned[0] == 0xFFFFFFFFu) fail(FFMM_ERR_ARGUMENT, "a target lies outside the tree's box");
   plan.tgt_ptr.resize(nleaves + 1);
   k_bucket_ptr<<<nblocks(nleaves + 1, TB), TB, 0, s>>>(skeys.p, nt, nleaves, plan.tgt_ptr.p);
   plan.tgt_sorted.resize(3 * nt);

@@ -100,7 +100,8 @@ void context_init(Context &c, int device, void *stream) {
   FFMM_BLAS(rocblas_create_handle(&c.blas));
   FFMM_BLAS(rocblas_set_stream(c.blas, c.stream));
   FFMM_BLAS(rocblas_set_pointer_mode(c.blas, rocblas_pointer_mode_host));
-  FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+  FFMM_HIP(hipHostMalloc((void **)&c.pinned, 64 * sizeof(int64_t), hipHostMallocMapped | hipHostMallocCoherent));
+  FFMM_HIP(hipHostGetDevicePointer((void **)&c.pinned_dev, c.pinned, 0));
   // idle plan memory kept for reuse: up to half of the device's memory
   size_t fr = 0, tot = 0;
   FFMM_HIP(hipMemGetInfo(&fr, &tot));
@@ -320,7 +321,8 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
         int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
         far_index[u] = std::max(std::abs(x), std::max(std::abs(y), std::abs(z))) >= 2 ? nf++ : -1;
       }
-      std::vector<int16_t> pf(NCOLL * NCHILD * NCHILD);
+      // one more row for the padding candidate of the spectral Hadamard: all near (zero)
+      std::vector<int16_t> pf((NCOLL * NCHILD + 1) * NCHILD, (int16_t)-1);
       for (int k = 0; k < NCOLL; k++)
         for (int c = 0; c < NCHILD; c++)
           for (int a = 0; a < NCHILD; a++) {
@@ -564,14 +566,7 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
     p.hscale.upload(hs.data(), p.L, ctx.stream);
     p.khat.upload(kh.data(), p.L, ctx.stream);
   }
-  {
-    // node -> leaf slot
-    std::vector<int32_t> slot(p.nnodes, -1), leaves(p.leaf_off[p.L]);
-    p.leaf_nodes.download(leaves.data(), leaves.size(), ctx.stream);
-    ctx.sync();
-    for (size_t s = 0; s < leaves.size(); s++) slot[leaves[s]] = (int32_t)s;
-    p.node_leafslot.upload(slot.data(), slot.size(), ctx.stream);
-  }
+  node_leafslots(p);
   {
     StageTimer tm(p, 1);
     build_lists(p);

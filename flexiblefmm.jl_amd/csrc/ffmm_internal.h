@@ -161,7 +161,8 @@ struct DBuf {
     size_t qb = 0;
     Context *qo = device_alloc((void **)&q, c * sizeof(T), &qb);
     if (n) FFMM_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    FFMM_HIP(hipStreamSynchronize(s));
+    // a cached block is reused in stream order after the copy; a hipFree'd one waits
+    if (!owner) FFMM_HIP(hipStreamSynchronize(s));
     if (p) device_free(p, blk, owner);
     p = q;
     owner = qo;
@@ -194,7 +195,8 @@ struct Context {
   bool own_stream = false;
   rocblas_handle blas = nullptr;
   std::unique_ptr<BlockCache> blocks = std::make_unique<BlockCache>();
-  int64_t *pinned = nullptr;   // small pinned readback area (64 entries)
+  int64_t *pinned = nullptr;   // small pinned readback area (64 entries), coherent
+  int64_t *pinned_dev = nullptr;   // its device address: kernels store results there directly
   DBuf<char> tmp;               // scratch of the primitives
   void sync() { FFMM_HIP(hipStreamSynchronize(stream)); }
   // read back `n` int64 values from device memory (blocking)
@@ -259,7 +261,8 @@ struct OpCache {
   DBuf<float> tables32;
   DBuf<int32_t> embed;                       // Ns: lattice index of each small-surface point
   DBuf<int32_t> sub8;                        // Ns: its index in the p^3 corner of the lattice
-  DBuf<int16_t> pairf;                       // [(k * 8 + c) * 8 + a]: far index of (sibling a, child c of colleague k)
+  DBuf<int16_t> pairf;                       // [(k * 8 + c) * 8 + a]: far index of (sibling a, child c of colleague k);
+                                             // row 216: the padding candidate, all near
   std::vector<int32_t> embed_h;
   std::vector<int32_t> far_class_h;
 };
@@ -329,6 +332,8 @@ struct Plan {
 };
 
 // ---- primitives (prims.hip) ----
+// out[0] = v, on the device (no host copy)
+void set_i32(Context &ctx, int32_t *out, int32_t v);
 // stable LSD radix sort of (key, value) pairs on the bits [0, end_bit)
 void sort_pairs_u64(Context &ctx, const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout,
                     int64_t n, int end_bit);
@@ -342,6 +347,7 @@ int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_
 // ---- tree (tree.hip) ----
 void bounds(Context &ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite);
 void build_tree(Plan &plan, const double *xyz);
+void node_leafslots(Plan &plan);
 
 // ---- lists (lists.hip) ----
 void build_lists(Plan &plan);

@@ -33,6 +33,18 @@ void sort_pairs_u32(Context &ctx, const uint32_t *kin, uint32_t *kout, const int
   FFMM_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, (size_t)n, 0, end_bit, ctx.stream));
 }
 
+// lane 0 of one wave: out[n-1] + in[n-1], the total of an exclusive scan, into pinned host memory
+__global__ void k_scan_total(const int32_t *__restrict__ out, const int32_t *__restrict__ in, int64_t n,
+                             int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) host[0] = (int64_t)out[n - 1] + (int64_t)in[n - 1];
+}
+__global__ void k_copy_count(const int32_t *__restrict__ dcount, int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) host[0] = dcount[0];
+}
+__global__ void k_set_value(int32_t *__restrict__ out, int32_t v) {
+  if (threadIdx.x == 0) out[0] = v;
+}
+
 int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_t n) {
   if (n <= 0) return 0;
   size_t bytes = 0;
@@ -40,11 +52,11 @@ int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_
                                    ctx.stream));
   void *t = scratch(ctx, bytes);
   FFMM_HIP(rocprim::exclusive_scan(t, bytes, in, out, int32_t(0), (size_t)n, rocprim::plus<int32_t>(), ctx.stream));
-  int32_t *h = reinterpret_cast<int32_t *>(ctx.pinned);
-  FFMM_HIP(hipMemcpyAsync(h, out + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx.stream));
-  FFMM_HIP(hipMemcpyAsync(h + 1, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx.stream));
+  // the total, stored by a kernel straight into the pinned readback area
+  k_scan_total<<<1, 64, 0, ctx.stream>>>(out, in, n, ctx.pinned_dev);
+  FFMM_HIP(hipGetLastError());
   ctx.sync();
-  return (int64_t)h[0] + (int64_t)h[1];
+  return ctx.pinned[0];
 }
 
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n) {
@@ -57,10 +69,15 @@ int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_
   char *t = static_cast<char *>(scratch(ctx, off + 256));
   int32_t *dcount = reinterpret_cast<int32_t *>(t + off);
   FFMM_HIP(rocprim::select(t, bytes, it, flag, idx, dcount, (size_t)n, ctx.stream));
-  int32_t *h = reinterpret_cast<int32_t *>(ctx.pinned);
-  FFMM_HIP(hipMemcpyAsync(h, dcount, sizeof(int32_t), hipMemcpyDeviceToHost, ctx.stream));
+  k_copy_count<<<1, 64, 0, ctx.stream>>>(dcount, ctx.pinned_dev);
+  FFMM_HIP(hipGetLastError());
   ctx.sync();
-  return h[0];
+  return ctx.pinned[0];
+}
+
+void set_i32(Context &ctx, int32_t *out, int32_t v) {
+  k_set_value<<<1, 64, 0, ctx.stream>>>(out, v);
+  FFMM_HIP(hipGetLastError());
 }
 
 }  // namespace ffmm

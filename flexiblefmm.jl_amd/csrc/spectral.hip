@@ -108,6 +108,11 @@ __global__ void __launch_bounds__(SP_TB) k_spec_forward(const double *__restrict
 // pairf[(k * 8 + c) * 8 + a]: far index of the pair (sibling a, child c of the parent's
 // colleague k), -1 when near.
 // ---------------------------------------------------------------------------
+// Candidate lists are padded to a multiple of CAND_PAD entries with CAND_DUMMY
+// candidates (pairf row 216: every pair near, so they multiply the zero table row).
+constexpr int CAND_PAD = 4;
+constexpr int CAND_DUMMY = NCOLL * NCHILD;
+
 // The far-pair sources of every sibling group of a level, compacted: the sourced
 // children of the parent's 27 colleagues as (level-local source << 8 | 8 k + c), the
 // group's entries at cand[gptr[g] .. gptr[g + 1]) (the enumeration of the lists,
@@ -130,7 +135,7 @@ __global__ void k_group_cand_count(const int32_t *__restrict__ parent, const int
       for (int c = 0; c < NCHILD; c++) n += has_source[cf + c];
     }
   }
-  cnt[g] = n;
+  cnt[g] = (n + CAND_PAD - 1) / CAND_PAD * CAND_PAD;   // padded with zero-table candidates
 }
 __global__ void k_group_cand_fill(const int32_t *__restrict__ parent, const int32_t *__restrict__ child_first,
                                   const int32_t *__restrict__ col, const uint8_t *__restrict__ ghost,
@@ -150,6 +155,9 @@ __global__ void k_group_cand_fill(const int32_t *__restrict__ parent, const int3
     for (int c = 0; c < NCHILD; c++)
       if (has_source[cf + c]) cand[pos++] = (int32_t)((cf + c - lo) << 8) | (k * NCHILD + c);
   }
+  // the padding: the first candidate's (finite) spectrum against the all-near row
+  const int32_t first_pos = gptr[g];
+  for (; pos < gptr[g + 1]; pos++) cand[pos] = (cand[first_pos] & ~255) | CAND_DUMMY;
 }
 
 // One wave per sibling group and frequency block of FB.  The 316 tables of the block
@@ -175,7 +183,7 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T
   constexpr int NW = HD_TB / 64;
   constexpr int MAXC = NCOLL * NCHILD;     // 216 candidates at most
   __shared__ C Tt[(NFAR + 1) * TS];
-  __shared__ __attribute__((aligned(16))) int16_t F[NCOLL * NCHILD * NCHILD];
+  __shared__ __attribute__((aligned(16))) int16_t F[(NCOLL * NCHILD + 1) * NCHILD];
   __shared__ int32_t Cl[NW][MAXC + 64];
   const int f0 = blockIdx.y * FB;
   const int t = threadIdx.x;
@@ -183,7 +191,7 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T
     const int f = i / TS, ph = i % TS;
     Tt[i] = (f < NFAR && ph < FB) ? tables[(int64_t)f * SP_NF + f0 + ph] : C{(T)0, (T)0};
   }
-  for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HD_TB) F[i] = pairf[i] < 0 ? (int16_t)NFAR : pairf[i];
+  for (int i = t; i < (NCOLL * NCHILD + 1) * NCHILD; i += HD_TB) F[i] = pairf[i] < 0 ? (int16_t)NFAR : pairf[i];
   __syncthreads();
   const int wave = t >> 6, lane = t & 63;
   const int phi = lane % FB, r = lane / FB;
@@ -238,72 +246,76 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T
   }
 }
 
-// The FP32 form with two adjacent frequencies per lane: one 16-byte table read and one
-// 16-byte spectrum load feed twice the multiply-adds of the kernel above, and the
-// far-index table holds row offsets (far index x row pitch) so that no address is
-// multiplied in the inner loop.  FB / 2 lanes share a candidate, R = 128 / FB
-// candidates per step; lane (ph, r) writes sibling r's two frequencies.
+// The FP32 form.  Each candidate is taken by 16 lanes with two adjacent frequencies
+// each, so a 16-lane group reads one contiguous 256-byte segment of one table row: it
+// covers the 64 LDS banks exactly once whatever the row, and the table reads are free of
+// bank conflicts (with narrower rows two candidates share a 16-lane group and their
+// rows collide on banks).  FB = 32 frequencies per block, R = 4 candidates per step;
+// one 16-byte table read and one 16-byte spectrum load feed 16 FMAs per lane, the
+// far-index table holds row offsets (no address multiplies in the inner loop), and a
+// 1024-thread workgroup (16 waves) shares one 81 KB copy of the block's tables.
+// Lane (ph, r) writes siblings r and r + 4, two frequencies each.
+constexpr int HD2_TB = 1024;
 template <int FB>
-__global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restrict__ S, float2 *__restrict__ U,
-                                                          const float2 *__restrict__ tables,
-                                                          const int16_t *__restrict__ pairf,
-                                                          const int32_t *__restrict__ gptr,
-                                                          const int32_t *__restrict__ cand, int64_t ngroups, int gpw,
-                                                          int gx_real) {
+__global__ void __launch_bounds__(HD2_TB) k_spec_hadamard2(const float2 *__restrict__ S, float2 *__restrict__ U,
+                                                           const float2 *__restrict__ tables,
+                                                           const int16_t *__restrict__ pairf,
+                                                           const int32_t *__restrict__ gptr,
+                                                           const int32_t *__restrict__ cand, int64_t ngroups, int gpw,
+                                                           int gx_real) {
   constexpr int LPC = FB / 2;              // lanes per candidate
   constexpr int R = 64 / LPC;              // candidates per step
-  constexpr int TS = FB + 2;               // padded table row, a multiple of two entries (16 bytes)
-  constexpr int NW = HD_TB / 64;
-  constexpr int MAXC = NCOLL * NCHILD;
-  static_assert(R == NCHILD, "one sibling per candidate slot at the write-out");
+  constexpr int TS = FB;                   // table row: 16 lanes x 16 bytes, no padding needed
+  constexpr int NW = HD2_TB / 64;
+  constexpr int MAXC = NCOLL * NCHILD + CAND_PAD;
+  static_assert(LPC == 16, "one 16-lane group per candidate");
+  static_assert(R == CAND_PAD, "candidate lists are padded to whole steps");
+  static_assert(NCHILD % R == 0, "siblings per lane at the write-out");
   __shared__ __attribute__((aligned(16))) float2 Tt[(NFAR + 1) * TS];
-  __shared__ __attribute__((aligned(16))) int16_t F[NCOLL * NCHILD * NCHILD];
-  __shared__ int32_t Cl[NW][MAXC + 64];
+  __shared__ int32_t F[(NCOLL * NCHILD + 1) * NCHILD];   // byte offsets of the table rows
+  __shared__ int32_t Cl[NW][MAXC + R];
   // XCD-aware order: each XCD takes a contiguous run of group chunks at one frequency block
   const int64_t Lb = xcd_block((int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x, (int64_t)gridDim.x * gridDim.y);
   const int bx = (int)(Lb % gridDim.x), by = (int)(Lb / gridDim.x);
   if (bx >= gx_real) return;   // a padding block
   const int f0 = by * FB;
   const int t = threadIdx.x;
-  for (int i = t; i < (NFAR + 1) * TS; i += HD_TB) {
+  for (int i = t; i < (NFAR + 1) * TS; i += HD2_TB) {
     const int f = i / TS, ph = i % TS;
-    Tt[i] = (f < NFAR && ph < FB) ? tables[(int64_t)f * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
+    Tt[i] = f < NFAR ? tables[(int64_t)f * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
   }
-  for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HD_TB)
-    F[i] = (int16_t)((pairf[i] < 0 ? NFAR : pairf[i]) * TS);
+  for (int i = t; i < (NCOLL * NCHILD + 1) * NCHILD; i += HD2_TB)
+    F[i] = (pairf[i] < 0 ? NFAR : pairf[i]) * TS * (int)sizeof(float2);
   __syncthreads();
   const int wave = t >> 6, lane = t & 63;
   const int ph = 2 * (lane % LPC), r = lane / LPC;
+  const char *tbase = reinterpret_cast<const char *>(Tt) + ph * sizeof(float2);
   int32_t *cl = Cl[wave];
   const int64_t g0 = ((int64_t)bx * NW + wave) * gpw;
   for (int64_t g = g0; g < g0 + gpw && g < ngroups; g++) {
-    const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;
+    const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;   // a multiple of R
     if (n == 0) continue;
-    for (int i = lane; i < n; i += 64) cl[i] = cand[cbeg + i];
+    for (int i = lane; i < n + R; i += 64) cl[i] = cand[cbeg + (i < n ? i : 0)];   // + R copies for the prefetch
     __builtin_amdgcn_wave_barrier();
     float4 acc[NCHILD];
 #pragma unroll
     for (int a = 0; a < NCHILD; a++) acc[a] = make_float4(0.f, 0.f, 0.f, 0.f);
-    int32_t e = cl[r < n ? r : 0];
-    float4 sv = *reinterpret_cast<const float4 *>(&S[(int64_t)(e >> 8) * SP_NF + f0 + ph]);
-    bool live = r < n;
+    int32_t e = cl[r];
+    float4 v = *reinterpret_cast<const float4 *>(&S[(int64_t)(e >> 8) * SP_NF + f0 + ph]);
     for (int j = 0; j < n; j += R) {
-      const int jn = j + R + r;
-      const int32_t en = cl[jn < n ? jn : 0];
-      const float4 svn = *reinterpret_cast<const float4 *>(&S[(int64_t)(en >> 8) * SP_NF + f0 + ph]);
-      const int16_t *Fk = &F[(e & 255) * NCHILD];
-      const float4 v = live ? sv : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int32_t en = cl[j + R + r];
+      const float4 vn = *reinterpret_cast<const float4 *>(&S[(int64_t)(en >> 8) * SP_NF + f0 + ph]);
+      const int32_t *Fk = &F[(e & 255) * NCHILD];
 #pragma unroll
       for (int a = 0; a < NCHILD; a++) {
-        const float4 tt = *reinterpret_cast<const float4 *>(&Tt[Fk[a] + ph]);
+        const float4 tt = *reinterpret_cast<const float4 *>(tbase + Fk[a]);
         acc[a].x = fmaf(tt.x, v.x, fmaf(-tt.y, v.y, acc[a].x));
         acc[a].y = fmaf(tt.x, v.y, fmaf(tt.y, v.x, acc[a].y));
         acc[a].z = fmaf(tt.z, v.z, fmaf(-tt.w, v.w, acc[a].z));
         acc[a].w = fmaf(tt.z, v.w, fmaf(tt.w, v.z, acc[a].w));
       }
       e = en;
-      sv = svn;
-      live = jn < n;
+      v = vn;
     }
 #pragma unroll
     for (int a = 0; a < NCHILD; a++) {
@@ -315,11 +327,15 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard2(const float2 *__restri
         acc[a].w += __shfl_xor(acc[a].w, off, 64);
       }
     }
-    float4 o = acc[0];
 #pragma unroll
-    for (int b = 1; b < NCHILD; b++)
-      if (b == r) o = acc[b];
-    *reinterpret_cast<float4 *>(&U[(NCHILD * g + r) * SP_NF + f0 + ph]) = o;
+    for (int m = 0; m < NCHILD / R; m++) {
+      const int a = r + m * R;
+      float4 o = acc[0];
+#pragma unroll
+      for (int b2 = 1; b2 < NCHILD; b2++)
+        if (b2 == a) o = acc[b2];
+      *reinterpret_cast<float4 *>(&U[(NCHILD * g + a) * SP_NF + f0 + ph]) = o;
+    }
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -422,8 +438,7 @@ void spectral_prepare(Plan &plan) {
                                                          plan.has_source.p, lo, ng, plan.nnodes, cnt.p);
     FFMM_LAUNCH_CHECK();
     const int64_t total = exclusive_scan_i32(ctx, cnt.p, plan.spec_gptr[l].p, ng);
-    const int32_t tot32 = (int32_t)total;
-    FFMM_HIP(hipMemcpyAsync(plan.spec_gptr[l].p + ng, &tot32, sizeof tot32, hipMemcpyHostToDevice, s));
+    set_i32(ctx, plan.spec_gptr[l].p + ng, (int32_t)total);
     plan.spec_cand[l].resize(total);
     if (total > 0) {
       k_group_cand_fill<<<nblocks(ng, 256), 256, 0, s>>>(plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p,
@@ -431,7 +446,6 @@ void spectral_prepare(Plan &plan) {
                                                           plan.spec_cand[l].p);
       FFMM_LAUNCH_CHECK();
     }
-    ctx.sync();   // tot32 lives on this stack frame
   }
 }
 
@@ -450,10 +464,12 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
   StageTimer tm(plan, 8, l);
   if constexpr (std::is_same<T, float>::value) {
-    const int gx = (int)grid.x;
-    const dim3 gpad((unsigned)pad_xcd(gx), grid.y);
-    k_spec_hadamard2<FB><<<gpad, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
-                                                ng, gpw, gx);
+    constexpr int FB2 = 32, gpw2 = 4;
+    const int64_t gpb2 = (HD2_TB / 64) * gpw2;
+    const int gx = (int)((ng + gpb2 - 1) / gpb2);
+    const dim3 gpad((unsigned)pad_xcd(gx), SP_NF / FB2);
+    k_spec_hadamard2<FB2><<<gpad, HD2_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                  ng, gpw2, gx);
   }
   else
     k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,

@@ -122,6 +122,29 @@ __global__ void k_gather_points(const double *__restrict__ xyz, const int32_t *_
 // ---------------------------------------------------------------------------
 // the per-level builder
 // ---------------------------------------------------------------------------
+// the root: cell 0, no parent or children, points [0, N), a leaf
+__global__ void k_init_root(int32_t *cx, int32_t *cy, int32_t *cz, int32_t *parent, int32_t *child_first, int32_t *lo,
+                            int32_t *hi, int32_t *depth, uint8_t *leaf, uint8_t *ghost, uint8_t *xlist, int32_t last) {
+  if (threadIdx.x != 0) return;
+  cx[0] = cy[0] = cz[0] = 0;
+  parent[0] = child_first[0] = -1;
+  lo[0] = 0;
+  hi[0] = last;
+  depth[0] = 0;
+  leaf[0] = 1;
+  ghost[0] = 0;
+  xlist[0] = 0;
+}
+
+__global__ void k_fill_i32(int32_t *__restrict__ a, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_leaf_slots(const int32_t *__restrict__ leaf_nodes, int64_t nleaves, int32_t *__restrict__ slot) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nleaves) slot[leaf_nodes[s]] = (int32_t)s;
+}
+
 struct DevLevel {
   int64_t n = 0;
   DBuf<int32_t> cx, cy, cz, parent, child_first, lo, hi, depth, col;   // col: [k * n + i]
@@ -389,20 +412,9 @@ void build_tree(Plan &plan, const double *xyz) {
   {
     DevLevel &root = levels[0];
     root.grow(1, s);
-    int32_t zero = 0, minus = -1, hiN = (int32_t)(N - 1);
-    uint8_t one = 1, z8 = 0;
-    FFMM_HIP(hipMemcpyAsync(root.cx.p, &zero, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.cy.p, &zero, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.cz.p, &zero, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.parent.p, &minus, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.child_first.p, &minus, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.lo.p, &zero, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.hi.p, &hiN, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.depth.p, &zero, 4, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.leaf.p, &one, 1, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.ghost.p, &z8, 1, hipMemcpyHostToDevice, s));
-    FFMM_HIP(hipMemcpyAsync(root.xlist.p, &z8, 1, hipMemcpyHostToDevice, s));
-    ctx.sync();   // the host values above live on this stack frame
+    k_init_root<<<1, 64, 0, s>>>(root.cx.p, root.cy.p, root.cz.p, root.parent.p, root.child_first.p, root.lo.p,
+                                 root.hi.p, root.depth.p, root.leaf.p, root.ghost.p, root.xlist.p, (int32_t)(N - 1));
+    FFMM_LAUNCH_CHECK();
   }
   DBuf<uint8_t> flag;
   DBuf<int32_t> which;
@@ -509,6 +521,16 @@ void build_tree(Plan &plan, const double *xyz) {
     FFMM_LAUNCH_CHECK();
   }
   ctx.sync();
+}
+
+// node -> position in the leaf list (-1: not a real leaf)
+void node_leafslots(Plan &plan) {
+  hipStream_t s = plan.ctx->stream;
+  const int64_t nleaves = plan.leaf_off[plan.L];
+  plan.node_leafslot.resize(plan.nnodes);
+  k_fill_i32<<<nblocks(plan.nnodes, TB), TB, 0, s>>>(plan.node_leafslot.p, plan.nnodes, -1);
+  k_leaf_slots<<<nblocks(nleaves, TB), TB, 0, s>>>(plan.leaf_nodes.p, nleaves, plan.node_leafslot.p);
+  FFMM_LAUNCH_CHECK();
 }
 
 }  // namespace ffmm
