@@ -331,7 +331,7 @@ def main():
         flops = float(pairs) * nfreq * 8
         t_k = stages["hadamard"] * 1e-3
         peak = PEAK_FP64_TFLOPS if args.precision == "f64" else PEAK_FP32_TFLOPS
-        kname = ("k_spec_hadamard2<16>" if args.precision == "mixed" else "k_spec_hadamard<double, 8>") + \
+        kname = ("k_spec_hadamard2<32>" if args.precision == "mixed" else "k_spec_hadamard<double, 8>") + \
             " (spectral M2L Hadamard products)"
     else:
         flops = float(pairs) * 2 * m.nsurface ** 2

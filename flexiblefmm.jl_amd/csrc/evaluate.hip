@@ -137,6 +137,138 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
   }
 }
 
+// The mixed form: the same traversal with FP32 arithmetic.  L2T works in coordinates
+// relative to the leaf centre (surface points and the target's offset rounded once to
+// FP32; they are at least ~w/2 apart).  The near field forms each pair difference in FP64
+// before rounding it, so close pairs keep their relative accuracy.  1/r is v_rsq_f32.
+// The potential is summed with Kahan compensation in FP32, the gradient with plain FP32
+// FMAs; both partial sums are folded into FP64 accumulators after every LDS tile.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float y) {   // y already has -c folded in
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+template <int OUT>
+__device__ __forceinline__ void accumulate32(float d0, float d1, float d2, float w, float &s, float &c, float g[3]) {
+  const float ir = __builtin_amdgcn_rsqf(fmaf(d2, d2, fmaf(d1, d1, d0 * d0)));
+  if (OUT != FFMM_GRADIENT) kahan_add(s, c, fmaf(ir, w, -c));
+  if (OUT != FFMM_POTENTIAL) {
+    const float a = ir * ir * ir * w;
+    g[0] = fmaf(a, d0, g[0]);
+    g[1] = fmaf(a, d1, g[1]);
+    g[2] = fmaf(a, d2, g[2]);
+  }
+}
+
+template <int OUT>
+__device__ __forceinline__ void fold32(float &s, float &c, float g[3], double acc[4]) {
+  acc[0] += (double)s - (double)c;
+  acc[1] += (double)g[0];
+  acc[2] += (double)g[1];
+  acc[3] += (double)g[2];
+  s = c = 0.f;
+  g[0] = g[1] = g[2] = 0.f;
+}
+
+template <int OUT>
+__global__ void __launch_bounds__(EV_TB) k_eval32(EvalArgs a) {
+  __shared__ double4 tile[EV_CK];
+  float4 *tile32 = reinterpret_cast<float4 *>(tile);
+  const int s = blockIdx.x;
+  const int i = a.leaf_nodes[s];
+  const bool self = a.tptr == nullptr;
+  const int32_t t0 = self ? a.plo[i] : a.tptr[s];
+  const int32_t t1 = self ? a.phi[i] + 1 : a.tptr[s + 1];
+  if (t0 >= t1) return;
+  const double c0 = a.centers[i], c1 = a.centers[a.nn + i], c2 = a.centers[2 * a.nn + i];
+  const double sc = a.lam[a.level[i]];
+  const double *__restrict__ L = a.Lc + (int64_t)s * a.Nl;
+  const double *__restrict__ X = a.surf;
+  const double *__restrict__ Y = a.surf + a.Nl;
+  const double *__restrict__ Z = a.surf + 2 * a.Nl;
+  int32_t nbr[NCOLL];
+#pragma unroll
+  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  for (int32_t tb = t0; tb < t1; tb += EV_TB) {
+    const int cnt = min(EV_TB, t1 - tb);
+    int ntp = 1;
+    while (ntp < cnt) ntp <<= 1;
+    const int R = EV_TB / ntp;
+    const int tl = (int)threadIdx.x % ntp, rep = (int)threadIdx.x / ntp;
+    const int32_t j = tb + tl;
+    const bool valid = tl < cnt;
+    const int32_t jj = valid ? j : t0;
+    const double x = self ? a.sx[jj] : a.tx[jj];
+    const double y = self ? a.sy[jj] : a.ty[jj];
+    const double z = self ? a.sz[jj] : a.tz[jj];
+    const float xr = (float)(x - c0), yr = (float)(y - c1), zr = (float)(z - c2);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    float ks = 0.f, kc = 0.f, g[3] = {0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < a.Nl; k0 += EV_CK) {
+      const int nk = min(EV_CK, a.Nl - k0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < nk; k += EV_TB)
+        tile32[k] = make_float4((float)(X[k0 + k] * sc), (float)(Y[k0 + k] * sc), (float)(Z[k0 + k] * sc),
+                                (float)L[k0 + k]);
+      __syncthreads();
+#pragma unroll 4
+      for (int k = rep; k < nk; k += R) {
+        const float4 v = tile32[k];
+        accumulate32<OUT>(xr - v.x, yr - v.y, zr - v.z, v.w, ks, kc, g);
+      }
+      fold32<OUT>(ks, kc, g, acc);
+    }
+    const int32_t skip = self ? j : -1;
+    for (int m = 0; m < NCOLL; m++) {
+      const int32_t nb = nbr[m];
+      if (nb < 0) continue;
+      const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
+      for (int32_t q0 = p0; q0 <= p1; q0 += EV_CK) {
+        const int nq = min(EV_CK, p1 - q0 + 1);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nq; k += EV_TB)
+          tile[k] = make_double4(a.sx[q0 + k], a.sy[q0 + k], a.sz[q0 + k], a.tau[q0 + k]);
+        __syncthreads();
+#pragma unroll 4
+        for (int k = rep; k < nq; k += R) {
+          const double4 v = tile[k];
+          // the target's own source: an infinite distance, so 1/r is exactly zero
+          const float d0 = (float)(x - v.x), d1 = (float)(y - v.y), d2 = (float)(z - v.z);
+          const float ir = __builtin_amdgcn_rsqf(q0 + k != skip ? fmaf(d2, d2, fmaf(d1, d1, d0 * d0)) : INFINITY);
+          const float w = (float)v.w;
+          if (OUT != FFMM_GRADIENT) kahan_add(ks, kc, fmaf(ir, w, -kc));
+          if (OUT != FFMM_POTENTIAL) {
+            const float aa = ir * ir * ir * w;
+            g[0] = fmaf(aa, d0, g[0]);
+            g[1] = fmaf(aa, d1, g[1]);
+            g[2] = fmaf(aa, d2, g[2]);
+          }
+        }
+        fold32<OUT>(ks, kc, g, acc);
+      }
+    }
+    for (int off = ntp; off < EV_TB; off <<= 1)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[c] += __shfl_xor(acc[c], off, EV_TB);
+    if (valid && rep == 0) {
+      const int64_t o = a.outidx[j];
+      if (OUT == FFMM_POTENTIAL) {
+        a.out[o] = C4PI * acc[0];
+      } else if (OUT == FFMM_GRADIENT) {
+        a.out[3 * o] = -C4PI * acc[1];
+        a.out[3 * o + 1] = -C4PI * acc[2];
+        a.out[3 * o + 2] = -C4PI * acc[3];
+      } else {
+        a.out[4 * o] = C4PI * acc[0];
+        a.out[4 * o + 1] = -C4PI * acc[1];
+        a.out[4 * o + 2] = -C4PI * acc[2];
+        a.out[4 * o + 3] = -C4PI * acc[3];
+      }
+    }
+  }
+}
+
 static void launch_eval(Plan &plan, EvalArgs &a, int output) {
   const int64_t nleaves = plan.leaf_off[plan.L];
   if (nleaves == 0) return;
@@ -146,7 +278,11 @@ static void launch_eval(Plan &plan, EvalArgs &a, int output) {
   }
   hipStream_t s = plan.ctx->stream;
   const unsigned g = (unsigned)nleaves;
-  if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<g, EV_TB, 0, s>>>(a);
+  if (plan.precision == FFMM_PREC_MIXED) {
+    if (output == FFMM_POTENTIAL) k_eval32<FFMM_POTENTIAL><<<g, EV_TB, 0, s>>>(a);
+    else if (output == FFMM_GRADIENT) k_eval32<FFMM_GRADIENT><<<g, EV_TB, 0, s>>>(a);
+    else k_eval32<2><<<g, EV_TB, 0, s>>>(a);
+  } else if (output == FFMM_POTENTIAL) k_eval<FFMM_POTENTIAL><<<g, EV_TB, 0, s>>>(a);
   else if (output == FFMM_GRADIENT) k_eval<FFMM_GRADIENT><<<g, EV_TB, 0, s>>>(a);
   else k_eval<2><<<g, EV_TB, 0, s>>>(a);
   FFMM_LAUNCH_CHECK();

@@ -286,6 +286,8 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
       for (int i = 0; i < Nl; i++) vd[(size_t)j * Nl + i] = V_dn[(size_t)j * Nl + i] * invS_dn[j];
     o.Vs_dn.upload(vd.data(), vd.size(), s);
     o.M2M.upload(M2M, (size_t)Ns * NCHILD * Ns, s);
+    std::vector<float> m32(M2M, M2M + (size_t)Ns * NCHILD * Ns);
+    o.M2M32.upload(m32.data(), m32.size(), s);
     o.L2L.upload(L2L, (size_t)NCHILD * Ns * Ns, s);
     o.class_ops.upload(class_ops, (size_t)nclass * Ns * Ns, s);
     std::vector<float> c32(class_ops, class_ops + (size_t)nclass * Ns * Ns);

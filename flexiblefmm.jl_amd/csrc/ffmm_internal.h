@@ -254,7 +254,7 @@ struct OpCache {
   DBuf<float> large32;
   DBuf<double> Uh_up, Vs_up, Uh_dn, Vs_dn;   // Vs = V diag(invS)
   DBuf<double> M2M, L2L, class_ops;
-  DBuf<float> class_ops32;
+  DBuf<float> M2M32, class_ops32;   // FP32 copies for the mixed form
   DBuf<int32_t> far_class, idx, inv;         // idx/inv: [f * Ns + i]
   DBuf<int8_t> sgn;
   DBuf<double> tables;                       // 316 x nfreq complex (interleaved)
@@ -314,6 +314,7 @@ struct Plan {
   // work buffers
   DBuf<double> pM, tup, Mup, tM, G, tdn, Gc, Yc;
   DBuf<float> Gc32, Yc32, pM32;
+  DBuf<float> B32, tM32;   // the mixed M2M: children block and parents in FP32
   // spectral M2L workspace
   DBuf<double> spec_S, spec_U;
   DBuf<float> spec_S32, spec_U32;

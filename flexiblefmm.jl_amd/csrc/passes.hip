@@ -81,6 +81,65 @@ __global__ void __launch_bounds__(S2M_TB) k_s2m(const int32_t *__restrict__ node
   }
 }
 
+// The mixed form of S2M: sources relative to the node centre (differences formed in
+// FP64, rounded once), surface points scaled and rounded once, v_rsq_f32, FP32 sums over
+// each LDS tile folded into FP64 check potentials.  The check surface lies at least
+// ~0.9 w from any source of the node, so no pair is close.
+__global__ void __launch_bounds__(S2M_TB) k_s2m32(const int32_t *__restrict__ nodes, const double *__restrict__ centers,
+                                                  int64_t nn, const int32_t *__restrict__ level,
+                                                  const double *__restrict__ lam, const double *__restrict__ surf, int Nl,
+                                                  const int32_t *__restrict__ plo, const int32_t *__restrict__ phi,
+                                                  const double *__restrict__ sx, const double *__restrict__ sy,
+                                                  const double *__restrict__ sz, const double *__restrict__ tau,
+                                                  double *__restrict__ pM) {
+  __shared__ float4 src[S2M_TB];
+  const int64_t q = blockIdx.x;
+  const int node = nodes[q];
+  const double c0 = centers[node], c1 = centers[nn + node], c2 = centers[2 * nn + node];
+  const double s = lam[level[node]];
+  const int t = threadIdx.x;
+  double acc[S2M_KP];
+  float y0[S2M_KP], y1[S2M_KP], y2[S2M_KP];
+#pragma unroll
+  for (int m = 0; m < S2M_KP; m++) {
+    const int k = t + S2M_TB * m;
+    acc[m] = 0.0;
+    if (k < Nl) {
+      y0[m] = (float)(surf[k] * s);
+      y1[m] = (float)(surf[Nl + k] * s);
+      y2[m] = (float)(surf[2 * Nl + k] * s);
+    } else {
+      y0[m] = y1[m] = y2[m] = 1.f;
+    }
+  }
+  const int32_t a = plo[node], b = phi[node];
+  for (int32_t j0 = a; j0 <= b; j0 += S2M_TB) {
+    const int32_t j = j0 + t;
+    if (j <= b) src[t] = make_float4((float)(sx[j] - c0), (float)(sy[j] - c1), (float)(sz[j] - c2), (float)tau[j]);
+    __syncthreads();
+    const int cnt = min(S2M_TB, b - j0 + 1);
+    float part[S2M_KP];
+#pragma unroll
+    for (int m = 0; m < S2M_KP; m++) part[m] = 0.f;
+    for (int jj = 0; jj < cnt; jj++) {
+      const float4 v = src[jj];
+#pragma unroll
+      for (int m = 0; m < S2M_KP; m++) {
+        const float d0 = y0[m] - v.x, d1 = y1[m] - v.y, d2 = y2[m] - v.z;
+        part[m] = fmaf(__builtin_amdgcn_rsqf(fmaf(d2, d2, fmaf(d1, d1, d0 * d0))), v.w, part[m]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < S2M_KP; m++) acc[m] += (double)part[m];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < S2M_KP; m++) {
+    const int k = t + S2M_TB * m;
+    if (k < Nl) pM[q * Nl + k] = C4PI * acc[m];
+  }
+}
+
 // dst[i, cols[q]] = alpha * src[i, q]
 __global__ void k_scatter_cols(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ cols,
                                int rows, int64_t ncols, double alpha) {
@@ -100,13 +159,19 @@ __global__ void k_gather_cols(double *__restrict__ dst, const double *__restrict
   dst[g] = src[(int64_t)cols[q] * rows + i];
 }
 // the parents of the sibling groups starting at node `first`: dst[i, parent[first + 8g]] = alpha src[i, g]
-__global__ void k_scatter_parents(double *__restrict__ dst, const double *__restrict__ src,
+template <class T>
+__global__ void k_scatter_parents(double *__restrict__ dst, const T *__restrict__ src,
                                   const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng, double alpha) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rows * ng) return;
   int64_t q = g / rows;
   int i = (int)(g % rows);
-  dst[(int64_t)parent[first + NCHILD * q] * rows + i] = alpha * src[g];
+  dst[(int64_t)parent[first + NCHILD * q] * rows + i] = alpha * (double)src[g];
+}
+// dst = (float) src
+__global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
 }
 __global__ void k_gather_parents(double *__restrict__ dst, const double *__restrict__ src,
                                  const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng) {
@@ -223,6 +288,10 @@ void allocate_pass_buffers(Plan &plan) {
     plan.Mup.resize((size_t)Ns * nq);
   }
   plan.tM.resize((size_t)Ns * ng);
+  if (o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED) {
+    plan.B32.resize((size_t)NCHILD * Ns * ng);
+    plan.tM32.resize((size_t)Ns * ng);
+  }
   plan.G.resize((size_t)Ns * std::max(nq, ng));
   int64_t pmax = 1;
   for (auto &li : plan.lists) pmax = std::max(pmax, std::min<int64_t>(PIECE, li.np));
@@ -265,9 +334,14 @@ void upward_pass(Plan &plan) {
       // the multipoles directly (partial_to_multipole! is a copy, src/operators/harmonics.jl:441)
       harmonic_s2m_level(plan, l);
     } else if (nq > 0) {
-      k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p, o.large.p,
-                                            Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p, plan.tau.p,
-                                            plan.pM.p);
+      if (plan.precision == FFMM_PREC_MIXED)
+        k_s2m32<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+                                                o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
+                                                plan.tau.p, plan.pM.p);
+      else
+        k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+                                              o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
+                                              plan.tau.p, plan.pM.p);
       FFMM_LAUNCH_CHECK();
       // UC2E: M = pinv_scale V diag(invS) (Uh pM)  (src/device_passes.jl:496-502)
       gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
@@ -279,9 +353,20 @@ void upward_pass(Plan &plan) {
       const int64_t ng = plan.nl[l + 1] / NCHILD;
       const double *B = plan.Mp.p + plan.lo[l + 1] * Ns;
       const double *M2M = o.method == METHOD_HARMONIC ? plan.hl[l]->M2M.p : o.M2M.p;
-      gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, M2M, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
-      k_scatter_parents<<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM.p, plan.parent.p,
-                                                                     plan.lo[l + 1], Ns, ng, 1.0);
+      if (o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED) {
+        // the mixed form: M2M as an FP32 GEMM (its rounding is not amplified: the
+        // multipoles feed M2L, never a pseudo-inverse)
+        const int64_t nB = (int64_t)NCHILD * Ns * ng;
+        k_to_f32<<<nblocks(nB, TB), TB, 0, s>>>(plan.B32.p, B, nB);
+        FFMM_LAUNCH_CHECK();
+        gemm32(ctx, Ns, (int)ng, NCHILD * Ns, o.M2M32.p, Ns, plan.B32.p, NCHILD * Ns, plan.tM32.p, Ns);
+        k_scatter_parents<float><<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM32.p, plan.parent.p,
+                                                                              plan.lo[l + 1], Ns, ng, 1.0);
+      } else {
+        gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, M2M, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
+        k_scatter_parents<double><<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM.p, plan.parent.p,
+                                                                               plan.lo[l + 1], Ns, ng, 1.0);
+      }
       FFMM_LAUNCH_CHECK();
     }
   }

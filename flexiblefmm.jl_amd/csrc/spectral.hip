@@ -462,19 +462,21 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   const int gpw = 4;
   const int64_t gpb = (HD_TB / 64) * gpw;
   dim3 grid((unsigned)((ng + gpb - 1) / gpb), SP_NF / FB);
-  StageTimer tm(plan, 8, l);
-  if constexpr (std::is_same<T, float>::value) {
-    constexpr int FB2 = 32, gpw2 = 4;
-    const int64_t gpb2 = (HD2_TB / 64) * gpw2;
-    const int gx = (int)((ng + gpb2 - 1) / gpb2);
-    const dim3 gpad((unsigned)pad_xcd(gx), SP_NF / FB2);
-    k_spec_hadamard2<FB2><<<gpad, HD2_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
-                                                  ng, gpw2, gx);
+  {
+    StageTimer tm(plan, 8, l);   // the Hadamard launch alone
+    if constexpr (std::is_same<T, float>::value) {
+      constexpr int FB2 = 32, gpw2 = 4;
+      const int64_t gpb2 = (HD2_TB / 64) * gpw2;
+      const int gx = (int)((ng + gpb2 - 1) / gpb2);
+      const dim3 gpad((unsigned)pad_xcd(gx), SP_NF / FB2);
+      k_spec_hadamard2<FB2><<<gpad, HD2_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                    ng, gpw2, gx);
+    }
+    else
+      k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
+                                                   ng, gpw);
+    FFMM_LAUNCH_CHECK();
   }
-  else
-    k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
-                                                 ng, gpw);
-  FFMM_LAUNCH_CHECK();
   const double alpha = (1.0 / plan.lam[l]) / (double)(SP_N * SP_N * SP_N);
   k_spec_inverse<T><<<(unsigned)nl, SP_TB, 0, s>>>(U, plan.lists[l].ptr.p, o.sub8.p, o.Ns, alpha,
                                                   plan.Pl.p + lo * o.Ns);

@@ -75,7 +75,7 @@ def test_matvec_spectral_vs_oracle_and_direct(setup, precision, tol_oracle, tol_
 
 def test_matvec_dense_fp32_classes_vs_oracle_and_direct(setup):
     """The mixed form of the dense class M2L: FP32 gather, FP32 GEMM (sgemm on FP32
-    MFMA), scatter into the FP64 partial locals; pseudo-inverses, M2M and L2L FP64."""
+    MFMA), scatter into the FP64 partial locals; pseudo-inverses and L2L FP64."""
     s = setup
     ffmm = s["ffmm"]
     print("[fmm] dense mixed", flush=True)
@@ -139,6 +139,31 @@ def test_targets_vs_oracle_and_direct(setup, out):
         vg = v if out == "gradient" else v[:, 1:]
         assert mrd(vg, gr) < 1e-11
         assert mrd(vg, O.direct(s["pts"], s["tau"], targets, grad=True)) < 1e-5
+
+
+def test_mixed_evaluation_vs_direct_and_f64(setup):
+    """The mixed form's FP32 stages: S2M (source offsets from the node centre formed in
+    FP64, then rounded), M2M (sgemm), and the evaluation (near-field pair differences in
+    FP64, then FP32 with v_rsq_f32 and a Kahan-compensated potential sum; L2T in
+    leaf-relative FP32 coordinates).  Potential and gradient at the sources and at
+    uniform targets against FP64 direct sums, and against the FP64 plan."""
+    s = setup
+    ffmm = s["ffmm"]
+    print("[fmm] mixed evaluation", flush=True)
+    plan = ffmm.DevicePlan(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"], m2l="spectral", precision="mixed")
+    targets = uniform(np.random.default_rng(5), 3000)
+    plan.build_expansions(s["tau"])
+    v = plan.evaluate(targets, ffmm.PotentialGradient).cpu().numpy()
+    assert mrd(v[:, 0], O.direct(s["pts"], s["tau"], targets)) < 2e-6
+    assert mrd(v[:, 1:], O.direct(s["pts"], s["tau"], targets, grad=True)) < 2e-5
+    s["plan"].build_expansions(s["tau"])
+    v64 = s["plan"].evaluate(targets, ffmm.PotentialGradient).cpu().numpy()
+    assert mrd(v[:, 0], v64[:, 0]) < 2e-6 and mrd(v[:, 1:], v64[:, 1:]) < 1e-5
+    u = plan.matvec(s["tau"], ffmm.PotentialGradient).cpu().numpy()
+    rows = s["rng"].choice(len(u), 400, replace=False)
+    assert mrd(u[rows, 0], O.direct_self(s["pts"], s["tau"], rows=rows)) < 2e-6
+    assert mrd(u[rows, 1:], O.direct_self(s["pts"], s["tau"], grad=True, rows=rows)) < 2e-5
+    assert np.all(np.isfinite(u))
 
 
 def test_gradient_at_sources(setup):
