@@ -262,7 +262,7 @@ struct OpCache {
   DBuf<int32_t> embed;                       // Ns: lattice index of each small-surface point
   DBuf<int32_t> sub8;                        // Ns: its index in the p^3 corner of the lattice
   DBuf<int16_t> pairf;                       // [(k * 8 + c) * 8 + a]: far index of (sibling a, child c of colleague k);
-                                             // row 216: the padding candidate, all near
+                                             // row 216: all near (read past the last candidate)
   std::vector<int32_t> embed_h;
   std::vector<int32_t> far_class_h;
 };
@@ -318,7 +318,9 @@ struct Plan {
   // spectral M2L workspace
   DBuf<double> spec_S, spec_U;
   DBuf<float> spec_S32, spec_U32;
-  std::vector<DBuf<int32_t>> spec_gptr, spec_cand;   // per level: sibling groups' far-source candidates
+  std::vector<DBuf<int32_t>> spec_gptr, spec_cand;   // per level: sibling groups' far-source candidates (FP64)
+  std::vector<DBuf<int32_t>> spec_gcol, spec_tiles;  // per level: groups' colleague children, active tiles (FP32)
+  std::vector<int64_t> spec_ntiles;
   // targets
   DBuf<double> tgt_sorted;
   DBuf<int32_t> tgt_node, tgt_leafslot, tgt_order, tgt_ptr;

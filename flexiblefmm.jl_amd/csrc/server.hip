@@ -57,6 +57,8 @@ size_t plan_bytes(const Plan &p) {
   s += cap_bytes(p.spec_S) + cap_bytes(p.spec_U) + cap_bytes(p.spec_S32) + cap_bytes(p.spec_U32);
   for (const auto &b : p.spec_gptr) s += cap_bytes(b);
   for (const auto &b : p.spec_cand) s += cap_bytes(b);
+  for (const auto &b : p.spec_gcol) s += cap_bytes(b);
+  for (const auto &b : p.spec_tiles) s += cap_bytes(b);
   s += cap_bytes(p.tgt_sorted) + cap_bytes(p.tgt_node) + cap_bytes(p.tgt_leafslot) + cap_bytes(p.tgt_order) +
        cap_bytes(p.tgt_ptr) + cap_bytes(p.tgt_keys);
   return s;

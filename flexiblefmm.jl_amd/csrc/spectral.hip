@@ -9,7 +9,8 @@
 //   2. Hadamard: per sibling group (8 targets sharing a parent) and frequency block,
 //      the far pairs are the sourced children of the parent's 27 colleagues; each
 //      source spectrum is read once for the 8 targets, the 316 tables of the block sit
-//      in LDS; one lane per (frequency, target), accumulators in registers, no atomics;
+//      in LDS, accumulators in registers, no atomics.  FP64: one lane per (frequency,
+//      candidate) on the vector ALUs; FP32: 16 groups at a time on the matrix cores;
 //   3. inverse: per target, the three inverse DFTs with only the p^3 corner of the
 //      output formed, the surface entries scaled and added into the partial locals.
 // Follows _spectral_m2l_level! and _hadamard_block! (src/operators/spectral_m2l.jl:
@@ -108,11 +109,6 @@ __global__ void __launch_bounds__(SP_TB) k_spec_forward(const double *__restrict
 // pairf[(k * 8 + c) * 8 + a]: far index of the pair (sibling a, child c of the parent's
 // colleague k), -1 when near.
 // ---------------------------------------------------------------------------
-// Candidate lists are padded to a multiple of CAND_PAD entries with CAND_DUMMY
-// candidates (pairf row 216: every pair near, so they multiply the zero table row).
-constexpr int CAND_PAD = 4;
-constexpr int CAND_DUMMY = NCOLL * NCHILD;
-
 // The far-pair sources of every sibling group of a level, compacted: the sourced
 // children of the parent's 27 colleagues as (level-local source << 8 | 8 k + c), the
 // group's entries at cand[gptr[g] .. gptr[g + 1]) (the enumeration of the lists,
@@ -135,7 +131,7 @@ __global__ void k_group_cand_count(const int32_t *__restrict__ parent, const int
       for (int c = 0; c < NCHILD; c++) n += has_source[cf + c];
     }
   }
-  cnt[g] = (n + CAND_PAD - 1) / CAND_PAD * CAND_PAD;   // padded with zero-table candidates
+  cnt[g] = n;
 }
 __global__ void k_group_cand_fill(const int32_t *__restrict__ parent, const int32_t *__restrict__ child_first,
                                   const int32_t *__restrict__ col, const uint8_t *__restrict__ ghost,
@@ -155,9 +151,6 @@ __global__ void k_group_cand_fill(const int32_t *__restrict__ parent, const int3
     for (int c = 0; c < NCHILD; c++)
       if (has_source[cf + c]) cand[pos++] = (int32_t)((cf + c - lo) << 8) | (k * NCHILD + c);
   }
-  // the padding: the first candidate's (finite) spectrum against the all-near row
-  const int32_t first_pos = gptr[g];
-  for (; pos < gptr[g + 1]; pos++) cand[pos] = (cand[first_pos] & ~255) | CAND_DUMMY;
 }
 
 // One wave per sibling group and frequency block of FB.  The 316 tables of the block
@@ -246,97 +239,181 @@ __global__ void __launch_bounds__(HD_TB) k_spec_hadamard(const typename cplx_t<T
   }
 }
 
-// The FP32 form.  Each candidate is taken by 16 lanes with two adjacent frequencies
-// each, so a 16-lane group reads one contiguous 256-byte segment of one table row: it
-// covers the 64 LDS banks exactly once whatever the row, and the table reads are free of
-// bank conflicts (with narrower rows two candidates share a 16-lane group and their
-// rows collide on banks).  FB = 32 frequencies per block, R = 4 candidates per step;
-// one 16-byte table read and one 16-byte spectrum load feed 16 FMAs per lane, the
-// far-index table holds row offsets (no address multiplies in the inner loop), and a
-// 1024-thread workgroup (16 waves) shares one 81 KB copy of the block's tables.
-// Lane (ph, r) writes siblings r and r + 4, two frequencies each.
-constexpr int HD2_TB = 1024;
-template <int FB>
-__global__ void __launch_bounds__(HD2_TB) k_spec_hadamard2(const float2 *__restrict__ S, float2 *__restrict__ U,
-                                                           const float2 *__restrict__ tables,
-                                                           const int16_t *__restrict__ pairf,
-                                                           const int32_t *__restrict__ gptr,
-                                                           const int32_t *__restrict__ cand, int64_t ngroups, int gpw,
-                                                           int gx_real) {
-  constexpr int LPC = FB / 2;              // lanes per candidate
-  constexpr int R = 64 / LPC;              // candidates per step
-  constexpr int TS = FB;                   // table row: 16 lanes x 16 bytes, no padding needed
-  constexpr int NW = HD2_TB / 64;
-  constexpr int MAXC = NCOLL * NCHILD + CAND_PAD;
-  static_assert(LPC == 16, "one 16-lane group per candidate");
-  static_assert(R == CAND_PAD, "candidate lists are padded to whole steps");
-  static_assert(NCHILD % R == 0, "siblings per lane at the write-out");
-  __shared__ __attribute__((aligned(16))) float2 Tt[(NFAR + 1) * TS];
-  __shared__ int32_t F[(NCOLL * NCHILD + 1) * NCHILD];   // byte offsets of the table rows
-  __shared__ int32_t Cl[NW][MAXC + R];
-  // XCD-aware order: each XCD takes a contiguous run of group chunks at one frequency block
-  const int64_t Lb = xcd_block((int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x, (int64_t)gridDim.x * gridDim.y);
-  const int bx = (int)(Lb % gridDim.x), by = (int)(Lb / gridDim.x);
-  if (bx >= gx_real) return;   // a padding block
-  const int f0 = by * FB;
-  const int t = threadIdx.x;
-  for (int i = t; i < (NFAR + 1) * TS; i += HD2_TB) {
-    const int f = i / TS, ph = i % TS;
-    Tt[i] = f < NFAR ? tables[(int64_t)f * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
+// ---------------------------------------------------------------------------
+// 2'. The FP32 Hadamard on the matrix cores.
+//
+// For one frequency the products of a sibling group are a small matrix product: the
+// targets a of group g gain sum over the parent's colleagues k and their children c of
+// T[F(k, c, a)] S[child c of colleague k], and the 8 x 8 block T[F(k, c, a)] is the same
+// for every group.  So 16 groups (columns) at a time go through
+// v_mfma_f32_16x16x4_f32 with rows (target a, real/imaginary part) and, per colleague,
+// two K-slices ("quads") of 4 children: one MFMA takes the real parts of the 4 children's
+// spectra (row Re: T.re, row Im: T.im), one the imaginary parts (row Re: -T.im, row Im:
+// T.re).  A wave works on one tile of 16 groups at a time, for a block of 16
+// frequencies (16 accumulators of 4 registers).  A quad's 64 spectrum slices (128
+// bytes each) are loaded coalesced, 8 lanes per slice, one quad ahead, and pass through
+// a per-wave LDS buffer whose 16-byte chunks are XOR-swizzled so that both the writes
+// and the MFMA operand reads are free of bank conflicts.  The block's 316 tables sit in
+// LDS, shared by the 8 waves (one tile each).  Quads with no sourced child in any group
+// of the tile are skipped; near pairs multiply the zero table row.  Tiles of ghost
+// groups are not launched.
+// ---------------------------------------------------------------------------
+constexpr int HM_FB = 16;            // frequencies per workgroup
+constexpr int HM_TS = HM_FB + 2;     // table row stride (144 bytes: rows spread over the LDS banks)
+constexpr int HM_W = 8;              // waves per workgroup
+constexpr int HM_R = 1;              // tiles per wave
+constexpr int HM_TB = 64 * HM_W;
+constexpr int HM_TILE = 16;          // sibling groups per tile: the MFMA's columns
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// gcol[g * 27 + k]: (first child of the parent's colleague k, level-local) << 8 | bit c
+// set when that child has sources; -1 when the colleague is missing or has no sourced
+// child, and for every k of a ghost group.  tflag[g / 16] = 1 when some entry is set.
+__global__ void k_group_cols(const int32_t *__restrict__ parent, const int32_t *__restrict__ child_first,
+                             const int32_t *__restrict__ col, const uint8_t *__restrict__ ghost,
+                             const uint8_t *__restrict__ has_source, int64_t lo, int64_t ngroups, int64_t nn,
+                             int32_t *__restrict__ gcol, uint8_t *__restrict__ tflag) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t first = lo + NCHILD * g;
+  const bool real = !ghost[first];
+  const int32_t P = parent[first];
+  bool any = false;
+  for (int k = 0; k < NCOLL; k++) {
+    int32_t e = -1;
+    const int32_t pc = real ? col[(int64_t)k * nn + P] : -1;
+    const int32_t cf = pc >= 0 ? child_first[pc] : -1;
+    if (cf >= 0) {
+      int32_t m = 0;
+      for (int c = 0; c < NCHILD; c++) m |= (int32_t)(has_source[cf + c] != 0) << c;
+      if (m) e = (int32_t)((cf - lo) << 8) | m;
+    }
+    gcol[g * NCOLL + k] = e;
+    any |= e >= 0 && k != SELF_SLOT;
   }
-  for (int i = t; i < (NCOLL * NCHILD + 1) * NCHILD; i += HD2_TB)
-    F[i] = (pairf[i] < 0 ? NFAR : pairf[i]) * TS * (int)sizeof(float2);
+  if (any) tflag[g / HM_TILE] = 1;   // every writer stores the same value
+}
+__global__ void k_fill_u8(uint8_t *__restrict__ a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// The staging buffer of a quad: row r = 16 q + j (child q of the quad in group j), 8
+// chunks of 16 bytes (2 frequencies); chunk h of row r sits at position h ^ ((r >> 1) & 7).
+// Load instruction n (0..7) covers rows 8 n .. 8 n + 7, 8 lanes per row.
+__device__ __forceinline__ void hm_quad_load(const float2 *__restrict__ S, int32_t e_lo, int32_t e_hi, int quad,
+                                             int f0, int lane, float4 v[8]) {
+  // row r = 8 n + (lane >> 3): group (lane >> 3) + 8 (n & 1), child (n >> 1) + 4 quad,
+  // position lane & 7 holding chunk (lane & 7) ^ (4 (n & 1) + (lane >> 4))
+  const int pos = lane & 7, hb = lane >> 4;
+  const int32_t m0 = e_lo >= 0 ? (e_lo >> (4 * quad)) & 15 : 0, m1 = e_hi >= 0 ? (e_hi >> (4 * quad)) & 15 : 0;
+  const float2 *b0 = S + (int64_t)((e_lo >> 8) + 4 * quad) * SP_NF + f0 + 2 * (pos ^ hb);
+  const float2 *b1 = S + (int64_t)((e_hi >> 8) + 4 * quad) * SP_NF + f0 + 2 * (pos ^ (4 + hb));
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const int32_t m = (n & 1) ? m1 : m0;
+    const float2 *bp = (n & 1) ? b1 : b0;
+    v[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((m >> (n >> 1)) & 1) v[n] = *reinterpret_cast<const float4 *>(bp + (n >> 1) * SP_NF);
+  }
+}
+__device__ __forceinline__ void hm_quad_store(float4 *buf, int lane, const float4 v[8]) {
+#pragma unroll
+  for (int n = 0; n < 8; n++) buf[(8 * n + (lane >> 3)) * 8 + (lane & 7)] = v[n];
+}
+
+// one quad: 16 frequencies x (real, imaginary) = 32 MFMAs
+__device__ __forceinline__ void hm_quad_mfma(f32x4 acc[HM_FB], const float4 *buf, const char *tb, int toff, int q,
+                                             int j, bool im_row) {
+  const float4 *row = buf + (16 * q + j) * 8;
+  const int sw = (j >> 1) & 7;   // row 16 q + j: (r >> 1) & 7 = (j >> 1) & 7
+#pragma unroll
+  for (int fp = 0; fp < HM_FB / 2; fp++) {
+    const float4 tt = *reinterpret_cast<const float4 *>(tb + toff + fp * (int)sizeof(float4));
+    const float4 s = row[fp ^ sw];
+    acc[2 * fp] = __builtin_amdgcn_mfma_f32_16x16x4f32(im_row ? tt.y : tt.x, s.x, acc[2 * fp], 0, 0, 0);
+    acc[2 * fp + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(im_row ? tt.w : tt.z, s.z, acc[2 * fp + 1], 0, 0, 0);
+    acc[2 * fp] = __builtin_amdgcn_mfma_f32_16x16x4f32(im_row ? tt.x : -tt.y, s.y, acc[2 * fp], 0, 0, 0);
+    acc[2 * fp + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(im_row ? tt.z : -tt.w, s.w, acc[2 * fp + 1], 0, 0, 0);
+  }
+}
+
+__global__ void __launch_bounds__(HM_TB, 1) k_spec_hadamard_mfma(const float2 *__restrict__ S, float2 *__restrict__ U,
+                                                                const float2 *__restrict__ tables,
+                                                                const int16_t *__restrict__ pairf,
+                                                                const int32_t *__restrict__ gcol,
+                                                                const int32_t *__restrict__ tiles, int64_t ntiles,
+                                                                int64_t ngroups, int64_t nwg, int64_t nreal) {
+  __shared__ __attribute__((aligned(16))) float2 Tt[(NFAR + 1) * HM_TS];
+  __shared__ int32_t F[NCOLL * NCHILD * NCHILD];   // byte offsets of the table rows
+  __shared__ int32_t Gs[HM_W][HM_TILE * NCOLL];
+  __shared__ float4 Sb[HM_W][64 * 8];               // per wave: one quad's spectra
+  // XCD-aware order: each XCD sweeps a contiguous range of (frequency block, tile)
+  const int64_t Lb = xcd_block(blockIdx.x, gridDim.x);
+  if (Lb >= nreal) return;   // a padding block
+  const int f0 = (int)(Lb / nwg) * HM_FB;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  for (int i = t; i < (NFAR + 1) * HM_FB; i += HM_TB) {
+    const int row = i / HM_FB, ph = i % HM_FB;
+    Tt[row * HM_TS + ph] = row < NFAR ? tables[(int64_t)row * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
+  }
+  for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HM_TB)
+    F[i] = (pairf[i] < 0 ? NFAR : pairf[i]) * HM_TS * (int)sizeof(float2);
   __syncthreads();
-  const int wave = t >> 6, lane = t & 63;
-  const int ph = 2 * (lane % LPC), r = lane / LPC;
-  const char *tbase = reinterpret_cast<const char *>(Tt) + ph * sizeof(float2);
-  int32_t *cl = Cl[wave];
-  const int64_t g0 = ((int64_t)bx * NW + wave) * gpw;
-  for (int64_t g = g0; g < g0 + gpw && g < ngroups; g++) {
-    const int32_t cbeg = gptr[g], n = gptr[g + 1] - cbeg;   // a multiple of R
-    if (n == 0) continue;
-    for (int i = lane; i < n + R; i += 64) cl[i] = cand[cbeg + (i < n ? i : 0)];   // + R copies for the prefetch
+  const int j = lane & 15, q = lane >> 4, a = j >> 1;
+  const bool im_row = j & 1;
+  const int32_t *gw = Gs[wave];
+  float4 *buf = Sb[wave];
+  const char *tb = reinterpret_cast<const char *>(Tt);
+  const int jl = lane >> 3;   // the two groups a lane loads for: jl and jl + 8
+  // the workgroup's HM_W * HM_R tiles, dealt to the waves round-robin
+  for (int r = 0; r < HM_R; r++) {
+    const int64_t ti = (Lb % nwg) * (HM_W * HM_R) + r * HM_W + wave;
+    if (ti >= ntiles) break;
+    const int64_t g0 = (int64_t)tiles[ti] * HM_TILE;
     __builtin_amdgcn_wave_barrier();
-    float4 acc[NCHILD];
-#pragma unroll
-    for (int a = 0; a < NCHILD; a++) acc[a] = make_float4(0.f, 0.f, 0.f, 0.f);
-    int32_t e = cl[r];
-    float4 v = *reinterpret_cast<const float4 *>(&S[(int64_t)(e >> 8) * SP_NF + f0 + ph]);
-    for (int j = 0; j < n; j += R) {
-      const int32_t en = cl[j + R + r];
-      const float4 vn = *reinterpret_cast<const float4 *>(&S[(int64_t)(en >> 8) * SP_NF + f0 + ph]);
-      const int32_t *Fk = &F[(e & 255) * NCHILD];
-#pragma unroll
-      for (int a = 0; a < NCHILD; a++) {
-        const float4 tt = *reinterpret_cast<const float4 *>(tbase + Fk[a]);
-        acc[a].x = fmaf(tt.x, v.x, fmaf(-tt.y, v.y, acc[a].x));
-        acc[a].y = fmaf(tt.x, v.y, fmaf(tt.y, v.x, acc[a].y));
-        acc[a].z = fmaf(tt.z, v.z, fmaf(-tt.w, v.w, acc[a].z));
-        acc[a].w = fmaf(tt.z, v.w, fmaf(tt.w, v.z, acc[a].w));
-      }
-      e = en;
-      v = vn;
-    }
-#pragma unroll
-    for (int a = 0; a < NCHILD; a++) {
-#pragma unroll
-      for (int off = LPC; off < 64; off <<= 1) {
-        acc[a].x += __shfl_xor(acc[a].x, off, 64);
-        acc[a].y += __shfl_xor(acc[a].y, off, 64);
-        acc[a].z += __shfl_xor(acc[a].z, off, 64);
-        acc[a].w += __shfl_xor(acc[a].w, off, 64);
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < NCHILD / R; m++) {
-      const int a = r + m * R;
-      float4 o = acc[0];
-#pragma unroll
-      for (int b2 = 1; b2 < NCHILD; b2++)
-        if (b2 == a) o = acc[b2];
-      *reinterpret_cast<float4 *>(&U[(NCHILD * g + a) * SP_NF + f0 + ph]) = o;
+    for (int i = lane; i < HM_TILE * NCOLL; i += 64) {
+      const int64_t g = g0 + i / NCOLL;
+      Gs[wave][i] = g < ngroups ? gcol[g * NCOLL + i % NCOLL] : -1;
     }
     __builtin_amdgcn_wave_barrier();
+    // the quads (colleague k, children 4 b .. 4 b + 3) with a sourced child in some
+    // group of the tile: bit 2 k + b (wave-uniform)
+    uint64_t qmask = 0;
+    for (int k = 0; k < NCOLL; k++) {
+      const int32_t e = gw[j * NCOLL + k];
+      const int32_t m = (k != SELF_SLOT && e >= 0) ? e & 255 : 0;
+      if (__ballot(m & 0x0F)) qmask |= 1ull << (2 * k);
+      if (__ballot(m & 0xF0)) qmask |= 1ull << (2 * k + 1);
+    }
+    f32x4 acc[HM_FB];
+#pragma unroll
+    for (int f = 0; f < HM_FB; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float4 v[8];
+    int qd = qmask ? __builtin_ctzll(qmask) : -1;
+    if (qd >= 0) hm_quad_load(S, gw[jl * NCOLL + (qd >> 1)], gw[(jl + 8) * NCOLL + (qd >> 1)], qd & 1, f0, lane, v);
+    while (qd >= 0) {
+      qmask &= qmask - 1;
+      const int qn = qmask ? __builtin_ctzll(qmask) : -1;
+      __builtin_amdgcn_wave_barrier();   // the previous quad's operand reads precede these writes
+      hm_quad_store(buf, lane, v);
+      __builtin_amdgcn_wave_barrier();
+      if (qn >= 0)
+        hm_quad_load(S, gw[jl * NCOLL + (qn >> 1)], gw[(jl + 8) * NCOLL + (qn >> 1)], qn & 1, f0, lane, v);
+      hm_quad_mfma(acc, buf, tb, F[((qd >> 1) * NCHILD + q + 4 * (qd & 1)) * NCHILD + a], q, j, im_row);
+      qd = qn;
+    }
+    // lane (q, j) holds rows 4q .. 4q + 3 of column j: targets 2q and 2q + 1, (re, im) each
+    const int64_t g = g0 + j;
+    if (g < ngroups) {
+      float4 *u0 = reinterpret_cast<float4 *>(U + (int64_t)(NCHILD * g + 2 * q) * SP_NF + f0);
+      float4 *u1 = reinterpret_cast<float4 *>(U + (int64_t)(NCHILD * g + 2 * q + 1) * SP_NF + f0);
+#pragma unroll
+      for (int fp = 0; fp < HM_FB / 2; fp++) {
+        u0[fp] = make_float4(acc[2 * fp][0], acc[2 * fp][1], acc[2 * fp + 1][0], acc[2 * fp + 1][1]);
+        u1[fp] = make_float4(acc[2 * fp][2], acc[2 * fp][3], acc[2 * fp + 1][2], acc[2 * fp + 1][3]);
+      }
+    }
   }
 }
 
@@ -422,9 +499,32 @@ void spectral_prepare(Plan &plan) {
     plan.spec_S.resize((size_t)2 * SP_NF * nmax);
     plan.spec_U.resize((size_t)2 * SP_NF * nmax);
   }
-  // the sibling groups' candidate lists, once per plan (they follow the tree only)
   Context &ctx = *plan.ctx;
   hipStream_t s = ctx.stream;
+  if (plan.precision == FFMM_PREC_MIXED) {
+    // the groups' colleague tables and the tiles with far sources, once per plan
+    plan.spec_gcol.clear();
+    plan.spec_tiles.clear();
+    plan.spec_gcol.resize(plan.L);
+    plan.spec_tiles.resize(plan.L);
+    plan.spec_ntiles.assign(plan.L, 0);
+    DBuf<uint8_t> tflag;
+    for (int l = 2; l < plan.L; l++) {
+      const int64_t lo = plan.lo[l], ng = plan.nl[l] / NCHILD, nt = (ng + HM_TILE - 1) / HM_TILE;
+      if (ng == 0) continue;
+      plan.spec_gcol[l].resize((size_t)ng * NCOLL);
+      tflag.resize(nt);
+      k_fill_u8<<<nblocks(nt, 256), 256, 0, s>>>(tflag.p, nt);
+      k_group_cols<<<nblocks(ng, 256), 256, 0, s>>>(plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p,
+                                                    plan.has_source.p, lo, ng, plan.nnodes, plan.spec_gcol[l].p,
+                                                    tflag.p);
+      FFMM_LAUNCH_CHECK();
+      plan.spec_tiles[l].resize(nt);
+      plan.spec_ntiles[l] = compact_flags_u8(ctx, tflag.p, plan.spec_tiles[l].p, nt);
+    }
+    return;
+  }
+  // the sibling groups' candidate lists, once per plan (they follow the tree only)
   plan.spec_gptr.clear();
   plan.spec_cand.clear();
   plan.spec_gptr.resize(plan.L);
@@ -465,12 +565,12 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   {
     StageTimer tm(plan, 8, l);   // the Hadamard launch alone
     if constexpr (std::is_same<T, float>::value) {
-      constexpr int FB2 = 32, gpw2 = 4;
-      const int64_t gpb2 = (HD2_TB / 64) * gpw2;
-      const int gx = (int)((ng + gpb2 - 1) / gpb2);
-      const dim3 gpad((unsigned)pad_xcd(gx), SP_NF / FB2);
-      k_spec_hadamard2<FB2><<<gpad, HD2_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
-                                                    ng, gpw2, gx);
+      const int64_t ntiles = plan.spec_ntiles[l];
+      if (ntiles > 0) {
+        const int64_t nwg = (ntiles + HM_W * HM_R - 1) / (HM_W * HM_R), nreal = nwg * (SP_NF / HM_FB);
+        k_spec_hadamard_mfma<<<(unsigned)pad_xcd(nreal), HM_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gcol[l].p,
+                                                                        plan.spec_tiles[l].p, ntiles, ng, nwg, nreal);
+      }
     }
     else
       k_spec_hadamard<T, FB><<<grid, HD_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gptr[l].p, plan.spec_cand[l].p,
