@@ -170,15 +170,29 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
         del plan
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t1
+    # and in the plain form (every request FP64 with dense class M2L, the reference
+    # device plan's form), one after another
+    t1 = time.perf_counter()
+    for r in reqs:
+        key = (r["lap"], r["fine"])
+        plan = ffmm.DevicePlan(methods[key], r["pts"], ppl=64, bbox=box, cache=caches[key], precision="f64",
+                               m2l="dense")
+        plan.matvec(r["tau"]).cpu()
+        del plan
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t1
     lat = np.array(lat)
     return {"requests": nreq, "points": total, "slots": nslots, "arrival_rate_per_s": rate,
             "n_range": [nmin, nmax], "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)), "served_points_per_s": total / t_served,
             "served_wall_s": t_served, "sequential_points_per_s": total / t_seq, "sequential_wall_s": t_seq,
-            "speedup_vs_sequential": t_seq / t_served, "plans_built": st["plans_built"],
+            "speedup_vs_sequential": t_seq / t_served, "sequential_plain_points_per_s": total / t_plain,
+            "sequential_plain_wall_s": t_plain, "speedup_vs_sequential_plain": t_plain / t_served,
+            "plans_built": st["plans_built"],
             "plans_reused": st["plans_reused"], "peak_device_bytes": st["peak_bytes"],
             "note": "arrivals replayed in real time (the served wall time includes the arrival gaps); "
-                    "the sequential leg solves back to back with a new plan per request"}
+                    "the sequential legs solve back to back with a new plan per request, in the served "
+                    "requests' forms and in the plain form (FP64, dense class M2L)"}
 
 
 def main():
@@ -199,7 +213,7 @@ def main():
     ap.add_argument("--serve-nmin", type=int, default=10_000)
     ap.add_argument("--serve-nmax", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="before the steps, copy a buffer of this size once on the device (a kernel of known "
                          "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile)")

@@ -761,9 +761,11 @@ int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *s
     Plan &p = plan->c;
     if (level < 0 || level >= p.L) fail(FFMM_ERR_ARGUMENT, "level out of range");
     set_device(*p.ctx);
+    AllocScope scope(*p.ctx);
     LevelLists &li = p.lists[level];
     hipStream_t s = p.ctx->stream;
     if (ptr) li.ptr.download(ptr, p.nl[level] + 1, s);
+    if ((src || far) && li.np) fill_level_pairs(p, level);
     if (src && li.np) li.src.download(src, li.np, s);
     if (far && li.np) li.far.download(far, li.np, s);
     p.ctx->sync();

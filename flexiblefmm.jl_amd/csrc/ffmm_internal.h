@@ -270,6 +270,7 @@ struct OpCache {
 // per-level lists of far pairs (targets in node order) and their class-major form
 struct LevelLists {
   int64_t np = 0, nruns = 0;
+  bool filled = false;                  // src/far written (the spectral M2L needs only ptr)
   DBuf<int32_t> ptr, src, far;          // by target: ptr (nl + 1, 0-based pair index)
   DBuf<int32_t> csrc, cfar, rtgt, rptr; // class major; runs: (level-local target, first pair)
   std::vector<int64_t> class_pair;      // nclass + 1 offsets into csrc
@@ -354,6 +355,7 @@ void node_leafslots(Plan &plan);
 
 // ---- lists (lists.hip) ----
 void build_lists(Plan &plan);
+void fill_level_pairs(Plan &plan, int l);   // src/far of level l, when not yet written
 
 // ---- passes (passes.hip) ----
 void allocate_pass_buffers(Plan &plan);

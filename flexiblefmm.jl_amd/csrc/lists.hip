@@ -141,6 +141,20 @@ __global__ void k_class_offsets(const int32_t *__restrict__ cstart, const int32_
   out[nclass + c] = (int32_t)a;
 }
 
+void fill_level_pairs(Plan &plan, int l) {
+  LevelLists &li = plan.lists[l];
+  if (li.filled) return;
+  TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};
+  li.src.resize(li.np);
+  li.far.resize(li.np);
+  if (li.np > 0) {
+    k_pair_fill<<<nblocks(plan.nl[l], TB), TB, 0, plan.ctx->stream>>>(t, plan.lo[l], plan.nl[l], li.ptr.p, li.src.p,
+                                                                      li.far.p);
+    FFMM_LAUNCH_CHECK();
+  }
+  li.filled = true;
+}
+
 void build_lists(Plan &plan) {
   Context &ctx = *plan.ctx;
   OpCache &ops = *plan.ops;
@@ -171,6 +185,7 @@ void build_lists(Plan &plan) {
     li.ptr.resize(nl + 1);
     if (l < 2) {
       FFMM_HIP(hipMemsetAsync(li.ptr.p, 0, sizeof(int32_t) * (nl + 1), s));
+      li.filled = true;
       continue;
     }
     cnt.resize(nl);
@@ -180,14 +195,11 @@ void build_lists(Plan &plan) {
     if (np >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many far pairs on one level for 32-bit pair indices");
     k_set_i32<<<1, 1, 0, s>>>(li.ptr.p + nl, (int32_t)np);
     li.np = np;
-    li.src.resize(np);
-    li.far.resize(np);
-    if (np > 0) {
-      k_pair_fill<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, li.ptr.p, li.src.p, li.far.p);
-      FFMM_LAUNCH_CHECK();
-    }
-    // the class lists (only the dense class translations read them)
+    // the pairs themselves: the spectral M2L reads only the counts (its Hadamard works
+    // from the colleague tables), so they are written on demand (fill_level_pairs)
     if (plan.m2l != FFMM_M2L_DENSE) continue;
+    fill_level_pairs(plan, l);
+    // the class lists (only the dense class translations read them)
     int64_t ne = (int64_t)nclass * nl;
     ccnt.resize(ne);
     cstart.resize(ne);

@@ -57,6 +57,23 @@ def test_device_tree_equals_host_tree(name, n, ppl):
     assert np.array_equal(device_plan(pts, ppl).download_tree().perm, dt.perm)
 
 
+def test_spectral_plan_pairs_written_on_demand():
+    """A spectral plan writes its far pairs only when they are downloaded (its M2L reads
+    the per-target counts alone); they equal the oracle's lists."""
+    import ffmm
+    rng = np.random.default_rng(43)
+    pts = DISTRIBUTIONS["mixed"](rng, 8000)
+    print("[tree] spectral plan lists", flush=True)
+    ref = O.build_tree(pts, 40, BOX[0], BOX[1])
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, m2l="spectral", precision="mixed")
+    for l in range(ref.nlevels):
+        ptr, src, far = plan.download_lists(l)
+        rptr, rsrc, rfar = ref.lists[l]
+        assert np.array_equal(ptr, rptr) and np.array_equal(src, rsrc) and np.array_equal(far, rfar), f"level {l}"
+    assert np.array_equal(plan.level_pairs, [len(ref.lists[l][1]) for l in range(ref.nlevels)])
+
+
 def test_device_tree_gaussian_mixture():
     import ffmm.workloads as W
     pts = W.gaussian_mixture(200_000, seed=7)
