@@ -347,6 +347,10 @@ void sort_pairs_u32(Context &ctx, const uint32_t *kin, uint32_t *kout, const int
 int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_t n);
 // indices i (ascending) with flag[i] != 0; returns their number (blocking read back)
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n);
+// the count into the context's pinned slot `slot` (0 .. 63) by the stream, no synchronisation
+void compact_flags_u8_async(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n, int slot);
+void set_pinned(Context &ctx, int slot, int64_t v);
+constexpr int NPINNED = 64;
 
 // ---- tree (tree.hip) ----
 void bounds(Context &ctx, const double *xyz, int64_t n, double lo[3], double hi[3], int *all_finite);

@@ -75,6 +75,36 @@ int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_
   return ctx.pinned[0];
 }
 
+__global__ void k_copy_count_slot(const int32_t *__restrict__ dcount, int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) host[0] = dcount[0];
+}
+
+// the same compaction with its count stored into pinned readback slot `slot` by the
+// stream: no synchronisation, so that several can be read back at once
+void compact_flags_u8_async(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n, int slot) {
+  if (n <= 0) {
+    set_pinned(ctx, slot, 0);
+    return;
+  }
+  rocprim::counting_iterator<int32_t> it(0);
+  size_t bytes = 0;
+  FFMM_HIP(rocprim::select(nullptr, bytes, it, flag, idx, (int32_t *)nullptr, (size_t)n, ctx.stream));
+  size_t off = (bytes + 255) & ~size_t(255);
+  char *t = static_cast<char *>(scratch(ctx, off + 256));
+  int32_t *dcount = reinterpret_cast<int32_t *>(t + off);
+  FFMM_HIP(rocprim::select(t, bytes, it, flag, idx, dcount, (size_t)n, ctx.stream));
+  k_copy_count_slot<<<1, 64, 0, ctx.stream>>>(dcount, ctx.pinned_dev + slot);
+  FFMM_HIP(hipGetLastError());
+}
+
+__global__ void k_set_slot(int64_t *__restrict__ host, int64_t v) {
+  if (threadIdx.x == 0) host[0] = v;
+}
+void set_pinned(Context &ctx, int slot, int64_t v) {
+  k_set_slot<<<1, 64, 0, ctx.stream>>>(ctx.pinned_dev + slot, v);
+  FFMM_HIP(hipGetLastError());
+}
+
 void set_i32(Context &ctx, int32_t *out, int32_t v) {
   k_set_value<<<1, 64, 0, ctx.stream>>>(out, v);
   FFMM_HIP(hipGetLastError());

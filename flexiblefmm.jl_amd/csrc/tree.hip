@@ -347,9 +347,29 @@ __global__ void k_flag_s2m(const int32_t *__restrict__ cf, const uint8_t *__rest
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) flag[i] = cf[lo + i] < 0 && hs[lo + i];
 }
-__global__ void k_add_offset(int32_t *__restrict__ a, int64_t n, int64_t off) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] += (int32_t)off;
+// the first node of each level (and the node count), for the boundaries below
+struct LevelStarts {
+  int64_t lo[MAX_KEY_LEVELS + 2];
+};
+static_assert(2 + 2 * (MAX_KEY_LEVELS + 2) <= NPINNED, "pinned slots of the level boundaries");
+// per level l (and l = L): the position in each node list (ascending node ids; their
+// lengths in pinned slots 0 and 1) of the first node >= lo[l]; into pinned slots 2 + l
+// and 2 + (MAX_KEY_LEVELS + 2) + l
+__global__ void k_level_bounds(const int32_t *__restrict__ s2m, const int32_t *__restrict__ leaves, LevelStarts ls,
+                               int L, int64_t *__restrict__ pinned) {
+  const int l = threadIdx.x;
+  if (l > L) return;
+  const int64_t key = ls.lo[l];
+  for (int w = 0; w < 2; w++) {
+    const int32_t *a = w ? leaves : s2m;
+    int64_t lo = 0, hi = pinned[w];
+    while (lo < hi) {
+      const int64_t m = (lo + hi) >> 1;
+      if ((int64_t)a[m] < key) lo = m + 1;
+      else hi = m;
+    }
+    pinned[2 + w * (MAX_KEY_LEVELS + 2) + l] = lo;
+  }
 }
 
 static void split(Context &ctx, DevLevel &par, const int32_t *which, int64_t nw, int plvl, DevLevel &chi,
@@ -465,12 +485,19 @@ void build_tree(Plan &plan, const double *xyz) {
                                                 levels[l].xlist.p);
     FFMM_LAUNCH_CHECK();
   }
+  // every level's xlist compacted first, the counts read back at once (the flags are
+  // fixed before any ghost split, and ghosts carry no xlist flag)
+  std::vector<DBuf<int32_t>> xw(L);
   for (int l = 1; l <= L - 2; l++) {
-    int64_t n = levels[l].n;
-    which.resize(n);
-    int64_t nx = compact_flags_u8(ctx, levels[l].xlist.p, which.p, n);
-    if (nx == 0) continue;
-    split(ctx, levels[l], which.p, nx, l, levels[l + 1], skeys.p, KL, true);
+    xw[l].resize(levels[l].n);
+    compact_flags_u8_async(ctx, levels[l].xlist.p, xw[l].p, levels[l].n, l);
+  }
+  ctx.sync();
+  std::vector<int64_t> nxl(L, 0);
+  for (int l = 1; l <= L - 2; l++) nxl[l] = ctx.pinned[l];
+  for (int l = 1; l <= L - 2; l++) {
+    if (nxl[l] == 0) continue;
+    split(ctx, levels[l], xw[l].p, nxl[l], l, levels[l + 1], skeys.p, KL, true);
     tag(ctx, levels[l + 1], levels[l]);
   }
   // flattening (src/tree/device_builder.jl:476-501)
@@ -502,25 +529,26 @@ void build_tree(Plan &plan, const double *xyz) {
   k_centers<<<nblocks(nn, TB), TB, 0, s>>>(plan.cell.p, plan.level.p, nn, plan.origin[0], plan.origin[1],
                                            plan.origin[2], plan.W0, plan.centers.p);
   FFMM_LAUNCH_CHECK();
-  // node lists per level
+  // node lists: both compacted over all nodes at once (node order is level order), the
+  // level boundaries found on the device; one read back
   plan.s2m_off.assign(L + 1, 0);
   plan.leaf_off.assign(L + 1, 0);
   plan.s2m_nodes.resize(nn);
   plan.leaf_nodes.resize(nn);
-  for (int l = 0; l < L; l++) {
-    int64_t lo = plan.lo[l], n = plan.nl[l];
-    flag.resize(n);
-    k_flag_s2m<<<nblocks(n, TB), TB, 0, s>>>(plan.child_first.p, plan.has_source.p, lo, n, flag.p);
-    FFMM_LAUNCH_CHECK();
-    int64_t c = compact_flags_u8(ctx, flag.p, plan.s2m_nodes.p + plan.s2m_off[l], n);
-    if (c) k_add_offset<<<nblocks(c, TB), TB, 0, s>>>(plan.s2m_nodes.p + plan.s2m_off[l], c, lo);
-    plan.s2m_off[l + 1] = plan.s2m_off[l] + c;
-    int64_t cl = compact_flags_u8(ctx, plan.leaf.p + lo, plan.leaf_nodes.p + plan.leaf_off[l], n);
-    if (cl) k_add_offset<<<nblocks(cl, TB), TB, 0, s>>>(plan.leaf_nodes.p + plan.leaf_off[l], cl, lo);
-    plan.leaf_off[l + 1] = plan.leaf_off[l] + cl;
-    FFMM_LAUNCH_CHECK();
-  }
+  flag.resize(nn);
+  k_flag_s2m<<<nblocks(nn, TB), TB, 0, s>>>(plan.child_first.p, plan.has_source.p, 0, nn, flag.p);
+  FFMM_LAUNCH_CHECK();
+  compact_flags_u8_async(ctx, flag.p, plan.s2m_nodes.p, nn, 0);
+  compact_flags_u8_async(ctx, plan.leaf.p, plan.leaf_nodes.p, nn, 1);
+  LevelStarts ls{};
+  for (int l = 0; l <= L; l++) ls.lo[l] = l < L ? plan.lo[l] : nn;
+  k_level_bounds<<<1, 64, 0, s>>>(plan.s2m_nodes.p, plan.leaf_nodes.p, ls, L, ctx.pinned_dev);
+  FFMM_LAUNCH_CHECK();
   ctx.sync();
+  for (int l = 0; l <= L; l++) {
+    plan.s2m_off[l] = ctx.pinned[2 + l];
+    plan.leaf_off[l] = ctx.pinned[2 + (MAX_KEY_LEVELS + 2) + l];
+  }
 }
 
 // node -> position in the leaf list (-1: not a real leaf)
