@@ -318,12 +318,15 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
         s8[i] = i1 + o.p * (i2 + o.p * i3);
       }
       o.sub8.upload(s8.data(), Ns, s);
+      std::vector<int16_t> l8((size_t)o.p * o.p * o.p, (int16_t)-1);   // its inverse: -1 off the surface
+      for (int i = 0; i < Ns; i++) l8[s8[i]] = (int16_t)i;
+      o.lat8.upload(l8.data(), l8.size(), s);
       int far_index[343];
       for (int u = 0, nf = 0; u < 343; u++) {
         int x = u % 7 - 3, y = (u / 7) % 7 - 3, z = u / 49 - 3;
         far_index[u] = std::max(std::abs(x), std::max(std::abs(y), std::abs(z))) >= 2 ? nf++ : -1;
       }
-      // one more row for the padding candidate of the spectral Hadamard: all near (zero)
+      // one more row, all near: the FP64 Hadamard's table copy reads past the last candidate
       std::vector<int16_t> pf((NCOLL * NCHILD + 1) * NCHILD, (int16_t)-1);
       for (int k = 0; k < NCOLL; k++)
         for (int c = 0; c < NCHILD; c++)

@@ -261,6 +261,7 @@ struct OpCache {
   DBuf<float> tables32;
   DBuf<int32_t> embed;                       // Ns: lattice index of each small-surface point
   DBuf<int32_t> sub8;                        // Ns: its index in the p^3 corner of the lattice
+  DBuf<int16_t> lat8;                        // p^3: the surface point at a corner lattice index, or -1
   DBuf<int16_t> pairf;                       // [(k * 8 + c) * 8 + a]: far index of (sibling a, child c of colleague k);
                                              // row 216: all near (read past the last candidate)
   std::vector<int32_t> embed_h;

@@ -28,13 +28,10 @@ constexpr int SP_P = 8;                       // points per edge
 constexpr int SP_N = 2 * SP_P;                // padded lattice
 constexpr int SP_H = SP_P + 1;                // half spectrum along the first axis
 constexpr int SP_NF = SP_H * SP_N * SP_N;     // 2304 frequencies
-constexpr int SP_TB = 256;
 
 template <class T> struct cplx_t;
 template <> struct cplx_t<double> { using type = double2; };
 template <> struct cplx_t<float> { using type = float2; };
-
-__constant__ double c_w16[2 * SP_N];   // exp(-2 pi i j / 16), j = 0..15: (cos, -sin)
 
 template <class C, class T>
 __device__ __forceinline__ void cmac(C &acc, C a, T br, T bi) {   // acc += a * (br + i bi)
@@ -42,65 +39,92 @@ __device__ __forceinline__ void cmac(C &acc, C a, T br, T bi) {   // acc += a * 
   acc.y = fma(a.x, bi, fma(a.y, br, acc.y));
 }
 
+// The 16th roots of unity, cos(2 pi j / 16) and sin(2 pi j / 16): every twiddle of the
+// transforms below is one of them, at an index known at compile time once the loops are
+// unrolled, so it is an immediate and no table is read.
+__device__ constexpr double W16C[SP_N] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+                                          0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+                                          -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977,
+                                          0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674};
+__device__ constexpr double W16S[SP_N] = {0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674,
+                                          1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+                                          0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+                                          -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977};
+
 // ---------------------------------------------------------------------------
-// 1. forward: S[node][k3][k2][k1] = sum_{i in [0,p)^3} x[i3][i2][i1] w^(i1 k1 + i2 k2 + i3 k3)
+// 1. forward: S[node][k3][k2][k1] = sum_{i in [0,p)^3} x[i3][i2][i1] w^(i1 k1 + i2 k2 + i3 k3),
+//    w = exp(-2 pi i / 16).  One workgroup per box; each pass is one short DFT per thread
+//    with its 8 inputs in registers: (i3, i2) rows over i1 (real input, k1 <= p), then
+//    (i3, k1) over i2, then (k2, k1) over i3, whose 16 outputs per thread go straight to
+//    the spectrum (consecutive threads, consecutive (k2, k1): coalesced).
 // ---------------------------------------------------------------------------
+constexpr int FW_TB = 192;   // 3 waves: 64, 72 and 144 threads work in the three passes
+
 template <class T>
-__global__ void __launch_bounds__(SP_TB) k_spec_forward(const double *__restrict__ Mp, int Ns,
-                                                        const int32_t *__restrict__ sub, int64_t lo, int64_t nl,
+__global__ void __launch_bounds__(FW_TB) k_spec_forward(const double *__restrict__ Mp, int Ns,
+                                                        const int16_t *__restrict__ lat8, int64_t lo, int64_t nl,
                                                         const uint8_t *__restrict__ has_source,
                                                         typename cplx_t<T>::type *__restrict__ S) {
   using C = typename cplx_t<T>::type;
-  __shared__ T X[SP_P * SP_P * SP_P];
-  __shared__ C A[SP_P * SP_P * SP_H];
-  __shared__ C B[SP_P * SP_N * SP_H];
-  __shared__ T wr[SP_N], wi[SP_N];
+  __shared__ C A[SP_P * SP_P * SP_H];   // [i3][i2][k1]
+  __shared__ C B[SP_P * SP_N * SP_H];   // [i3][k2][k1]
   const int64_t q = blockIdx.x;
   const int64_t node = lo + q;
   if (!has_source[node]) return;
   const int t = threadIdx.x;
-  if (t < SP_N) { wr[t] = (T)c_w16[2 * t]; wi[t] = (T)c_w16[2 * t + 1]; }
-  for (int i = t; i < SP_P * SP_P * SP_P; i += SP_TB) X[i] = (T)0;
-  __syncthreads();
-  for (int i = t; i < Ns; i += SP_TB) X[sub[i]] = (T)Mp[node * Ns + i];
-  __syncthreads();
-  // first axis (real input): A[i3][i2][k1], k1 < p + 1
-  for (int o = t; o < SP_P * SP_P * SP_H; o += SP_TB) {
-    const int k1 = o % SP_H, r = o / SP_H;
-    C acc = {(T)0, (T)0};
+  const double *__restrict__ m = Mp + node * Ns;
+  if (t < SP_P * SP_P) {   // row (i3, i2) = t: the p lattice values (zero off the surface)
+    T x[SP_P];
 #pragma unroll
     for (int i1 = 0; i1 < SP_P; i1++) {
-      const int w = (i1 * k1) & (SP_N - 1);
-      const T x = X[r * SP_P + i1];
-      acc.x = fma(x, wr[w], acc.x);
-      acc.y = fma(x, wi[w], acc.y);
+      const int sidx = lat8[t * SP_P + i1];
+      x[i1] = sidx >= 0 ? (T)m[sidx] : (T)0;
     }
-    A[o] = acc;
+#pragma unroll
+    for (int k1 = 0; k1 < SP_H; k1++) {
+      C acc = {(T)0, (T)0};
+#pragma unroll
+      for (int i1 = 0; i1 < SP_P; i1++) {
+        const int w = (i1 * k1) & (SP_N - 1);
+        acc.x = fma(x[i1], (T)W16C[w], acc.x);
+        acc.y = fma(x[i1], -(T)W16S[w], acc.y);
+      }
+      A[t * SP_H + k1] = acc;
+    }
   }
   __syncthreads();
-  // second axis: B[i3][k2][k1]
-  for (int o = t; o < SP_P * SP_N * SP_H; o += SP_TB) {
-    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, i3 = o / (SP_H * SP_N);
-    C acc = {(T)0, (T)0};
+  if (t < SP_P * SP_H) {   // (i3, k1)
+    const int i3 = t / SP_H, k1 = t % SP_H;
+    C a[SP_P];
 #pragma unroll
-    for (int i2 = 0; i2 < SP_P; i2++) {
-      const int w = (i2 * k2) & (SP_N - 1);
-      cmac(acc, A[(i3 * SP_P + i2) * SP_H + k1], wr[w], wi[w]);
+    for (int i2 = 0; i2 < SP_P; i2++) a[i2] = A[(i3 * SP_P + i2) * SP_H + k1];
+#pragma unroll
+    for (int k2 = 0; k2 < SP_N; k2++) {
+      C acc = {(T)0, (T)0};
+#pragma unroll
+      for (int i2 = 0; i2 < SP_P; i2++) {
+        const int w = (i2 * k2) & (SP_N - 1);
+        cmac(acc, a[i2], (T)W16C[w], -(T)W16S[w]);
+      }
+      B[(i3 * SP_N + k2) * SP_H + k1] = acc;
     }
-    B[o] = acc;
   }
   __syncthreads();
-  // third axis, straight to the spectrum
-  C *__restrict__ out = S + q * SP_NF;
-  for (int o = t; o < SP_NF; o += SP_TB) {
-    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, k3 = o / (SP_H * SP_N);
-    C acc = {(T)0, (T)0};
+  if (t < SP_N * SP_H) {   // (k2, k1) = t
+    C b[SP_P];
 #pragma unroll
-    for (int i3 = 0; i3 < SP_P; i3++) {
-      const int w = (i3 * k3) & (SP_N - 1);
-      cmac(acc, B[(i3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    for (int i3 = 0; i3 < SP_P; i3++) b[i3] = B[i3 * SP_N * SP_H + t];
+    C *__restrict__ out = S + q * SP_NF + t;
+#pragma unroll
+    for (int k3 = 0; k3 < SP_N; k3++) {
+      C acc = {(T)0, (T)0};
+#pragma unroll
+      for (int i3 = 0; i3 < SP_P; i3++) {
+        const int w = (i3 * k3) & (SP_N - 1);
+        cmac(acc, b[i3], (T)W16C[w], -(T)W16S[w]);
+      }
+      out[k3 * SP_N * SP_H] = acc;
     }
-    out[o] = acc;
   }
 }
 
@@ -420,76 +444,82 @@ __global__ void __launch_bounds__(HM_TB, 1) k_spec_hadamard_mfma(const float2 *_
 // ---------------------------------------------------------------------------
 // 3. inverse: x = IFFT(U_t) on the [0, p)^3 corner, surface entries into pL
 // ---------------------------------------------------------------------------
+// One workgroup per target with far pairs: (k2, k1) over k3 (16 inputs read coalesced,
+// the p outputs i3 < p kept), then (i3, k1) over k2, then per lattice row (i3, i2) the
+// real output over the Hermitian half k1 (x = d0 + (-1)^i1 d_p + 2 Re sum_{k1=1}^{p-1}
+// d_k1 w^-(i1 k1)), added into the row's surface points of the partial local.
+constexpr int IV_TB = 192;
+
 template <class T>
-__global__ void __launch_bounds__(SP_TB) k_spec_inverse(const typename cplx_t<T>::type *__restrict__ U,
+__global__ void __launch_bounds__(IV_TB) k_spec_inverse(const typename cplx_t<T>::type *__restrict__ U,
                                                         const int32_t *__restrict__ ptr,
-                                                        const int32_t *__restrict__ sub, int Ns, double alpha,
+                                                        const int16_t *__restrict__ lat8, int Ns, double alpha,
                                                         double *__restrict__ pL) {
   using C = typename cplx_t<T>::type;
-  __shared__ C Cb[SP_P * SP_N * SP_H];
-  __shared__ C Db[SP_P * SP_P * SP_H];
-  __shared__ T wr[SP_N], wi[SP_N];
+  __shared__ C Cb[SP_P * SP_N * SP_H];   // [i3][k2][k1]
+  __shared__ C Db[SP_P * SP_P * SP_H];   // [i3][i2][k1]
   const int64_t q = blockIdx.x;
-  if (ptr[q + 1] == ptr[q]) return;          // no far pairs (ghosts included)
+  if (ptr[q + 1] == ptr[q]) return;      // no far pairs (ghosts included)
   const int t = threadIdx.x;
-  if (t < SP_N) { wr[t] = (T)c_w16[2 * t]; wi[t] = -(T)c_w16[2 * t + 1]; }   // conjugate twiddles
-  __syncthreads();
-  const C *__restrict__ in = U + q * SP_NF;
-  // third axis: Cb[i3][k2][k1], i3 < p
-  for (int o = t; o < SP_P * SP_N * SP_H; o += SP_TB) {
-    const int k1 = o % SP_H, k2 = (o / SP_H) % SP_N, i3 = o / (SP_H * SP_N);
-    C acc = {(T)0, (T)0};
+  if (t < SP_N * SP_H) {   // (k2, k1) = t
+    const C *__restrict__ in = U + q * SP_NF + t;
+    C u[SP_N];
 #pragma unroll
-    for (int k3 = 0; k3 < SP_N; k3++) {
-      const int w = (i3 * k3) & (SP_N - 1);
-      cmac(acc, in[(k3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    for (int k3 = 0; k3 < SP_N; k3++) u[k3] = in[k3 * SP_N * SP_H];
+#pragma unroll
+    for (int i3 = 0; i3 < SP_P; i3++) {
+      C acc = {(T)0, (T)0};
+#pragma unroll
+      for (int k3 = 0; k3 < SP_N; k3++) {
+        const int w = (i3 * k3) & (SP_N - 1);
+        cmac(acc, u[k3], (T)W16C[w], (T)W16S[w]);
+      }
+      Cb[i3 * SP_N * SP_H + t] = acc;
     }
-    Cb[o] = acc;
   }
   __syncthreads();
-  // second axis: Db[i3][i2][k1], i2 < p
-  for (int o = t; o < SP_P * SP_P * SP_H; o += SP_TB) {
-    const int k1 = o % SP_H, i2 = (o / SP_H) % SP_P, i3 = o / (SP_H * SP_P);
-    C acc = {(T)0, (T)0};
+  if (t < SP_P * SP_H) {   // (i3, k1)
+    const int i3 = t / SP_H, k1 = t % SP_H;
+    C c[SP_N];
 #pragma unroll
-    for (int k2 = 0; k2 < SP_N; k2++) {
-      const int w = (i2 * k2) & (SP_N - 1);
-      cmac(acc, Cb[(i3 * SP_N + k2) * SP_H + k1], wr[w], wi[w]);
+    for (int k2 = 0; k2 < SP_N; k2++) c[k2] = Cb[(i3 * SP_N + k2) * SP_H + k1];
+#pragma unroll
+    for (int i2 = 0; i2 < SP_P; i2++) {
+      C acc = {(T)0, (T)0};
+#pragma unroll
+      for (int k2 = 0; k2 < SP_N; k2++) {
+        const int w = (i2 * k2) & (SP_N - 1);
+        cmac(acc, c[k2], (T)W16C[w], (T)W16S[w]);
+      }
+      Db[(i3 * SP_P + i2) * SP_H + k1] = acc;
     }
-    Db[o] = acc;
   }
   __syncthreads();
-  // first axis (real output, Hermitian): at the surface points only
-  double *__restrict__ out = pL + q * Ns;
-  for (int j = t; j < Ns; j += SP_TB) {
-    const int s8 = sub[j];
-    const int i1 = s8 % SP_P, r = s8 / SP_P;
-    const C *d = &Db[r * SP_H];
-    T x = d[0].x + ((i1 & 1) ? -d[SP_P].x : d[SP_P].x);
-    T y = (T)0;
+  if (t < SP_P * SP_P) {   // row (i3, i2) = t
+    C d[SP_H];
 #pragma unroll
-    for (int k1 = 1; k1 < SP_P; k1++) {
-      const int w = (i1 * k1) & (SP_N - 1);
-      y = fma(d[k1].x, wr[w], fma(-d[k1].y, wi[w], y));
+    for (int k1 = 0; k1 < SP_H; k1++) d[k1] = Db[t * SP_H + k1];
+    double *__restrict__ out = pL + q * Ns;
+#pragma unroll
+    for (int i1 = 0; i1 < SP_P; i1++) {
+      const int sidx = lat8[t * SP_P + i1];
+      if (sidx < 0) continue;
+      const T x = d[0].x + ((i1 & 1) ? -d[SP_P].x : d[SP_P].x);
+      T y = (T)0;
+#pragma unroll
+      for (int k1 = 1; k1 < SP_P; k1++) {
+        const int w = (i1 * k1) & (SP_N - 1);
+        y = fma(d[k1].x, (T)W16C[w], fma(-d[k1].y, (T)W16S[w], y));
+      }
+      out[sidx] += alpha * (double)(x + (T)2 * y);
     }
-    out[j] += alpha * (double)(x + (T)2 * y);
   }
 }
 
 // host side -----------------------------------------------------------------
-static void upload_spectral_constants() {
-  double w[2 * SP_N];
-  for (int j = 0; j < SP_N; j++) {
-    w[2 * j] = std::cos(2 * M_PI * j / SP_N);
-    w[2 * j + 1] = -std::sin(2 * M_PI * j / SP_N);
-  }
-  FFMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w16), w, sizeof w));
-}
-
 void spectral_prepare(Plan &plan) {
   OpCache &o = *plan.ops;
   if (o.p != SP_P) fail(FFMM_ERR_ARGUMENT, "the spectral M2L is built for p = 8");
-  once_on_device(plan.ctx->device, 2, upload_spectral_constants);
   int64_t nmax = 1;
   for (int l = 2; l < plan.L; l++) nmax = std::max(nmax, plan.nl[l]);
   if (plan.precision == FFMM_PREC_MIXED) {
@@ -557,7 +587,7 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   hipStream_t s = ctx.stream;
   const int64_t lo = plan.lo[l], nl = plan.nl[l], ng = nl / NCHILD;
   if (plan.lists[l].np == 0) return;
-  k_spec_forward<T><<<(unsigned)nl, SP_TB, 0, s>>>(plan.Mp.p, o.Ns, o.sub8.p, lo, nl, plan.has_source.p, S);
+  k_spec_forward<T><<<(unsigned)nl, FW_TB, 0, s>>>(plan.Mp.p, o.Ns, o.lat8.p, lo, nl, plan.has_source.p, S);
   FFMM_LAUNCH_CHECK();
   const int gpw = 4;
   const int64_t gpb = (HD_TB / 64) * gpw;
@@ -578,7 +608,7 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
     FFMM_LAUNCH_CHECK();
   }
   const double alpha = (1.0 / plan.lam[l]) / (double)(SP_N * SP_N * SP_N);
-  k_spec_inverse<T><<<(unsigned)nl, SP_TB, 0, s>>>(U, plan.lists[l].ptr.p, o.sub8.p, o.Ns, alpha,
+  k_spec_inverse<T><<<(unsigned)nl, IV_TB, 0, s>>>(U, plan.lists[l].ptr.p, o.lat8.p, o.Ns, alpha,
                                                   plan.Pl.p + lo * o.Ns);
   FFMM_LAUNCH_CHECK();
 }
