@@ -758,6 +758,16 @@ int ffmm_plan_download_tree(ffmm_plan plan, int32_t *cell, int32_t *level, int32
   });
 }
 
+int ffmm_plan_perm(ffmm_plan plan, int32_t *perm_dev) {
+  return guard([&] {
+    if (!plan || !perm_dev) fail(FFMM_ERR_ARGUMENT, "null argument");
+    Plan &p = plan->c;
+    set_device(*p.ctx);
+    FFMM_HIP(hipMemcpyAsync(perm_dev, p.perm.p, sizeof(int32_t) * p.N, hipMemcpyDeviceToDevice, p.ctx->stream));
+    p.ctx->sync();
+  });
+}
+
 int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *src, int32_t *far) {
   return guard([&] {
     if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");

@@ -56,6 +56,7 @@ SIGNATURES = {
     "ffmm_plan_sizes": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i64), _vp, _vp, _vp,
                                _vp, _vp]),
     "ffmm_plan_download_tree": (_i32, [_vp] + [_vp] * 14),
+    "ffmm_plan_perm": (_i32, [_vp, _vp]),
     "ffmm_plan_download_lists": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "ffmm_build_expansions": (_i32, [_vp, _vp, _i32]),
     "ffmm_build_expansions_nd": (_i32, [_vp, _vp, _i32, _i32]),

@@ -470,10 +470,22 @@ class DevicePlan:
     """``DevicePlan(method, xs; ppl=50, bbox, cache, max_levels=30)``: the device tree,
     its lists, the operators' device copies and the work buffers
     (src/device_passes.jl:98-134).  ``precision`` is ``"f64"`` or ``"mixed"``; ``m2l`` is
-    ``"dense"`` (class GEMMs) or ``"spectral"`` (batched FFT)."""
+    ``"dense"`` (class GEMMs) or ``"spectral"`` (batched FFT).
+
+    ``xs`` may be a tuple of point sets, one per group (``DevicePlan(method, groups)``,
+    src/device_passes.jl:113-134): one tree over all of them; densities are then given per
+    group (``None``: targets only) and outputs come back per group."""
 
     def __init__(self, method, xs, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
                  max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
+        if isinstance(xs, tuple):
+            if len(xs) == 0:
+                raise ArgumentError("no groups")
+            parts = [_dev_f64(g, device, cols=3) for g in xs]
+            self.group_sizes = [int(g.shape[0]) for g in parts]
+            xs = parts[0] if len(parts) == 1 else _torch().cat(parts, 0)
+        else:
+            self.group_sizes = None
         cache = OperatorCache(method) if cache is None else cache
         if cache.method is not method:
             raise ArgumentError("the cache was built for a different method")
@@ -505,6 +517,10 @@ class DevicePlan:
         self._h = h
         self.N = n
         self.ppl = int(ppl)
+        if self.group_sizes is None:
+            self.group_sizes = [n]
+        self.group_offsets = np.concatenate([[0], np.cumsum(self.group_sizes)]).astype(np.int64)
+        self._gperm = None
         self._read_sizes()
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
             # the per-width Yukawa operators of the levels with translations (2 .. L-1)
@@ -572,11 +588,70 @@ class DevicePlan:
         names = ["tree", "lists", "upward", "downward", "m2l", "eval_sources", "eval_targets", "gather", "hadamard"]
         return dict(zip(names, ms[:]))
 
+    @property
+    def ngroups(self) -> int:
+        return len(self.group_sizes)
+
+    def _group_positions(self):
+        """Per group, the tree-order positions of its points, in tree order (the
+        reference's per-group gperm, src/tree/device_builder.jl:86-116)."""
+        if self._gperm is None:
+            torch = _torch()
+            perm = torch.empty(self.N, dtype=torch.int32, device=f"cuda:{self.device}")
+            call("ffmm_plan_perm", self._h, _ptr(perm))
+            perm = perm.to(torch.int64)
+            self._gperm = [torch.nonzero((perm >= a) & (perm < b)).flatten()
+                           for a, b in zip(self.group_offsets[:-1], self.group_offsets[1:])]
+        return self._gperm
+
+    def _join_densities(self, tau, sorted: bool):
+        """One density array over all points from the groups' (``None``: targets only,
+        zero densities); a Stokes group of stokeslets alone (nd = 3) joins others with
+        stresslets (nd = 9) with a zero stresslet part."""
+        torch = _torch()
+        G = self.ngroups
+        if not isinstance(tau, (tuple, list)):
+            if G != 1:
+                raise DimensionMismatch(f"expected {G} groups of sources and densities")
+            return _dev_f64(tau, self.device)
+        if len(tau) != G:
+            raise DimensionMismatch(f"expected {G} groups of sources and densities")
+        parts = [None if t is None else _dev_f64(t, self.device) for t in tau]
+        nds = {1 if t.ndim == 1 else int(t.shape[1]) for t in parts if t is not None}
+        if not nds:
+            nds = {3 if isinstance(self.method, StokesHarmonics) else 1}
+        nd = 9 if nds == {3, 9} else max(nds)
+        if len(nds) > 1 and nds != {3, 9}:
+            raise ArgumentError("the groups' densities do not combine")
+        for g, t in enumerate(parts):
+            if t is not None and t.shape[0] != self.group_sizes[g]:
+                raise DimensionMismatch(f"expected {self.group_sizes[g]} entries in group {g + 1}")
+        shape = (self.N,) if nd == 1 else (self.N, nd)
+        out = torch.zeros(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        pos = self._group_positions() if sorted else None
+        for g, t in enumerate(parts):
+            if t is None:
+                continue
+            if t.ndim == 2 and t.shape[1] == 3 and nd == 9:
+                t = torch.cat([t, torch.zeros((t.shape[0], 6), dtype=t.dtype, device=t.device)], 1)
+            if sorted:
+                out[pos[g]] = t
+            else:
+                a, b = self.group_offsets[g], self.group_offsets[g + 1]
+                out[a:b] = t
+        torch.cuda.synchronize(self.device)
+        return out
+
+    def _split(self, out, sorted: bool = False):
+        """The per-group outputs (original order within each group)."""
+        return tuple(out[a:b] for a, b in zip(self.group_offsets[:-1], self.group_offsets[1:]))
+
     # -- the passes (device tensors in and out) --
     def build_expansions(self, tau, sorted: bool = False):
         """Densities: N charges, or for Stokes (N, 3) stokeslets or (N, 9) rows
-        (f, g, nu) of a stokeslet f plus a stresslet (g, nu)."""
-        t = _dev_f64(tau, self.device)
+        (f, g, nu) of a stokeslet f plus a stresslet (g, nu); with groups, a tuple of
+        them, one per group (``None`` for a group of targets only)."""
+        t = self._join_densities(tau, sorted and self.ngroups > 1)
         if t.shape[0] != self.N or t.ndim > 2:
             raise DimensionMismatch(f"expected {self.N} entries in group 1")
         nd = 1 if t.ndim == 1 else t.shape[1]
@@ -613,8 +688,37 @@ class DevicePlan:
         return out
 
     def matvec(self, tau, output=Potential, sorted: bool = False, out=None):
+        """``device_matvec!``; with groups, ``output`` may be a tuple of one output type
+        per group (``None``: not wanted) and the result is a tuple of per-group outputs
+        (src/device_passes.jl:619-640)."""
+        if self.ngroups == 1:
+            if isinstance(output, tuple):
+                raise ArgumentError("several outputs for one group")
+            self.build_expansions(tau, sorted)
+            return self.evaluate_at_sources(output, out)
+        if out is not None:
+            raise ArgumentError("several groups: outputs are returned per group")
+        outputs = output if isinstance(output, tuple) else (output,) * self.ngroups
+        if len(outputs) != self.ngroups:
+            raise DimensionMismatch(f"expected {self.ngroups} outputs")
         self.build_expansions(tau, sorted)
-        return self.evaluate_at_sources(output, out)
+        res, cache = [], {}
+        for g, o in enumerate(outputs):
+            if o is None:
+                res.append(None)
+                continue
+            if o not in cache:
+                cache[o] = self._split(self.evaluate_at_sources(o))
+            v = cache[o][g]
+            if sorted:
+                a = self.group_offsets[g]
+                torch = _torch()
+                perm = torch.empty(self.N, dtype=torch.int32, device=v.device)
+                call("ffmm_plan_perm", self._h, _ptr(perm))
+                # the group's outputs in its tree order
+                v = v[(perm[self._group_positions()[g]].to(torch.int64) - a)]
+            res.append(v)
+        return tuple(res)
 
 
 def device_build_expansions_(plan: DevicePlan, ds, sorted: bool = False) -> DevicePlan:
@@ -639,9 +743,7 @@ class FMM:
 
     def __init__(self, method, sources, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
                  max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
-        if isinstance(sources, tuple):
-            if len(sources) != 1:
-                raise ArgumentError("this build takes one group of sources")
+        if isinstance(sources, tuple) and len(sources) == 1:
             sources = sources[0]
         self.method = method
         self.plan = DevicePlan(method, sources, ppl=ppl, bbox=bbox, cache=cache, max_levels=max_levels,
@@ -669,7 +771,26 @@ def build_expansions_(state: FMMState, f: FMM, sources, tau) -> FMMState:
     """``build_expansions!(state, fmm, sources, tau)`` (src/fmm.jl:264-298)."""
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
+    G = f.plan.ngroups
+    if G > 1:
+        # one tuple entry per group (src/fmm.jl:270-281)
+        if not isinstance(tau, (tuple, list)) or len(tau) != G or (
+                sources is not None and (not isinstance(sources, (tuple, list)) or len(sources) != G)):
+            raise DimensionMismatch(f"expected {G} groups of sources and densities")
+        for g in range(G):
+            n = f.plan.group_sizes[g]
+            if sources is not None and len(sources[g]) != n:
+                raise DimensionMismatch(f"expected {n} sources in group {g + 1}")
+            if tau[g] is not None and len(tau[g]) != n:
+                raise DimensionMismatch(f"expected {n} densities in group {g + 1}")
+        f.plan.build_expansions(tuple(tau))
+        state.tau = tau
+        return state
     n = f.plan.N
+    if isinstance(sources, tuple) and len(sources) == 1:
+        sources = sources[0]
+    if isinstance(tau, tuple) and len(tau) == 1:
+        tau = tau[0]
     if sources is not None and len(sources) != n:
         raise DimensionMismatch(f"expected {n} sources in group 1")
     tau = np.asarray(tau) if not hasattr(tau, "data_ptr") else tau
@@ -687,12 +808,15 @@ def build_expansions(f: FMM, sources, tau) -> FMMState:
 
 def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -> np.ndarray:
     """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:418-446): the
-    output at every source in the original order."""
-    if g != 1:
-        raise ArgumentError(f"group {g} of 1")
+    output at every source of group g (1-based) in the original order."""
+    G = f.plan.ngroups
+    if not 1 <= g <= G:
+        raise ArgumentError(f"group {g} of {G}")
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
-    return f.plan.evaluate_at_sources(output).cpu().numpy()
+    out = f.plan.evaluate_at_sources(output)
+    a, b = f.plan.group_offsets[g - 1], f.plan.group_offsets[g]
+    return out[a:b].cpu().numpy()
 
 
 def evaluate(f: FMM, state: FMMState, targets, output=Potential) -> np.ndarray:

@@ -158,6 +158,11 @@ int ffmm_plan_download_tree(ffmm_plan plan, int32_t *cell, int32_t *level, int32
                             uint8_t *has_source, int32_t *colleagues, int32_t *perm, int32_t *point_node,
                             double *sorted_points);
 
+/* The tree's point permutation into a device buffer of N int32 (perm[j]: the input index
+ * of the j-th point in tree order), the device tree's perm (src/tree/device_builder.jl:
+ * 21-48); the groups' per-group orders (gperm, 86-116) follow from it and the group sizes. */
+int ffmm_plan_perm(ffmm_plan plan, int32_t *perm_dev);
+
 /* Download one level's far pairs, targets in node order: ptr (count + 1, relative to the
  * level's first node), src (global node), far (0-based far offset index), as
  * _pair_fill! writes them (src/device_lists.jl:286-312; src/tree/lists.jl:96-124). */
