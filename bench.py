@@ -121,7 +121,7 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
                (False, False): ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=4)}
     caches = {k: ffmm.OperatorCache(m) for k, m in methods.items()}
     form = {(True, True): ("mixed", "spectral"), (True, False): ("f64", "dense"),
-            (False, True): ("f64", "dense"), (False, False): ("f64", "dense")}
+            (False, True): ("mixed", "dense"), (False, False): ("f64", "dense")}
     reqs = serving_stream(nreq, nmin, nmax, rate)
     total = float(sum(len(r["pts"]) for r in reqs))
     log(f"[bench] serving: {nreq} requests, {total:.3e} points, {nslots} slots")
@@ -195,6 +195,51 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
                     "requests' forms and in the plain form (FP64, dense class M2L)"}
 
 
+def stokes_bench(n: int, ppl: int, nerr: int):
+    """The Stokes surface workload: n points on 64 ellipsoids, stokeslets + stresslets
+    with the surface normals, StokesHarmonics p=10, velocity at the sources; the mixed
+    form (FP32 class M2L) and the FP64 form, one warm-up and one timed solve each; the
+    error against the device FP64 direct sum at sampled sources."""
+    import ffmm
+    from ffmm import workloads as W
+    torch = __import__("torch")
+    pts, nrm = W.ellipsoids(n, seed=4)
+    dens = W.stokes_densities(nrm, seed=5)
+    m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=10)
+    cache = ffmm.OperatorCache(m)
+    box = (np.zeros(3), 1.0)
+    dev = "cuda:0"
+    xs_d = torch.from_numpy(pts).to(dev)
+    d_d = torch.from_numpy(dens).to(dev)
+    u_d = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    res = {"workload": f"StokesHarmonics p=10, {n} points on 64 ellipsoids, stokeslets + stresslets, "
+                       f"velocity at the sources, ppl={ppl}"}
+    rows = np.sort(np.random.default_rng(9).choice(n, nerr, replace=False))
+    ud = None
+    for prec in ("mixed", "f64"):
+        def solve():
+            plan = ffmm.DevicePlan(m, xs_d, ppl=ppl, bbox=box, cache=cache, precision=prec)
+            plan.matvec(d_d, ffmm.Velocity, out=u_d)
+            return plan
+        del_plan = solve()
+        del del_plan
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan = solve()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t)
+        if ud is None:
+            ud = ffmm.direct_self(ffmm.Stokeslet3D(), pts, dens, rows=rows)
+        u = u_d.cpu().numpy()[rows]
+        err = float(np.linalg.norm(u - ud) / np.linalg.norm(ud))
+        res[prec] = {"ms_per_solve": ms, "points_per_s": n / (ms * 1e-3), "rel_l2_velocity": err,
+                     "stages_ms": plan.stage_times(), "levels": plan.nlevels}
+        log(f"[bench] stokes {prec}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} points/s, rel L2 {err:.2e}")
+        del plan
+    res["mixed_over_f64"] = res["f64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +259,8 @@ def main():
     ap.add_argument("--serve-nmax", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--stokes-n", type=int, default=2_000_000,
+                    help="points of the Stokes surface leg (0: skip the leg)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="before the steps, copy a buffer of this size once on the device (a kernel of known "
                          "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile)")
@@ -371,6 +418,10 @@ def main():
                                 args.serve_slots)
         log("[bench] serving: " + json.dumps(serving))
 
+    stokes = None
+    if args.stokes_n > 0:
+        stokes = stokes_bench(args.stokes_n, args.ppl, min(args.nerr, 1000))
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle on {args.cpu_sample} points")
@@ -389,7 +440,8 @@ def main():
             "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
             "tree": {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
                      "leaves": int(plan.level_leaves.sum())},
-            "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "plain_f64": plain}
+            "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "plain_f64": plain,
+            "stokes_surface": stokes}
     print(json.dumps(line), flush=True)
 
 

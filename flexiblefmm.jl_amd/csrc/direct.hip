@@ -91,4 +91,78 @@ void direct(Context &ctx, const double *src, const double *tau, int64_t ns, cons
   ctx.sync();   // the partial sums are freed on return
 }
 
+// The Stokes sums: velocity of stokeslets f (nd = 3) and, with nd = 9, stresslets of
+// strength g and normal nu (rows [f, g, nu]):
+//   u = 1/(8 pi) (f / r + (d . f) d / r^3) - 6/(8 pi) (d . g)(d . nu) d / r^5,  d = t - s
+// (direct! for Stokeslet3D, src/kernels.jl:463-487; the stresslet as in
+// src/operators/harmonics_stokes.jl:197-217).  The sources' rows stream through LDS.
+constexpr double C8PI = 0.039788735772973836;   // 1 / (8 pi)
+
+template <int ND>
+__global__ void __launch_bounds__(DT) k_direct_stokes(const double *__restrict__ src, const double *__restrict__ dens,
+                                                      int64_t ns, const double *__restrict__ tgt, int64_t nt,
+                                                      const int64_t *__restrict__ skip, int64_t chunk,
+                                                      double *__restrict__ part) {
+  __shared__ double xs[DT][3 + ND];
+  const int64_t j = (int64_t)blockIdx.x * DT + threadIdx.x;
+  const int64_t jj = j < nt ? j : nt - 1;
+  const double x = tgt[3 * jj], y = tgt[3 * jj + 1], z = tgt[3 * jj + 2];
+  const int64_t sk = skip ? skip[jj] : -1;
+  const int64_t ib = (int64_t)blockIdx.y * chunk, ie = std::min<int64_t>(ns, ib + chunk);
+  double acc[3] = {0, 0, 0};
+  for (int64_t i0 = ib; i0 < ie; i0 += DT) {
+    const int64_t i = i0 + threadIdx.x;
+    if (i < ie) {
+      for (int c = 0; c < 3; c++) xs[threadIdx.x][c] = src[3 * i + c];
+      for (int c = 0; c < ND; c++) xs[threadIdx.x][3 + c] = dens[ND * i + c];
+    }
+    __syncthreads();
+    const int cnt = (int)std::min<int64_t>(DT, ie - i0);
+    for (int q = 0; q < cnt; q++) {
+      const double *v = xs[q];
+      const double d0 = x - v[0], d1 = y - v[1], d2 = z - v[2];
+      const double ir = rsqrt_f64(fma(d2, d2, fma(d1, d1, d0 * d0)));
+      const double a = C8PI * ir;
+      const double b = a * ir * ir * (d0 * v[3] + d1 * v[4] + d2 * v[5]);
+      double u0 = a * v[3] + b * d0, u1 = a * v[4] + b * d1, u2 = a * v[5] + b * d2;
+      if (ND == 9) {
+        const double ir2 = ir * ir;
+        const double e = -(6 * C8PI) * (d0 * v[6] + d1 * v[7] + d2 * v[8]) * (d0 * v[9] + d1 * v[10] + d2 * v[11]) *
+                         (ir2 * ir2 * ir);
+        u0 += e * d0;
+        u1 += e * d1;
+        u2 += e * d2;
+      }
+      const bool keep = i0 + q != sk;
+      acc[0] += keep ? u0 : 0.0;
+      acc[1] += keep ? u1 : 0.0;
+      acc[2] += keep ? u2 : 0.0;
+    }
+    __syncthreads();
+  }
+  if (j >= nt) return;
+  double *p = part + ((int64_t)blockIdx.y * nt + j) * 4;
+  p[0] = 0.0; p[1] = acc[0]; p[2] = acc[1]; p[3] = acc[2];
+}
+
+void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
+                   int64_t nt, const int64_t *skip, double *out) {
+  if (nt <= 0) return;
+  const unsigned gx = nblocks(nt, DT);
+  int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / gx, (ns + 4095) / 4096));
+  nchunk = std::min<int64_t>(nchunk, 65535);
+  const int64_t chunk = std::max<int64_t>(1, (ns + nchunk - 1) / nchunk);
+  nchunk = std::max<int64_t>(1, (ns + chunk - 1) / chunk);
+  DBuf<double> part;
+  part.resize((size_t)nchunk * nt * 4);
+  dim3 grid(gx, (unsigned)nchunk);
+  if (nd == 3) k_direct_stokes<3><<<grid, DT, 0, ctx.stream>>>(src, dens, ns, tgt, nt, skip, chunk, part.p);
+  else k_direct_stokes<9><<<grid, DT, 0, ctx.stream>>>(src, dens, ns, tgt, nt, skip, chunk, part.p);
+  FFMM_LAUNCH_CHECK();
+  // the velocity is the three gradient slots of the reduction
+  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part.p, nt, (int)nchunk, FFMM_GRADIENT, out);
+  FFMM_LAUNCH_CHECK();
+  ctx.sync();
+}
+
 }  // namespace ffmm

@@ -216,6 +216,19 @@ int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t 
   });
 }
 
+int ffmm_direct_stokes(ffmm_context ctx, const double *src, const double *dens, int nd, int64_t ns,
+                       const double *tgt, int64_t nt, const int64_t *skip, double *out) {
+  return guard([&] {
+    if (!ctx) fail(FFMM_ERR_ARGUMENT, "null context");
+    if (ns < 0 || nt < 0) fail(FFMM_ERR_DIMENSION, "negative sizes");
+    if (nd != 3 && nd != 9) fail(FFMM_ERR_ARGUMENT, "Stokes densities are stokeslets (3) or stokeslets + stresslets (9)");
+    if (nt == 0) return;
+    if (!tgt || !out || (ns > 0 && (!src || !dens))) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(ctx->c);
+    direct_stokes(ctx->c, src, dens, nd, ns, tgt, nt, skip, out);
+  });
+}
+
 int ffmm_opcache_create(ffmm_context ctx, int p, int pc, double d, ffmm_opcache *out) {
   return guard([&] {
     if (!ctx || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
@@ -494,6 +507,8 @@ int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const doub
       lv->L2L.upload(Ld.data(), Ld.size(), s);
     }
     lv->reps.upload(reps, (size_t)o.nclass * N * N, s);
+    std::vector<float> r32(reps, reps + (size_t)o.nclass * N * N);   // the mixed form's class GEMMs
+    lv->reps32.upload(r32.data(), r32.size(), s);
     ctx.sync();
     std::lock_guard<std::mutex> lk(o.hmu);
     for (auto &h : o.hlev)
@@ -525,7 +540,6 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   if (m2l != FFMM_M2L_DENSE && m2l != FFMM_M2L_SPECTRAL) fail(FFMM_ERR_ARGUMENT, "unknown M2L form");
   if (o.method == METHOD_HARMONIC) {
     if (m2l != FFMM_M2L_DENSE) fail(FFMM_ERR_ARGUMENT, "the harmonic methods take the dense class M2L");
-    if (precision != FFMM_PREC_F64) fail(FFMM_ERR_ARGUMENT, "the harmonic methods run in FP64");
     if (o.kernel != FFMM_KERNEL_YUKAWA && !o.find_width(o.w_ref))
       fail(FFMM_ERR_STATE, "the cache holds no operators of the reference width");
   }

@@ -212,6 +212,7 @@ struct HarmonicLevel {
   double w = 0.0;
   DBuf<double> M2M, L2L;   // NB x 8NB, 8NB x NB (NB = nb N; the dense Stokes forms for nb = 4)
   DBuf<double> reps;       // nclass x (N x N)
+  DBuf<float> reps32;      // the same in FP32 (the mixed form's M2L)
 };
 
 // device pointers of the recurrence tables (HarmonicTables, src/operators/harmonics.jl:163-187)
@@ -374,6 +375,8 @@ void m2l_spectral_level(Plan &plan, int l);
 // ---- direct sums (direct.hip) ----
 void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
             const int64_t *skip, double *out, int output);
+void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
+                   int64_t nt, const int64_t *skip, double *out);
 
 // ---- evaluation (evaluate.hip) ----
 void evaluate_at_sources(Plan &plan, double *out, int output);

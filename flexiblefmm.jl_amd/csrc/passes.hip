@@ -384,7 +384,8 @@ void m2l_dense_level(Plan &plan, int l) {
   // m2l_scale of the level: (w_l / w_ref)^(-1) for the homogeneous kernels, 1 for Yukawa
   const double alpha = harm ? plan.alpha[l] : 1.0 / plan.lam[l];
   const double *reps = harm ? plan.hl[l]->reps.p : o.class_ops.p;
-  const bool f32 = !harm && plan.precision == FFMM_PREC_MIXED;
+  const float *reps32 = harm ? plan.hl[l]->reps32.p : o.class_ops32.p;
+  const bool f32 = plan.precision == FFMM_PREC_MIXED;
   for (int c = 0; c < o.nclass; c++) {
     const int64_t pa = li.class_pair[c], pb = li.class_pair[c + 1];
     const int64_t r0 = li.class_run[c], nr = li.class_run[c + 1] - r0;
@@ -395,7 +396,7 @@ void m2l_dense_level(Plan &plan, int l) {
         k_class_gather<float><<<gg, TB, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a, li.cfar.p + a, npk,
                                                 o.idx.p, o.sgn.p);
         FFMM_LAUNCH_CHECK();
-        gemm32(ctx, N, (int)(nb * npk), N, o.class_ops32.p + (size_t)c * N * N, N, plan.Gc32.p, N, plan.Yc32.p, N);
+        gemm32(ctx, N, (int)(nb * npk), N, reps32 + (size_t)c * N * N, N, plan.Gc32.p, N, plan.Yc32.p, N);
         k_class_scatter<float><<<gs, TB, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0, nr, li.cfar.p, a,
                                                  npk, o.inv.p, o.sgn_t.p, o.offs.p, alpha);
       } else {

@@ -237,8 +237,10 @@ def _ptr(t) -> ctypes.c_void_p:
 def direct(K, sources, tau, targets, output=Potential, skip=None, device: int = 0) -> np.ndarray:
     """``direct(K, sources, tau, targets; output)``: the O(N M) sum at the targets,
     excluding source ``skip[j]`` at target ``j`` when ``skip`` is given."""
+    if isinstance(K, Stokeslet3D):
+        return _direct_stokes(sources, tau, targets, skip, device)
     if not isinstance(K, Laplace3D):
-        raise ArgumentError("the device direct sum takes Laplace3D")
+        raise ArgumentError("the device direct sum takes Laplace3D or Stokeslet3D")
     o = _output(output)
     torch = _torch()
     s = _dev_f64(sources, device, cols=3)
@@ -255,6 +257,26 @@ def direct(K, sources, tau, targets, output=Potential, skip=None, device: int = 
             raise DimensionMismatch("skip and targets differ in length")
     call("ffmm_direct", context(device), _ptr(s), _ptr(t), s.shape[0], _ptr(x), nt,
          None if sk is None else _ptr(sk), _ptr(out), o.code)
+    return out.cpu().numpy()
+
+
+def _direct_stokes(sources, dens, targets, skip, device: int) -> np.ndarray:
+    """The Stokes velocity at the targets: dens (N, 3) stokeslets or (N, 9) rows [f, g, nu]."""
+    torch = _torch()
+    s = _dev_f64(sources, device, cols=3)
+    d = _dev_f64(dens, device)
+    x = _dev_f64(targets, device, cols=3)
+    if d.ndim != 2 or d.shape[1] not in (3, 9) or d.shape[0] != s.shape[0]:
+        raise DimensionMismatch("expected (N, 3) stokeslets or (N, 9) rows [f, g, nu]")
+    nt = x.shape[0]
+    out = torch.empty((nt, 3), dtype=torch.float64, device=f"cuda:{device}")
+    sk = None
+    if skip is not None:
+        sk = torch.as_tensor(np.asarray(skip, dtype=np.int64)).to(f"cuda:{device}")
+        if sk.shape[0] != nt:
+            raise DimensionMismatch("skip and targets differ in length")
+    call("ffmm_direct_stokes", context(device), _ptr(s), _ptr(d), int(d.shape[1]), s.shape[0], _ptr(x), nt,
+         None if sk is None else _ptr(sk), _ptr(out))
     return out.cpu().numpy()
 
 

@@ -64,8 +64,8 @@ class Server:
     def submit(self, cache: OperatorCache, points, dens, targets=None, *, geometry: int = -1, ppl: int = 64,
                bbox=None, output=Potential, precision: str = "f64", m2l: str = "dense", max_levels: int = 0) -> int:
         m = cache.method
-        if _is_harmonic(m) and (precision != "f64" or m2l != "dense"):
-            raise ArgumentError("the harmonic methods run in FP64 with the dense class M2L")
+        if _is_harmonic(m) and m2l != "dense":
+            raise ArgumentError("the harmonic methods take the dense class M2L")
         pts = _host(points, 3)
         d = _host(dens)
         n = len(pts)

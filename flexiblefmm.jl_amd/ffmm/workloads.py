@@ -6,7 +6,12 @@
   the unit cube;
 * ``uniform``: N points uniform in [0, 1]^3 (test/testutils.jl:52);
 * ``sphere``, ``mixed``: the reference's sphere and sphere + cluster + background
-  layouts (test/testutils.jl:57-68), the latter forcing level restriction and ghosts.
+  layouts (test/testutils.jl:57-68), the latter forcing level restriction and ghosts;
+* ``ellipsoids``: N points on the surfaces of 64 ellipsoids (semi-axes uniform in
+  [0.02, 0.08], random rotations, centres uniform in [0.15, 0.85]^3), area-uniform on
+  each, with their outward unit normals (the Stokes surface workload);
+* ``stokes_densities``: stokeslet strengths N(0,1)^3 plus stresslets (strength
+  N(0,1)^3, the given normals), as the (N, 9) rows [f, g, nu].
 """
 from __future__ import annotations
 
@@ -44,3 +49,45 @@ def mixed(n: int, seed: int = 0) -> np.ndarray:
 
 def charges(n: int, seed: int = 1) -> np.ndarray:
     return np.random.default_rng(seed).standard_normal(n)
+
+
+def _rotation(rng) -> np.ndarray:
+    """A Haar-random rotation: QR of a Gaussian matrix with the signs of R's diagonal."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoids(n: int, seed: int = 0, nell: int = 64):
+    """(points, normals): n points, split evenly over nell ellipsoids.  On an ellipsoid
+    x = c + R diag(a) u (u on the unit sphere) the area element is proportional to
+    |diag(1/a) u|, so directions u uniform on the sphere are kept with probability
+    min(a) |diag(1/a) u| (at most 1): the kept points are area-uniform."""
+    rng = np.random.default_rng(seed)
+    centers = 0.15 + 0.7 * rng.random((nell, 3))
+    axes = rng.uniform(0.02, 0.08, (nell, 3))
+    counts = np.full(nell, n // nell)
+    counts[: n - counts.sum()] += 1
+    pts, nrm = [], []
+    for e in range(nell):
+        R, a, m = _rotation(rng), axes[e], int(counts[e])
+        got = []
+        while sum(len(g) for g in got) < m:
+            u = rng.standard_normal((2 * m + 16, 3))
+            u /= np.linalg.norm(u, axis=1, keepdims=True)
+            w = np.linalg.norm(u / a[None, :], axis=1) * a.min()
+            got.append(u[rng.random(len(u)) < w])
+        u = np.vstack(got)[:m]
+        nv = u / a[None, :]
+        nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+        pts.append(centers[e] + (u * a[None, :]) @ R.T)
+        nrm.append(nv @ R.T)
+    return np.clip(np.vstack(pts), 0.0, 1.0), np.vstack(nrm)
+
+
+def stokes_densities(normals: np.ndarray, seed: int = 2) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    n = len(normals)
+    return np.hstack([rng.standard_normal((n, 3)), rng.standard_normal((n, 3)), normals])

@@ -71,6 +71,13 @@ int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], do
 int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
                 const int64_t *skip, double *out, int output);
 
+/* FP64 direct Stokes velocity: direct! for Stokeslet3D (src/kernels.jl:463-487) with the
+ * stresslet of src/operators/harmonics_stokes.jl:197-217.  dens: nd x ns, point major,
+ * nd = 3 (stokeslets f) or 9 (rows [f, g, nu]: stokeslet f, stresslet strength g and
+ * normal nu); out: 3 x nt; skip as for ffmm_direct. */
+int ffmm_direct_stokes(ffmm_context ctx, const double *src, const double *dens, int nd, int64_t ns,
+                       const double *tgt, int64_t nt, const int64_t *skip, double *out);
+
 /* The device copies of the operators of one method, shared by every plan built with
  * the cache: OperatorCache.device (src/operators/cache.jl:196-213,
  * src/device_passes.jl:136-146).  The method is CubeKernelIndependent{3}(Laplace3D(); p, pc, d)

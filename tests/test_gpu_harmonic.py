@@ -110,8 +110,6 @@ def test_harmonic_sorted_densities_linearity_and_errors():
     assert np.array_equal(us, ua)
     with pytest.raises(ffmm.ArgumentError, match="dense class M2L"):
         ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX, m2l="spectral")
-    with pytest.raises(ffmm.ArgumentError, match="FP64"):
-        ffmm.DevicePlan(m, pts, ppl=30, bbox=BOX, precision="mixed")
     ms = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=4)
     ps = ffmm.DevicePlan(ms, pts, ppl=30, bbox=BOX)
     with pytest.raises(ffmm.ArgumentError, match="stokeslets"):
@@ -119,3 +117,53 @@ def test_harmonic_sorted_densities_linearity_and_errors():
     ps.build_expansions(rng.standard_normal((n, 3)))
     with pytest.raises(ffmm.ArgumentError, match="velocity"):
         ps.evaluate_at_sources(ffmm.Gradient)
+
+
+@pytest.mark.parametrize("kind", ["yukawa", "stokes"])
+def test_harmonic_mixed_form(kind):
+    """The mixed form of the harmonic methods: the class M2L as FP32 gathers, sgemm (FP32
+    MFMA) and scatters into the FP64 expansions; S2M, M2M, L2L and the evaluation FP64.
+    Against the FP64 plan on the same tree, and against direct sums."""
+    import ffmm
+    rng = np.random.default_rng(14)
+    n = 5000
+    pts = mixed(rng, n)
+    if kind == "yukawa":
+        m = ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10)
+        dens, out = rng.standard_normal(n), ffmm.PotentialGradient
+    else:
+        m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=10)
+        dens = rng.standard_normal((n, 9))
+        dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+        out = ffmm.Velocity
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] mixed form {kind} p=10", flush=True)
+    p64 = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    p32 = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache, precision="mixed")
+    u64 = p64.matvec(dens, out).cpu().numpy()
+    u32 = p32.matvec(dens, out).cpu().numpy()
+    assert rel(u32, u64) < 1e-6
+    rows = rng.choice(n, 300, replace=False)
+    if kind == "stokes":
+        ud = HO.direct("stokes", pts, dens, pts[rows], skip=rows)
+        assert rel(u32[rows], ud) < 1e-4
+    else:
+        ud = HO.direct("yukawa", pts, dens, pts[rows], skip=rows, kappa=5.0)
+        assert rel(u32[rows, 0], np.ravel(ud)) < 2e-5     # the truncation error of p = 10
+
+
+@pytest.mark.parametrize("nd", [3, 9])
+def test_device_stokes_direct_sum(nd):
+    """ffmm_direct_stokes (direct! for Stokeslet3D, stresslets as harmonics_stokes.jl)
+    against the oracle's FP64 sum, with and without the self term."""
+    import ffmm
+    rng = np.random.default_rng(15)
+    s, t = rng.random((6000, 3)), rng.random((400, 3))
+    d = rng.standard_normal((6000, nd))
+    if nd == 9:
+        d[:, 6:] /= np.linalg.norm(d[:, 6:], axis=1)[:, None]
+    K = ffmm.Stokeslet3D()
+    print(f"[harmonic] stokes direct nd={nd}", flush=True)
+    assert rel(ffmm.direct(K, s, d, t), HO.direct("stokes", s, d, t)) < 1e-13
+    rows = np.arange(0, 6000, 17)
+    assert rel(ffmm.direct_self(K, s, d, rows=rows), HO.direct("stokes", s, d, s[rows], skip=rows)) < 1e-13
