@@ -49,6 +49,8 @@ __device__ __forceinline__ void accumulate_masked(double d0, double d1, double d
 template <int OUT>
 __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
   __shared__ double4 tile[EV_CK];
+  __shared__ int32_t tile_id[EV_CK];
+  __shared__ NearList nl;
   // blocks in leaf order, dealt round-robin to the XCDs: the leaves' work varies by
   // level (they are stored level by level), and round-robin keeps the XCDs balanced
   const int s = blockIdx.x;
@@ -63,9 +65,8 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
   const double *__restrict__ X = a.surf;
   const double *__restrict__ Y = a.surf + a.Nl;
   const double *__restrict__ Z = a.surf + 2 * a.Nl;
-  int32_t nbr[NCOLL];
-#pragma unroll
-  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  near_list(a, i, nl);
+  const int32_t nsrc = nl.pre[NCOLL];
   for (int32_t tb = t0; tb < t1; tb += EV_TB) {
     // a tile of up to 64 targets; when it has fewer, R = 64 / ntp replicas of each
     // target split the surface points and the sources, and are summed at the end
@@ -97,23 +98,21 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
       }
     }
     // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363), their
-    // sources staged through the same tiles
+    // sources as one list through the same tiles
     const int32_t skip = self ? j : -1;
-    for (int m = 0; m < NCOLL; m++) {
-      const int32_t nb = nbr[m];
-      if (nb < 0) continue;
-      const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
-      for (int32_t q0 = p0; q0 <= p1; q0 += EV_CK) {
-        const int nq = min(EV_CK, p1 - q0 + 1);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nq; k += EV_TB)
-          tile[k] = make_double4(a.sx[q0 + k], a.sy[q0 + k], a.sz[q0 + k], a.tau[q0 + k]);
-        __syncthreads();
+    for (int32_t q0 = 0; q0 < nsrc; q0 += EV_CK) {
+      const int nq = min(EV_CK, nsrc - q0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < nq; k += EV_TB) {
+        const int32_t q = near_source(nl, q0 + k);
+        tile[k] = make_double4(a.sx[q], a.sy[q], a.sz[q], a.tau[q]);
+        tile_id[k] = q;
+      }
+      __syncthreads();
 #pragma unroll 4
-        for (int k = rep; k < nq; k += R) {
-          const double4 v = tile[k];
-          accumulate_masked<OUT>(x - v.x, y - v.y, z - v.z, v.w, q0 + k != skip, acc);
-        }
+      for (int k = rep; k < nq; k += R) {
+        const double4 v = tile[k];
+        accumulate_masked<OUT>(x - v.x, y - v.y, z - v.z, v.w, tile_id[k] != skip, acc);
       }
     }
     for (int off = ntp; off < EV_TB; off <<= 1)
@@ -171,9 +170,18 @@ __device__ __forceinline__ void fold32(float &s, float &c, float g[3], double ac
   g[0] = g[1] = g[2] = 0.f;
 }
 
+// a near-field source in LDS: FP64 position (the pair difference is formed in FP64),
+// FP32 charge, and its index for the self-exclusion
+struct NearSrc32 {
+  double x, y, z;
+  float w;
+  int32_t id;
+};
+
 template <int OUT>
 __global__ void __launch_bounds__(EV_TB) k_eval32(EvalArgs a) {
-  __shared__ double4 tile[EV_CK];
+  __shared__ NearSrc32 tile[EV_CK];
+  __shared__ NearList nl;
   float4 *tile32 = reinterpret_cast<float4 *>(tile);
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
@@ -187,9 +195,8 @@ __global__ void __launch_bounds__(EV_TB) k_eval32(EvalArgs a) {
   const double *__restrict__ X = a.surf;
   const double *__restrict__ Y = a.surf + a.Nl;
   const double *__restrict__ Z = a.surf + 2 * a.Nl;
-  int32_t nbr[NCOLL];
-#pragma unroll
-  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  near_list(a, i, nl);
+  const int32_t nsrc = nl.pre[NCOLL];
   for (int32_t tb = t0; tb < t1; tb += EV_TB) {
     const int cnt = min(EV_TB, t1 - tb);
     int ntp = 1;
@@ -219,34 +226,32 @@ __global__ void __launch_bounds__(EV_TB) k_eval32(EvalArgs a) {
       }
       fold32<OUT>(ks, kc, g, acc);
     }
+    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363), their
+    // sources as one list through the same tiles
     const int32_t skip = self ? j : -1;
-    for (int m = 0; m < NCOLL; m++) {
-      const int32_t nb = nbr[m];
-      if (nb < 0) continue;
-      const int32_t p0 = a.plo[nb], p1 = a.phi[nb];
-      for (int32_t q0 = p0; q0 <= p1; q0 += EV_CK) {
-        const int nq = min(EV_CK, p1 - q0 + 1);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nq; k += EV_TB)
-          tile[k] = make_double4(a.sx[q0 + k], a.sy[q0 + k], a.sz[q0 + k], a.tau[q0 + k]);
-        __syncthreads();
-#pragma unroll 4
-        for (int k = rep; k < nq; k += R) {
-          const double4 v = tile[k];
-          // the target's own source: an infinite distance, so 1/r is exactly zero
-          const float d0 = (float)(x - v.x), d1 = (float)(y - v.y), d2 = (float)(z - v.z);
-          const float ir = __builtin_amdgcn_rsqf(q0 + k != skip ? fmaf(d2, d2, fmaf(d1, d1, d0 * d0)) : INFINITY);
-          const float w = (float)v.w;
-          if (OUT != FFMM_GRADIENT) kahan_add(ks, kc, fmaf(ir, w, -kc));
-          if (OUT != FFMM_POTENTIAL) {
-            const float aa = ir * ir * ir * w;
-            g[0] = fmaf(aa, d0, g[0]);
-            g[1] = fmaf(aa, d1, g[1]);
-            g[2] = fmaf(aa, d2, g[2]);
-          }
-        }
-        fold32<OUT>(ks, kc, g, acc);
+    for (int32_t q0 = 0; q0 < nsrc; q0 += EV_CK) {
+      const int nq = min(EV_CK, nsrc - q0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < nq; k += EV_TB) {
+        const int32_t q = near_source(nl, q0 + k);
+        tile[k] = NearSrc32{a.sx[q], a.sy[q], a.sz[q], (float)a.tau[q], q};
       }
+      __syncthreads();
+#pragma unroll 4
+      for (int k = rep; k < nq; k += R) {
+        const NearSrc32 v = tile[k];
+        // the target's own source: an infinite distance, so 1/r is exactly zero
+        const float d0 = (float)(x - v.x), d1 = (float)(y - v.y), d2 = (float)(z - v.z);
+        const float ir = __builtin_amdgcn_rsqf(v.id != skip ? fmaf(d2, d2, fmaf(d1, d1, d0 * d0)) : INFINITY);
+        if (OUT != FFMM_GRADIENT) kahan_add(ks, kc, fmaf(ir, v.w, -kc));
+        if (OUT != FFMM_POTENTIAL) {
+          const float aa = ir * ir * ir * v.w;
+          g[0] = fmaf(aa, d0, g[0]);
+          g[1] = fmaf(aa, d1, g[1]);
+          g[2] = fmaf(aa, d2, g[2]);
+        }
+      }
+      fold32<OUT>(ks, kc, g, acc);
     }
     for (int off = ntp; off < EV_TB; off <<= 1)
 #pragma unroll

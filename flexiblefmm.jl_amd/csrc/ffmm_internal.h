@@ -406,6 +406,51 @@ struct EvalArgs {
   double kappa;
 };
 
+// The near-field sources of a leaf as one list: the 27 colleagues' point ranges
+// [lo[m], lo[m] + pre[m + 1] - pre[m]) back to back, so that they pass through the LDS
+// tiles in full chunks (an average leaf's ~350 colleague sources in two chunks rather
+// than one short tile per colleague, each behind a global-load round trip).
+struct NearList {
+  int32_t lo[NCOLL];
+  int32_t pre[NCOLL + 1];
+};
+
+// built by the block's first wave (any block of at least 32 threads); synchronises the block
+__device__ __forceinline__ void near_list(const EvalArgs &a, int i, NearList &nl) {
+  const int t = threadIdx.x;
+  int32_t lo = 0, cnt = 0;
+  if (t < NCOLL) {
+    const int32_t nb = a.col[(int64_t)t * a.nn + i];
+    if (nb >= 0) {
+      lo = a.plo[nb];
+      cnt = max(0, a.phi[nb] - lo + 1);
+    }
+  }
+  int32_t v = cnt;   // inclusive scan over lanes 0..31
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) {
+    const int32_t u = __shfl_up(v, off, 64);
+    if (t >= off) v += u;
+  }
+  if (t < NCOLL) {
+    nl.lo[t] = lo;
+    nl.pre[t + 1] = v;
+  }
+  if (t == 0) nl.pre[0] = 0;
+  __syncthreads();
+}
+
+// the source index of entry f of the list (0 <= f < pre[NCOLL]): the last m with pre[m] <= f
+__device__ __forceinline__ int32_t near_source(const NearList &nl, int32_t f) {
+  int m = 0, h = NCOLL;
+  while (h - m > 1) {
+    const int mid = (m + h) >> 1;
+    if (nl.pre[mid] <= f) m = mid;
+    else h = mid;
+  }
+  return nl.lo[m] + (f - nl.pre[m]);
+}
+
 // ---- the life cycles (ffmm_abi.hip), shared by the C entry points and the server ----
 void context_init(Context &c, int device, void *stream);
 void context_release(Context &c);

@@ -338,9 +338,10 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   const int lv = a.level[i];
   const double sc = a.hscale[lv];
   const double kh = KIND == 1 ? a.khat[lv] : 0.0;
-  int32_t nbr[NCOLL];
-#pragma unroll
-  for (int m = 0; m < NCOLL; m++) nbr[m] = a.col[(int64_t)m * a.nn + i];
+  __shared__ NearList nl;
+  __shared__ int32_t tile_id[HT_CK];
+  near_list(a, i, nl);
+  const int32_t nsrc = nl.pre[NCOLL];
   for (int32_t tbk = t0; tbk < t1; tbk += HB) {
     const int cnt = min(HB, t1 - tbk);
     int ntp = 1;
@@ -451,27 +452,24 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
       }
     }
     // the near field of the 27 colleagues (src/device.jl:89-100), their sources and
-    // densities staged through LDS tiles (broadcast reads)
+    // densities as one list staged through LDS tiles (broadcast reads)
     const int32_t skip = self ? j : -1;
-    for (int m = 0; m < NCOLL; m++) {
-      const int32_t nb2 = nbr[m];
-      if (nb2 < 0) continue;
-      const int32_t p0 = a.plo[nb2], p1 = a.phi[nb2];
-      for (int32_t q0 = p0; q0 <= p1; q0 += HT_CK) {
-        const int nq = min(HT_CK, p1 - q0 + 1);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nq; k += HB) {
-          double *e = tile + k * TW;
-          e[0] = a.sx[q0 + k];
-          e[1] = a.sy[q0 + k];
-          e[2] = a.sz[q0 + k];
-          for (int c = 0; c < a.nd; c++) e[3 + c] = a.tau[(int64_t)(q0 + k) * a.nd + c];
-        }
-        __syncthreads();
-        for (int k = rep; k < nq; k += R) {
-          const double *e = tile + k * TW;
-          near_term<KIND, OUT>(x - e[0], y - e[1], z - e[2], e + 3, a.nd, q0 + k != skip, a.kappa, acc);
-        }
+    for (int32_t q0 = 0; q0 < nsrc; q0 += HT_CK) {
+      const int nq = min(HT_CK, nsrc - q0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < nq; k += HB) {
+        const int32_t q = near_source(nl, q0 + k);
+        double *e = tile + k * TW;
+        e[0] = a.sx[q];
+        e[1] = a.sy[q];
+        e[2] = a.sz[q];
+        for (int c = 0; c < a.nd; c++) e[3 + c] = a.tau[(int64_t)q * a.nd + c];
+        tile_id[k] = q;
+      }
+      __syncthreads();
+      for (int k = rep; k < nq; k += R) {
+        const double *e = tile + k * TW;
+        near_term<KIND, OUT>(x - e[0], y - e[1], z - e[2], e + 3, a.nd, tile_id[k] != skip, a.kappa, acc);
       }
     }
     for (int off = ntp; off < HB; off <<= 1)

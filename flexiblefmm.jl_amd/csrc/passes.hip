@@ -158,6 +158,29 @@ __global__ void k_gather_cols(double *__restrict__ dst, const double *__restrict
   int i = (int)(g % rows);
   dst[g] = src[(int64_t)cols[q] * rows + i];
 }
+// dst[i, cols[q]] = lam[level[cols[q]]] * src[i, q]: the columns of several levels at once,
+// each scaled by its level's width factor
+__global__ void k_scatter_cols_lam(double *__restrict__ dst, const double *__restrict__ src,
+                                   const int32_t *__restrict__ cols, int rows, int64_t ncols,
+                                   const int32_t *__restrict__ level, const double *__restrict__ lam) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ncols) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  const int32_t c = cols[q];
+  dst[(int64_t)c * rows + i] = lam[level[c]] * src[g];
+}
+// dst[i, q] = lam[level[cols[q]]] * src[i, cols[q]]
+__global__ void k_gather_cols_lam(double *__restrict__ dst, const double *__restrict__ src,
+                                  const int32_t *__restrict__ cols, int rows, int64_t ncols,
+                                  const int32_t *__restrict__ level, const double *__restrict__ lam) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)rows * ncols) return;
+  int64_t q = g / rows;
+  int i = (int)(g % rows);
+  const int32_t c = cols[q];
+  dst[g] = lam[level[c]] * src[(int64_t)c * rows + i];
+}
 // the parents of the sibling groups starting at node `first`: dst[i, parent[first + 8g]] = alpha src[i, g]
 template <class T>
 __global__ void k_scatter_parents(double *__restrict__ dst, const T *__restrict__ src,
@@ -270,14 +293,15 @@ static constexpr int64_t PIECE = 1 << 18;   // pairs per dense M2L GEMM
 void allocate_pass_buffers(Plan &plan) {
   OpCache &o = *plan.ops;
   const int Ns = o.Ns, Nl = o.Nl;
-  int64_t nq = 1, ng = 1, nlmax = 1;
+  int64_t ng = 1, nlmax = 1;
   for (int l = 0; l < plan.L; l++) {
-    nq = std::max(nq, plan.s2m_off[l + 1] - plan.s2m_off[l]);
-    nq = std::max(nq, plan.leaf_off[l + 1] - plan.leaf_off[l]);
     ng = std::max(ng, plan.nl[l] / NCHILD);
     nlmax = std::max(nlmax, plan.nl[l]);
   }
   int64_t nleaves = plan.leaf_off[plan.L];
+  // UC2E and DC2E run once over the leaves of all levels from 2 down
+  const int l2 = std::min(2, plan.L);
+  const int64_t nq = std::max<int64_t>({1, plan.s2m_off[plan.L] - plan.s2m_off[l2], nleaves - plan.leaf_off[l2]});
   plan.Mp.resize((size_t)Ns * plan.nnodes);
   plan.Pl.resize((size_t)Ns * plan.nnodes);
   plan.Lc.resize((size_t)Nl * std::max<int64_t>(nleaves, 1));
@@ -327,27 +351,33 @@ void upward_pass(Plan &plan) {
   // every column is written before it is read except by M2M of empty groups: clear
   k_fill<<<nblocks(plan.Mp.n, TB), TB, 0, s>>>(plan.Mp.p, plan.Mp.n, 0.0);
   FFMM_LAUNCH_CHECK();
+  const int l2 = std::min(2, plan.L);
+  const int64_t nq = plan.s2m_off[plan.L] - plan.s2m_off[l2];
+  if (o.method == METHOD_CUBE && nq > 0) {
+    // the childless sourced nodes of all levels at once (each node's width from its level):
+    // S2M into check potentials, then UC2E, M = pinv_scale V diag(invS) (Uh pM)
+    // (src/device_passes.jl:496-502), as two GEMMs over all of them.  pinv_scale = lam[l]
+    // is a power of two, applied exactly in the scatter.
+    const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l2];
+    if (plan.precision == FFMM_PREC_MIXED)
+      k_s2m32<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+                                              o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
+                                              plan.tau.p, plan.pM.p);
+    else
+      k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+                                            o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
+                                            plan.tau.p, plan.pM.p);
+    FFMM_LAUNCH_CHECK();
+    gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
+    gemm(ctx, Ns, (int)nq, o.r_up, 1.0, o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
+    k_scatter_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq,
+                                                                    plan.level.p, plan.lam_d.p);
+    FFMM_LAUNCH_CHECK();
+  }
   for (int l = plan.L - 1; l >= 2; l--) {
-    const int64_t nq = plan.s2m_off[l + 1] - plan.s2m_off[l];
-    const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];
     if (o.method == METHOD_HARMONIC) {
       // the multipoles directly (partial_to_multipole! is a copy, src/operators/harmonics.jl:441)
       harmonic_s2m_level(plan, l);
-    } else if (nq > 0) {
-      if (plan.precision == FFMM_PREC_MIXED)
-        k_s2m32<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
-                                                o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
-                                                plan.tau.p, plan.pM.p);
-      else
-        k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
-                                              o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
-                                              plan.tau.p, plan.pM.p);
-      FFMM_LAUNCH_CHECK();
-      // UC2E: M = pinv_scale V diag(invS) (Uh pM)  (src/device_passes.jl:496-502)
-      gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
-      gemm(ctx, Ns, (int)nq, o.r_up, plan.lam[l], o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
-      k_scatter_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq, 1.0);
-      FFMM_LAUNCH_CHECK();
     }
     if (l < plan.L - 1) {
       const int64_t ng = plan.nl[l + 1] / NCHILD;
@@ -442,24 +472,23 @@ void downward_pass(Plan &plan) {
     if (plan.m2l == FFMM_M2L_SPECTRAL) m2l_spectral_level(plan, l);
     else m2l_dense_level(plan, l);
   }
-  // the densities of the real leaves: DC2E (src/device_passes.jl:582-593)
-  for (int l = 2; l < plan.L; l++) {
-    const int64_t nq = plan.leaf_off[l + 1] - plan.leaf_off[l];
-    if (nq == 0) continue;
-    if (o.method == METHOD_HARMONIC) {
-      // partial_to_local! is a copy (src/operators/harmonics.jl:442)
-      k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Lc.p + plan.leaf_off[l] * Nl, plan.Pl.p,
-                                                                 plan.leaf_nodes.p + plan.leaf_off[l], Ns, nq);
-      FFMM_LAUNCH_CHECK();
-      continue;
-    }
-    k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, plan.leaf_nodes.p + plan.leaf_off[l],
-                                                               Ns, nq);
+  // the densities of the real leaves of all levels at once: DC2E (src/device_passes.jl:582-593)
+  const int64_t nq = plan.leaf_off[plan.L] - plan.leaf_off[l2];
+  if (nq == 0) return;
+  const int32_t *leaves = plan.leaf_nodes.p + plan.leaf_off[l2];
+  double *Lc = plan.Lc.p + plan.leaf_off[l2] * Nl;
+  if (o.method == METHOD_HARMONIC) {
+    // partial_to_local! is a copy (src/operators/harmonics.jl:442)
+    k_gather_cols<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(Lc, plan.Pl.p, leaves, Ns, nq);
     FFMM_LAUNCH_CHECK();
-    gemm(ctx, o.r_dn, (int)nq, Ns, 1.0, o.Uh_dn.p, o.r_dn, plan.G.p, Ns, 0.0, plan.tup.p, o.r_dn);
-    gemm(ctx, Nl, (int)nq, o.r_dn, plan.lam[l], o.Vs_dn.p, Nl, plan.tup.p, o.r_dn, 0.0,
-         plan.Lc.p + plan.leaf_off[l] * Nl, Nl);
+    return;
   }
+  // the level's pinv_scale (a power of two) on the gathered check potentials: exact
+  k_gather_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, leaves, Ns, nq, plan.level.p,
+                                                                 plan.lam_d.p);
+  FFMM_LAUNCH_CHECK();
+  gemm(ctx, o.r_dn, (int)nq, Ns, 1.0, o.Uh_dn.p, o.r_dn, plan.G.p, Ns, 0.0, plan.tup.p, o.r_dn);
+  gemm(ctx, Nl, (int)nq, o.r_dn, 1.0, o.Vs_dn.p, Nl, plan.tup.p, o.r_dn, 0.0, Lc, Nl);
 }
 
 }  // namespace ffmm

@@ -66,11 +66,13 @@ size_t plan_bytes(const Plan &p) {
 
 // a conservative estimate of a plan's device memory before its tree exists: per node
 // (about 12 N / ppl of them) the multipole and partial-local columns and, spectral,
-// the single-precision spectra; per leaf (about 5 N / ppl) the local column; per point
-// the sorted points, densities, outputs and the sort's buffers
+// the single-precision spectra; per leaf (about 5 N / ppl) the local column and, cube
+// method, the check potentials and pseudo-inverse intermediates of all leaves at once;
+// per point the sorted points, densities, outputs and the sort's buffers
 static int64_t estimate_bytes(const OpCache &o, int64_t n, int ppl, int m2l, int64_t nt) {
   const double nodes = 12.0 * (double)n / ppl + 64, leaves = 5.0 * (double)n / ppl + 64;
   double b = nodes * (2.0 * o.Ns * 8) + leaves * (o.Nl * 8.0) + 256.0 * n + 96.0 * nt;
+  if (o.method == METHOD_CUBE) b += leaves * (o.Nl + 3.0 * o.Ns) * 8;
   if (m2l == FFMM_M2L_SPECTRAL) b += 0.6 * nodes * 2304 * 16;
   if (m2l == FFMM_M2L_DENSE) b += 2.0 * o.Ns * 8 * std::min<double>(1 << 18, 200.0 * nodes);
   return (int64_t)b + (64ll << 20);
