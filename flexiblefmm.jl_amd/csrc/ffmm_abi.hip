@@ -257,6 +257,26 @@ int ffmm_opcache_destroy(ffmm_opcache cache) {
   });
 }
 
+// the permutations of the class M2L with their signs folded into the top bit (signs are
+// +1 or -1): gsel[f, i] = idx[f, i] | sign(sgn[f, i]); ssel[f, j] = inv[f, j] | sign(sgn_t[f, inv[f, j]])
+static void upload_signed_tables(OpCache &o, const int32_t *idx, const int8_t *sgn, const int32_t *inv,
+                                 const int8_t *sgn_t, int N, hipStream_t s) {
+  const size_t n = (size_t)NFAR * N;
+  std::vector<int32_t> g(n), t(n);
+  for (int f = 0; f < NFAR; f++)
+    for (int i = 0; i < N; i++) {
+      const size_t e = (size_t)f * N + i;
+      if ((sgn[e] != 1 && sgn[e] != -1) || (sgn_t[e] != 1 && sgn_t[e] != -1))
+        fail(FFMM_ERR_ARGUMENT, "signs must be +1 or -1");
+      g[e] = idx[e] | (sgn[e] < 0 ? INT32_MIN : 0);
+      const int32_t k = inv[e];
+      t[e] = k | (sgn_t[(size_t)f * N + k] < 0 ? INT32_MIN : 0);
+    }
+  o.gsel.upload(g.data(), n, s);
+  o.ssel.upload(t.data(), n, s);
+  FFMM_HIP(hipStreamSynchronize(s));   // before the host tables go
+}
+
 int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small, const double *large, int r_up,
                           const double *Uh_up, const double *invS_up, const double *V_up, int r_dn,
                           const double *Uh_dn, const double *invS_dn, const double *V_dn, const double *M2M,
@@ -320,6 +340,7 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     o.inv.upload(inv.data(), inv.size(), s);
     o.sgn.upload(sgn, (size_t)NFAR * Ns, s);
     o.sgn_t.upload(sgn, (size_t)NFAR * Ns, s);   // the cube lattices: one permutation for both sides
+    upload_signed_tables(o, idx, sgn, inv.data(), sgn, Ns, s);
     o.embed.upload(o.embed_h.data(), o.embed_h.size(), s);
     {
       // the p^3-corner index of each surface point, and the far index of every
@@ -456,6 +477,7 @@ int ffmm_opcache_put_harmonic_classes(ffmm_opcache cache, int nclass, const int3
     o.sgn.upload(sgn, (size_t)NFAR * N, s);
     o.inv.upload(inv.data(), inv.size(), s);
     o.sgn_t.upload(sgn_t, (size_t)NFAR * N, s);
+    upload_signed_tables(o, idx, sgn, inv.data(), sgn_t, N, s);
     ctx.sync();
     o.has_classes = true;
     o.ready = true;

@@ -257,6 +257,9 @@ struct OpCache {
   DBuf<double> M2M, L2L, class_ops;
   DBuf<float> M2M32, class_ops32;   // FP32 copies for the mixed form
   DBuf<int32_t> far_class, idx, inv;         // idx/inv: [f * Ns + i]
+  // the same permutations with their signs in the top bit: gsel[f, i] = idx[f, i] (sign
+  // sgn[f, i]); ssel[f, j] = i = inv[f, j] (sign sgn_t[f, i]); one load per entry
+  DBuf<int32_t> gsel, ssel;
   DBuf<int8_t> sgn;
   DBuf<double> tables;                       // 316 x nfreq complex (interleaved)
   DBuf<float> tables32;
@@ -271,12 +274,17 @@ struct OpCache {
 
 // per-level lists of far pairs (targets in node order) and their class-major form
 struct LevelLists {
-  int64_t np = 0, nruns = 0;
+  int64_t np = 0;
   bool filled = false;                  // src/far written (the spectral M2L needs only ptr)
   DBuf<int32_t> ptr, src, far;          // by target: ptr (nl + 1, 0-based pair index)
-  DBuf<int32_t> csrc, cfar, rtgt, rptr; // class major; runs: (level-local target, first pair)
+  DBuf<int32_t> csrc, cfar;             // class major, by target within a class
+  DBuf<int32_t> cstart;                 // nclass nl + 1: first class-major pair of (class c, target t) at c nl + t
   std::vector<int64_t> class_pair;      // nclass + 1 offsets into csrc
-  std::vector<int64_t> class_run;       // nclass + 1 offsets into rtgt
+  // target chunks of the dense class M2L: chunk k covers the level-local targets
+  // [chunk_t[k], chunk_t[k + 1]) and, in class c, the pairs
+  // [chunk_cs[k nclass + c], chunk_cs[(k + 1) nclass + c])
+  std::vector<int32_t> chunk_t, chunk_cs;
+  int64_t chunk_max = 0;                // the most pairs in one chunk
 };
 
 struct Plan {
@@ -362,6 +370,14 @@ void node_leafslots(Plan &plan);
 // ---- lists (lists.hip) ----
 void build_lists(Plan &plan);
 void fill_level_pairs(Plan &plan, int l);   // src/far of level l, when not yet written
+// far pairs per target chunk of the dense class M2L: its gathered columns and products
+// (two buffers of one column per pair and block) take about 2 GiB
+inline int64_t m2l_chunk_pairs(const Plan &p) {
+  const OpCache &o = *p.ops;
+  const int64_t col = o.method == METHOD_HARMONIC ? (int64_t)o.nb * o.hN : (int64_t)o.Ns;
+  const int64_t bytes = 2 * col * (p.precision == FFMM_PREC_MIXED ? 4 : 8);
+  return std::max<int64_t>(4096, (2ll << 30) / bytes);
+}
 
 // ---- passes (passes.hip) ----
 void allocate_pass_buffers(Plan &plan);

@@ -106,40 +106,30 @@ __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__r
   }
 }
 
-__global__ void k_flag_pos(const int32_t *__restrict__ cnt, int64_t n, uint8_t *__restrict__ flag) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) flag[e] = cnt[e] > 0;
-}
-
-// _run_fill! (src/device_lists.jl:335-344): run r = compacted entry e of the class-major
-// (class, target) table: level-local target e % nl, first pair cstart[e]
-__global__ void k_run_fill(const int32_t *__restrict__ ent, int64_t nruns, const int32_t *__restrict__ cstart,
-                           int64_t nl, int32_t *__restrict__ rtgt, int32_t *__restrict__ rptr) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
-  int64_t e = ent[r];
-  rtgt[r] = (int32_t)(e % nl);
-  rptr[r] = cstart[e];
+// the target chunks of a level's dense class M2L: chunk k starts at the first target t
+// with ptr[t] >= k cap (t = nl for k = nch); per chunk start, the class-major position of
+// (class c, target t) for every class.  out: (nch + 1) rows of 1 + nclass
+__global__ void k_chunk_bounds(const int32_t *__restrict__ ptr, int64_t nl, int64_t cap, int64_t nch,
+                               const int32_t *__restrict__ cstart, int nclass, int32_t *__restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nch) return;
+  int64_t t = nl;
+  if (k < nch) {
+    const int64_t key = k * cap;
+    int64_t a = 0, b = nl;
+    while (a < b) {
+      const int64_t m = (a + b) >> 1;
+      if (ptr[m] < key) a = m + 1;
+      else b = m;
+    }
+    t = a;
+  }
+  int32_t *o = out + k * (1 + nclass);
+  o[0] = (int32_t)t;
+  for (int c = 0; c < nclass; c++) o[1 + c] = cstart[(int64_t)c * nl + t];
 }
 
 __global__ void k_set_i32(int32_t *a, int32_t v) { a[0] = v; }
-
-// per class c: the first pair (cstart at entry c * nl) and the first run (the first
-// compacted entry >= c * nl; the entries are ascending)
-__global__ void k_class_offsets(const int32_t *__restrict__ cstart, const int32_t *__restrict__ ent, int64_t nruns,
-                                int64_t nl, int nclass, int32_t *__restrict__ out) {
-  int c = threadIdx.x;
-  if (c >= nclass) return;
-  int64_t key = (int64_t)c * nl;
-  out[c] = cstart[key];
-  int64_t a = 0, b = nruns;
-  while (a < b) {
-    int64_t m = (a + b) >> 1;
-    if (ent[m] < key) a = m + 1;
-    else b = m;
-  }
-  out[nclass + c] = (int32_t)a;
-}
 
 void fill_level_pairs(Plan &plan, int l) {
   LevelLists &li = plan.lists[l];
@@ -174,14 +164,14 @@ void build_lists(Plan &plan) {
   TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};
   plan.lists.clear();
   plan.lists.resize(plan.L);
-  DBuf<int32_t> cnt, ccnt, cstart, ent, offs;
-  DBuf<uint8_t> flag;
-  offs.resize(2 * nclass);
+  DBuf<int32_t> cnt, ccnt, bounds;
   for (int l = 0; l < plan.L; l++) {
     LevelLists &li = plan.lists[l];
     int64_t lo = plan.lo[l], nl = plan.nl[l];
     li.class_pair.assign(nclass + 1, 0);
-    li.class_run.assign(nclass + 1, 0);
+    li.chunk_t.clear();
+    li.chunk_cs.clear();
+    li.chunk_max = 0;
     li.ptr.resize(nl + 1);
     if (l < 2) {
       FFMM_HIP(hipMemsetAsync(li.ptr.p, 0, sizeof(int32_t) * (nl + 1), s));
@@ -202,43 +192,41 @@ void build_lists(Plan &plan) {
     // the class lists (only the dense class translations read them)
     int64_t ne = (int64_t)nclass * nl;
     ccnt.resize(ne);
-    cstart.resize(ne);
+    li.cstart.resize(ne + 1);
     k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ops.far_class.p, ccnt.p);
     FFMM_LAUNCH_CHECK();
-    exclusive_scan_i32(ctx, ccnt.p, cstart.p, ne);
+    exclusive_scan_i32(ctx, ccnt.p, li.cstart.p, ne);
+    k_set_i32<<<1, 1, 0, s>>>(li.cstart.p + ne, (int32_t)np);
     li.csrc.resize(np);
     li.cfar.resize(np);
     if (np > 0) {
-      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, cstart.p, nl, nclass,
+      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, li.cstart.p, nl, nclass,
                                                   ops.far_class.p, li.csrc.p, li.cfar.p);
       FFMM_LAUNCH_CHECK();
     }
-    // the runs
-    flag.resize(ne);
-    ent.resize(ne);
-    k_flag_pos<<<nblocks(ne, TB), TB, 0, s>>>(ccnt.p, ne, flag.p);
+    // the target chunks and their class offsets, one read back
+    const int64_t cap = m2l_chunk_pairs(plan);
+    const int64_t nch = (np + cap - 1) / cap;
+    const int64_t nrow = 1 + nclass;
+    bounds.resize((nch + 1) * nrow);
+    k_chunk_bounds<<<nblocks(nch + 1, 64), 64, 0, s>>>(li.ptr.p, nl, cap, nch, li.cstart.p, nclass, bounds.p);
     FFMM_LAUNCH_CHECK();
-    int64_t nruns = compact_flags_u8(ctx, flag.p, ent.p, ne);
-    li.nruns = nruns;
-    li.rtgt.resize(nruns);
-    li.rptr.resize(nruns + 1);
-    if (nruns > 0) {
-      k_run_fill<<<nblocks(nruns, TB), TB, 0, s>>>(ent.p, nruns, cstart.p, nl, li.rtgt.p, li.rptr.p);
-      FFMM_LAUNCH_CHECK();
-    }
-    k_set_i32<<<1, 1, 0, s>>>(li.rptr.p + nruns, (int32_t)np);
-    // per-class offsets of the pairs (the class-major scan at each class's first entry)
-    // and of the runs (the first compacted entry of each class), one read back
-    k_class_offsets<<<1, 64, 0, s>>>(cstart.p, ent.p, nruns, nl, nclass, offs.p);
-    FFMM_LAUNCH_CHECK();
-    std::vector<int32_t> ho(2 * nclass);
-    offs.download(ho.data(), 2 * nclass, s);
+    std::vector<int32_t> hb((size_t)((nch + 1) * nrow));
+    bounds.download(hb.data(), hb.size(), s);
     ctx.sync();
-    for (int c = 0; c < nclass; c++) {
-      li.class_pair[c] = ho[c];
-      li.class_run[c] = ho[nclass + c];
+    li.chunk_t.resize(nch + 1);
+    li.chunk_cs.resize((size_t)((nch + 1) * nclass));
+    for (int64_t k = 0; k <= nch; k++) {
+      li.chunk_t[k] = hb[k * nrow];
+      for (int c = 0; c < nclass; c++) li.chunk_cs[k * nclass + c] = hb[k * nrow + 1 + c];
     }
-    li.class_run[nclass] = nruns;
+    li.chunk_max = 0;
+    for (int64_t k = 0; k < nch; k++) {
+      int64_t n = 0;
+      for (int c = 0; c < nclass; c++) n += li.chunk_cs[(k + 1) * nclass + c] - li.chunk_cs[k * nclass + c];
+      li.chunk_max = std::max(li.chunk_max, n);
+    }
+    for (int c = 0; c < nclass; c++) li.class_pair[c] = hb[1 + c];
     li.class_pair[nclass] = np;
   }
   ctx.sync();

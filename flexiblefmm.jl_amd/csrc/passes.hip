@@ -230,47 +230,117 @@ __global__ void k_gather_perm_nd(double *__restrict__ dst, const double *__restr
 // on each block (nb columns per pair, as the reference lays out the Stokes blocks).
 // Gc[i, q nb + b] = sgn[f, i] Mp[b N + idx[f, i], src[q]]  (P_f^T m, block b)
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void k_class_gather(T *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
-                               const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int64_t npk,
-                               const int32_t *__restrict__ idx, const int8_t *__restrict__ sgn) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)N * nb * npk) return;
-  int64_t c = g / N;
-  int i = (int)(g % N);
-  int64_t q = c / nb;
-  int b = (int)(c % nb);
-  int64_t f = cfar[q];
-  Gc[g] = (T)((double)sgn[f * N + i] * Mp[(int64_t)csrc[q] * Ns + b * N + idx[f * N + i]]);
-}
-// target-owned scatter of P_t y over the runs of a class that fall in the piece
-// [a, a + npk): pL[b N + j, rtgt[r]] += alpha sum_p sgn_t[f, i] Y_b[i, p - a], i = inv[f, j];
-// for Stokes (nb = 4) the fourth block adds the coupling Psi += o_f . Phi with the
-// pair's own offset (src/device_passes.jl:449-473, src/operators/harmonics.jl:604-628)
-template <class T>
-__global__ void k_class_scatter(double *__restrict__ pL, const T *__restrict__ Yc, int Ns, int N, int nb,
-                                const int32_t *__restrict__ rtgt, const int32_t *__restrict__ rptr, int64_t r0,
-                                int64_t nr, const int32_t *__restrict__ cfar, int64_t a, int64_t npk,
-                                const int32_t *__restrict__ inv, const int8_t *__restrict__ sgn_t,
-                                const double *__restrict__ offs, double alpha) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)Ns * nr) return;
-  int64_t r = r0 + g / Ns;
-  int jb = (int)(g % Ns);
-  int b = jb / N, j = jb % N;
-  int64_t p0 = max((int64_t)rptr[r], a), p1 = min((int64_t)rptr[r + 1], a + npk);
-  if (p0 >= p1) return;
-  double acc = 0.0;
-  for (int64_t p = p0; p < p1; p++) {
-    int64_t f = cfar[p];
-    int i = inv[f * N + j];
-    const T *Y = Yc + (p - a) * nb * N;
-    double y = (double)Y[b * N + i];
-    if (nb == 4 && b == 3)
-      y = y + offs[3 * f] * (double)Y[i] + offs[3 * f + 1] * (double)Y[N + i] + offs[3 * f + 2] * (double)Y[2 * N + i];
-    acc += (double)sgn_t[f * N + i] * y;
+// One target chunk of a level's class M2L: the chunk-local columns of class c are
+// [co[c], co[c + 1]) and hold the class-major pairs cs[c] + (q - co[c]).
+struct ChunkArgs {
+  int nclass;
+  int32_t co[MAX_CLASSES + 1];
+  int32_t cs[MAX_CLASSES];
+};
+
+// the class of chunk column q: the last c with co[c] <= q
+__device__ __forceinline__ int chunk_class(const ChunkArgs &ca, int32_t q) {
+  int c = 0, h = ca.nclass;
+  while (h - c > 1) {
+    const int m = (c + h) >> 1;
+    if (ca.co[m] <= q) c = m;
+    else h = m;
   }
-  pL[(int64_t)rtgt[r] * Ns + jb] += alpha * acc;
+  return c;
+}
+
+constexpr int GATHER_PAIRS = 8;   // pairs per workgroup of the gather
+constexpr int SCATTER_MAXP = 320; // far pairs of one target: at most the 316 far offsets
+
+// Gc[b N + i, q] = sgn[f, i] Mp[b N + idx[f, i], src]  (P_f^T m per block b) for the pairs
+// of a chunk, all classes in one launch; one workgroup per GATHER_PAIRS columns, one lane
+// per entry of a column, the pairs' loads independent of each other
+template <class T>
+__global__ void k_chunk_gather(T *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
+                               const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int32_t npk,
+                               const int32_t *__restrict__ gsel, ChunkArgs ca) {
+  __shared__ int32_t pf[GATHER_PAIRS], ps[GATHER_PAIRS];
+  const int nbN = nb * N;
+  const int32_t q0 = (int32_t)blockIdx.x * GATHER_PAIRS;
+  const int nq = min(GATHER_PAIRS, npk - q0);
+  if ((int)threadIdx.x < nq) {
+    const int32_t q = q0 + (int32_t)threadIdx.x;
+    const int c = chunk_class(ca, q);
+    const int32_t p = ca.cs[c] + (q - ca.co[c]);
+    pf[threadIdx.x] = cfar[p] * N;
+    ps[threadIdx.x] = csrc[p];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nbN; e += blockDim.x) {
+    const int b = e / N, i = e - b * N;
+#pragma unroll
+    for (int k = 0; k < GATHER_PAIRS; k++) {
+      if (k < nq) {
+        const int32_t g = gsel[pf[k] + i];
+        const double v = Mp[(int64_t)ps[k] * Ns + b * N + (g & 0x7fffffff)];
+        Gc[(int64_t)(q0 + k) * nbN + e] = (T)(g < 0 ? -v : v);
+      }
+    }
+  }
+}
+
+// Target-owned scatter of the chunk's products, all classes at once: one workgroup per
+// target t, one lane per entry (b, j); pL[b N + j, t] += alpha sum over the target's pairs
+// (class by class, in pair order) of sgn_t[f, i] Y_b[i, q], i = inv[f, j].  For Stokes
+// (nb = 4) the fourth block adds the coupling Psi += o_f . Phi with the pair's own offset
+// (src/device_passes.jl:449-473, src/operators/harmonics.jl:604-628).  The target's pairs
+// (offset, chunk column) are staged in LDS first, so that each lane's loop over them
+// issues independent loads.
+template <class T>
+__global__ void k_chunk_scatter(double *__restrict__ pL, const T *__restrict__ Yc, int Ns, int N, int nb,
+                                const int32_t *__restrict__ cstart, int64_t nl, int32_t t0,
+                                const int32_t *__restrict__ cfar, const int32_t *__restrict__ ssel,
+                                const double *__restrict__ offs, double alpha, ChunkArgs ca) {
+  __shared__ int32_t pf[SCATTER_MAXP], pq[SCATTER_MAXP];
+  __shared__ int32_t npair;
+  const int32_t t = t0 + (int32_t)blockIdx.x;
+  const int nbN = nb * N;
+  if (threadIdx.x < 64) {
+    // lane c: class c's pairs of the target; an exclusive scan over the classes places them
+    const int c = threadIdx.x;
+    int32_t p0 = 0, n = 0;
+    if (c < ca.nclass) {
+      const int64_t ec = (int64_t)c * nl + t;
+      p0 = cstart[ec];
+      n = cstart[ec + 1] - p0;
+    }
+    int32_t v = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t u = __shfl_up(v, off, 64);
+      if (c >= off) v += u;
+    }
+    const int32_t start = v - n;
+    const int32_t qoff = c < ca.nclass ? ca.co[c] - ca.cs[c] : 0;
+    for (int32_t k = 0; k < n; k++) {
+      pf[start + k] = cfar[p0 + k];
+      pq[start + k] = p0 + k + qoff;
+    }
+    if (c == 63) npair = v;
+  }
+  __syncthreads();
+  const int32_t np = npair;
+  for (int e = threadIdx.x; e < nbN; e += blockDim.x) {
+    const int b = e / N, j = e - b * N;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int32_t k = 0; k < np; k++) {
+      const int32_t f = pf[k];
+      const int32_t sv = ssel[f * N + j];
+      const int i = sv & 0x7fffffff;
+      const T *__restrict__ Y = Yc + (int64_t)pq[k] * nbN;
+      double y = (double)Y[b * N + i];
+      if (nb == 4 && b == 3)
+        y = y + offs[3 * f] * (double)Y[i] + offs[3 * f + 1] * (double)Y[N + i] + offs[3 * f + 2] * (double)Y[2 * N + i];
+      acc += sv < 0 ? -y : y;
+    }
+    pL[(int64_t)t * Ns + e] += alpha * acc;
+  }
 }
 
 static void gemm(Context &ctx, int m, int n, int k, double alpha, const double *A, int lda, const double *B, int ldb,
@@ -287,8 +357,6 @@ static void gemm32(Context &ctx, int m, int n, int k, const float *A, int lda, c
   FFMM_BLAS(rocblas_sgemm(ctx.blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero,
                           C, ldc));
 }
-
-static constexpr int64_t PIECE = 1 << 18;   // pairs per dense M2L GEMM
 
 void allocate_pass_buffers(Plan &plan) {
   OpCache &o = *plan.ops;
@@ -318,7 +386,7 @@ void allocate_pass_buffers(Plan &plan) {
   }
   plan.G.resize((size_t)Ns * std::max(nq, ng));
   int64_t pmax = 1;
-  for (auto &li : plan.lists) pmax = std::max(pmax, std::min<int64_t>(PIECE, li.np));
+  for (auto &li : plan.lists) pmax = std::max(pmax, li.chunk_max);
   if (plan.m2l == FFMM_M2L_DENSE && plan.precision == FFMM_PREC_MIXED) {
     // the mixed form: the class GEMMs in FP32 (FP32 MFMA), gathered from and scattered
     // into the FP64 expansions
@@ -416,29 +484,46 @@ void m2l_dense_level(Plan &plan, int l) {
   const double *reps = harm ? plan.hl[l]->reps.p : o.class_ops.p;
   const float *reps32 = harm ? plan.hl[l]->reps32.p : o.class_ops32.p;
   const bool f32 = plan.precision == FFMM_PREC_MIXED;
-  for (int c = 0; c < o.nclass; c++) {
-    const int64_t pa = li.class_pair[c], pb = li.class_pair[c + 1];
-    const int64_t r0 = li.class_run[c], nr = li.class_run[c + 1] - r0;
-    for (int64_t a = pa; a < pb; a += PIECE) {
-      const int64_t npk = std::min(PIECE, pb - a);
-      const unsigned gg = nblocks((int64_t)N * nb * npk, TB), gs = nblocks((int64_t)Ns * nr, TB);
-      if (f32) {
-        k_class_gather<float><<<gg, TB, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a, li.cfar.p + a, npk,
-                                                o.idx.p, o.sgn.p);
-        FFMM_LAUNCH_CHECK();
-        gemm32(ctx, N, (int)(nb * npk), N, reps32 + (size_t)c * N * N, N, plan.Gc32.p, N, plan.Yc32.p, N);
-        k_class_scatter<float><<<gs, TB, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0, nr, li.cfar.p, a,
-                                                 npk, o.inv.p, o.sgn_t.p, o.offs.p, alpha);
-      } else {
-        k_class_gather<double><<<gg, TB, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p + a, li.cfar.p + a, npk,
-                                                 o.idx.p, o.sgn.p);
-        FFMM_LAUNCH_CHECK();
-        gemm(ctx, N, (int)(nb * npk), N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p, N, 0.0, plan.Yc.p, N);
-        k_class_scatter<double><<<gs, TB, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.rtgt.p, li.rptr.p, r0, nr, li.cfar.p, a,
-                                                  npk, o.inv.p, o.sgn_t.p, o.offs.p, alpha);
-      }
-      FFMM_LAUNCH_CHECK();
+  const int nbN = N * nb;
+  const unsigned bt = (unsigned)std::min(1024, (nbN + 63) / 64 * 64);
+  const int64_t nch = (int64_t)li.chunk_t.size() - 1;
+  for (int64_t k = 0; k < nch; k++) {
+    ChunkArgs ca{};
+    ca.nclass = o.nclass;
+    for (int c = 0; c < o.nclass; c++) {
+      ca.cs[c] = li.chunk_cs[k * o.nclass + c];
+      ca.co[c + 1] = ca.co[c] + (li.chunk_cs[(k + 1) * o.nclass + c] - ca.cs[c]);
     }
+    const int32_t npk = ca.co[o.nclass];
+    const int32_t t0 = li.chunk_t[k], nt = li.chunk_t[k + 1] - t0;
+    if (npk == 0 || nt == 0) continue;
+    const unsigned gg = (unsigned)((npk + GATHER_PAIRS - 1) / GATHER_PAIRS);
+    if (f32) {
+      k_chunk_gather<float><<<gg, bt, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
+                                              ca);
+      FFMM_LAUNCH_CHECK();
+      for (int c = 0; c < o.nclass; c++) {
+        const int32_t n = ca.co[c + 1] - ca.co[c];
+        if (n == 0) continue;
+        const size_t off = (size_t)ca.co[c] * nbN;
+        gemm32(ctx, N, nb * n, N, reps32 + (size_t)c * N * N, N, plan.Gc32.p + off, N, plan.Yc32.p + off, N);
+      }
+      k_chunk_scatter<float><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
+                                                         li.cfar.p, o.ssel.p, o.offs.p, alpha, ca);
+    } else {
+      k_chunk_gather<double><<<gg, bt, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
+                                               ca);
+      FFMM_LAUNCH_CHECK();
+      for (int c = 0; c < o.nclass; c++) {
+        const int32_t n = ca.co[c + 1] - ca.co[c];
+        if (n == 0) continue;
+        const size_t off = (size_t)ca.co[c] * nbN;
+        gemm(ctx, N, nb * n, N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p + off, N, 0.0, plan.Yc.p + off, N);
+      }
+      k_chunk_scatter<double><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
+                                                          li.cfar.p, o.ssel.p, o.offs.p, alpha, ca);
+    }
+    FFMM_LAUNCH_CHECK();
   }
 }
 

@@ -50,7 +50,7 @@ size_t plan_bytes(const Plan &p) {
   s += cap_bytes(p.s2m_nodes) + cap_bytes(p.leaf_nodes);
   for (const auto &li : p.lists)
     s += cap_bytes(li.ptr) + cap_bytes(li.src) + cap_bytes(li.far) + cap_bytes(li.csrc) + cap_bytes(li.cfar) +
-         cap_bytes(li.rtgt) + cap_bytes(li.rptr);
+         cap_bytes(li.cstart);
   s += cap_bytes(p.Mp) + cap_bytes(p.Pl) + cap_bytes(p.Lc) + cap_bytes(p.Lc32) + cap_bytes(p.tau) + cap_bytes(p.tau32);
   s += cap_bytes(p.pM) + cap_bytes(p.tup) + cap_bytes(p.Mup) + cap_bytes(p.tM) + cap_bytes(p.G) + cap_bytes(p.tdn) +
        cap_bytes(p.Gc) + cap_bytes(p.Yc) + cap_bytes(p.Gc32) + cap_bytes(p.Yc32) + cap_bytes(p.pM32) + cap_bytes(p.B32) + cap_bytes(p.tM32);
@@ -74,7 +74,9 @@ static int64_t estimate_bytes(const OpCache &o, int64_t n, int ppl, int m2l, int
   double b = nodes * (2.0 * o.Ns * 8) + leaves * (o.Nl * 8.0) + 256.0 * n + 96.0 * nt;
   if (o.method == METHOD_CUBE) b += leaves * (o.Nl + 3.0 * o.Ns) * 8;
   if (m2l == FFMM_M2L_SPECTRAL) b += 0.6 * nodes * 2304 * 16;
-  if (m2l == FFMM_M2L_DENSE) b += 2.0 * o.Ns * 8 * std::min<double>(1 << 18, 200.0 * nodes);
+  // dense class M2L: the class-major pair table, and the gathered columns and products of
+  // one target chunk (at most about 2 GiB)
+  if (m2l == FFMM_M2L_DENSE) b += 4.0 * o.nclass * nodes + std::min<double>(2.2e9, 2.0 * o.Ns * 8 * 200.0 * nodes);
   return (int64_t)b + (64ll << 20);
 }
 
