@@ -254,11 +254,15 @@ constexpr int SCATTER_MAXP = 320; // far pairs of one target: at most the 316 fa
 
 // Gc[b N + i, q] = sgn[f, i] Mp[b N + idx[f, i], src]  (P_f^T m per block b) for the pairs
 // of a chunk, all classes in one launch; one workgroup per GATHER_PAIRS columns, one lane
-// per entry of a column, the pairs' loads independent of each other
+// per coefficient i with all nb blocks, the pairs' loads independent of each other.  For
+// Stokes (nb = 4) the fourth block takes the coupling Psi += o_f . Phi of the translation
+// (src/device_passes.jl:465-468, src/operators/harmonics.jl:604-628) before the product:
+// the class operator is linear and the same for every block, so T (G_3 + o_f . (G_0, G_1,
+// G_2)) is the coupled fourth block of the output.
 template <class T>
 __global__ void k_chunk_gather(T *__restrict__ Gc, const double *__restrict__ Mp, int Ns, int N, int nb,
                                const int32_t *__restrict__ csrc, const int32_t *__restrict__ cfar, int32_t npk,
-                               const int32_t *__restrict__ gsel, ChunkArgs ca) {
+                               const int32_t *__restrict__ gsel, const double *__restrict__ offs, ChunkArgs ca) {
   __shared__ int32_t pf[GATHER_PAIRS], ps[GATHER_PAIRS];
   const int nbN = nb * N;
   const int32_t q0 = (int32_t)blockIdx.x * GATHER_PAIRS;
@@ -267,18 +271,28 @@ __global__ void k_chunk_gather(T *__restrict__ Gc, const double *__restrict__ Mp
     const int32_t q = q0 + (int32_t)threadIdx.x;
     const int c = chunk_class(ca, q);
     const int32_t p = ca.cs[c] + (q - ca.co[c]);
-    pf[threadIdx.x] = cfar[p] * N;
+    pf[threadIdx.x] = cfar[p];
     ps[threadIdx.x] = csrc[p];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < nbN; e += blockDim.x) {
-    const int b = e / N, i = e - b * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < GATHER_PAIRS; k++) {
       if (k < nq) {
-        const int32_t g = gsel[pf[k] + i];
-        const double v = Mp[(int64_t)ps[k] * Ns + b * N + (g & 0x7fffffff)];
-        Gc[(int64_t)(q0 + k) * nbN + e] = (T)(g < 0 ? -v : v);
+        const int32_t f = pf[k];
+        const int32_t g = gsel[f * N + i];
+        const double *__restrict__ m = Mp + (int64_t)ps[k] * Ns + (g & 0x7fffffff);
+        T *__restrict__ o = Gc + (int64_t)(q0 + k) * nbN + i;
+        const double sg = g < 0 ? -1.0 : 1.0;
+        if (nb == 4) {
+          const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N];
+          o[0] = (T)(sg * m0);
+          o[N] = (T)(sg * m1);
+          o[2 * N] = (T)(sg * m2);
+          o[3 * N] = (T)(sg * (m3 + offs[3 * f] * m0 + offs[3 * f + 1] * m1 + offs[3 * f + 2] * m2));
+        } else {
+          for (int b = 0; b < nb; b++) o[b * N] = (T)(sg * m[b * N]);
+        }
       }
     }
   }
@@ -286,16 +300,16 @@ __global__ void k_chunk_gather(T *__restrict__ Gc, const double *__restrict__ Mp
 
 // Target-owned scatter of the chunk's products, all classes at once: one workgroup per
 // target t, one lane per entry (b, j); pL[b N + j, t] += alpha sum over the target's pairs
-// (class by class, in pair order) of sgn_t[f, i] Y_b[i, q], i = inv[f, j].  For Stokes
-// (nb = 4) the fourth block adds the coupling Psi += o_f . Phi with the pair's own offset
-// (src/device_passes.jl:449-473, src/operators/harmonics.jl:604-628).  The target's pairs
+// (class by class, in pair order) of sgn_t[f, i] Y_b[i, q], i = inv[f, j]
+// (src/device_passes.jl:449-473; the Stokes coupling is already in the gathered fourth
+// block).  The target's pairs
 // (offset, chunk column) are staged in LDS first, so that each lane's loop over them
 // issues independent loads.
 template <class T>
 __global__ void k_chunk_scatter(double *__restrict__ pL, const T *__restrict__ Yc, int Ns, int N, int nb,
                                 const int32_t *__restrict__ cstart, int64_t nl, int32_t t0,
-                                const int32_t *__restrict__ cfar, const int32_t *__restrict__ ssel,
-                                const double *__restrict__ offs, double alpha, ChunkArgs ca) {
+                                const int32_t *__restrict__ cfar, const int32_t *__restrict__ ssel, double alpha,
+                                ChunkArgs ca) {
   __shared__ int32_t pf[SCATTER_MAXP], pq[SCATTER_MAXP];
   __shared__ int32_t npair;
   const int32_t t = t0 + (int32_t)blockIdx.x;
@@ -333,10 +347,7 @@ __global__ void k_chunk_scatter(double *__restrict__ pL, const T *__restrict__ Y
       const int32_t f = pf[k];
       const int32_t sv = ssel[f * N + j];
       const int i = sv & 0x7fffffff;
-      const T *__restrict__ Y = Yc + (int64_t)pq[k] * nbN;
-      double y = (double)Y[b * N + i];
-      if (nb == 4 && b == 3)
-        y = y + offs[3 * f] * (double)Y[i] + offs[3 * f + 1] * (double)Y[N + i] + offs[3 * f + 2] * (double)Y[2 * N + i];
+      const double y = (double)Yc[(int64_t)pq[k] * nbN + b * N + i];
       acc += sv < 0 ? -y : y;
     }
     pL[(int64_t)t * Ns + e] += alpha * acc;
@@ -485,7 +496,8 @@ void m2l_dense_level(Plan &plan, int l) {
   const float *reps32 = harm ? plan.hl[l]->reps32.p : o.class_ops32.p;
   const bool f32 = plan.precision == FFMM_PREC_MIXED;
   const int nbN = N * nb;
-  const unsigned bt = (unsigned)std::min(1024, (nbN + 63) / 64 * 64);
+  const unsigned bt = (unsigned)std::min(1024, (nbN + 63) / 64 * 64);   // scatter: a lane per entry
+  const unsigned btg = (unsigned)std::min(1024, (N + 63) / 64 * 64);    // gather: a lane per coefficient
   const int64_t nch = (int64_t)li.chunk_t.size() - 1;
   for (int64_t k = 0; k < nch; k++) {
     ChunkArgs ca{};
@@ -499,8 +511,8 @@ void m2l_dense_level(Plan &plan, int l) {
     if (npk == 0 || nt == 0) continue;
     const unsigned gg = (unsigned)((npk + GATHER_PAIRS - 1) / GATHER_PAIRS);
     if (f32) {
-      k_chunk_gather<float><<<gg, bt, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
-                                              ca);
+      k_chunk_gather<float><<<gg, btg, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
+                                              o.offs.p, ca);
       FFMM_LAUNCH_CHECK();
       for (int c = 0; c < o.nclass; c++) {
         const int32_t n = ca.co[c + 1] - ca.co[c];
@@ -509,10 +521,10 @@ void m2l_dense_level(Plan &plan, int l) {
         gemm32(ctx, N, nb * n, N, reps32 + (size_t)c * N * N, N, plan.Gc32.p + off, N, plan.Yc32.p + off, N);
       }
       k_chunk_scatter<float><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
-                                                         li.cfar.p, o.ssel.p, o.offs.p, alpha, ca);
+                                                         li.cfar.p, o.ssel.p, alpha, ca);
     } else {
-      k_chunk_gather<double><<<gg, bt, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
-                                               ca);
+      k_chunk_gather<double><<<gg, btg, 0, s>>>(plan.Gc.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
+                                               o.offs.p, ca);
       FFMM_LAUNCH_CHECK();
       for (int c = 0; c < o.nclass; c++) {
         const int32_t n = ca.co[c + 1] - ca.co[c];
@@ -521,7 +533,7 @@ void m2l_dense_level(Plan &plan, int l) {
         gemm(ctx, N, nb * n, N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p + off, N, 0.0, plan.Yc.p + off, N);
       }
       k_chunk_scatter<double><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
-                                                          li.cfar.p, o.ssel.p, o.offs.p, alpha, ca);
+                                                          li.cfar.p, o.ssel.p, alpha, ca);
     }
     FFMM_LAUNCH_CHECK();
   }
