@@ -355,6 +355,7 @@ void sort_pairs_u32(Context &ctx, const uint32_t *kin, uint32_t *kout, const int
                     int64_t n, int end_bit);
 // out[i] = sum of in[0..i), i < n; returns the total (blocking read back)
 int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_t n);
+void exclusive_scan_i32_async(Context &ctx, const int32_t *in, int32_t *out, int64_t n, int slot);
 // indices i (ascending) with flag[i] != 0; returns their number (blocking read back)
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n);
 // the count into the context's pinned slot `slot` (0 .. 63) by the stream, no synchronisation

@@ -164,10 +164,13 @@ void build_lists(Plan &plan) {
   TreePtrs t{plan.cell.p, plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p, plan.has_source.p, plan.nnodes};
   plan.lists.clear();
   plan.lists.resize(plan.L);
+  static_assert(MAX_KEY_LEVELS + 1 <= NPINNED, "a pinned slot per level");
+  // the pair counts and their scans, all levels queued at once: one read back
   DBuf<int32_t> cnt, ccnt, bounds;
+  cnt.resize(std::max<int64_t>(1, plan.nnodes));
   for (int l = 0; l < plan.L; l++) {
     LevelLists &li = plan.lists[l];
-    int64_t lo = plan.lo[l], nl = plan.nl[l];
+    const int64_t lo = plan.lo[l], nl = plan.nl[l];
     li.class_pair.assign(nclass + 1, 0);
     li.chunk_t.clear();
     li.chunk_cs.clear();
@@ -178,25 +181,43 @@ void build_lists(Plan &plan) {
       li.filled = true;
       continue;
     }
-    cnt.resize(nl);
-    k_pair_count<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, cnt.p);
+    k_pair_count<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, cnt.p + lo);
     FFMM_LAUNCH_CHECK();
-    int64_t np = exclusive_scan_i32(ctx, cnt.p, li.ptr.p, nl);
+    exclusive_scan_i32_async(ctx, cnt.p + lo, li.ptr.p, nl, l);
+  }
+  ctx.sync();
+  for (int l = 2; l < plan.L; l++) {
+    const int64_t np = ctx.pinned[l];
     if (np >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many far pairs on one level for 32-bit pair indices");
-    k_set_i32<<<1, 1, 0, s>>>(li.ptr.p + nl, (int32_t)np);
-    li.np = np;
-    // the pairs themselves: the spectral M2L reads only the counts (its Hadamard works
-    // from the colleague tables), so they are written on demand (fill_level_pairs)
-    if (plan.m2l != FFMM_M2L_DENSE) continue;
+    plan.lists[l].np = np;
+  }
+  // the pairs themselves: the spectral M2L reads only the counts (its Hadamard works
+  // from the colleague tables), so they are written on demand (fill_level_pairs)
+  if (plan.m2l != FFMM_M2L_DENSE) return;
+  // the class lists (only the dense class translations read them) and the target chunks
+  // of all levels, queued at once: one read back of the chunk bounds
+  const int64_t cap = m2l_chunk_pairs(plan);
+  const int64_t nrow = 1 + nclass;
+  std::vector<int64_t> boff(plan.L + 1, 0);
+  int64_t nemax = 1;
+  for (int l = 0; l < plan.L; l++) {
+    const int64_t np = l < 2 ? 0 : plan.lists[l].np;
+    const int64_t nch = (np + cap - 1) / cap;
+    boff[l + 1] = boff[l] + (l < 2 ? 0 : (nch + 1) * nrow);
+    nemax = std::max<int64_t>(nemax, (int64_t)nclass * plan.nl[l]);
+  }
+  ccnt.resize(nemax);
+  bounds.resize(std::max<int64_t>(1, boff[plan.L]));
+  for (int l = 2; l < plan.L; l++) {
+    LevelLists &li = plan.lists[l];
+    const int64_t nl = plan.nl[l], np = li.np;
     fill_level_pairs(plan, l);
-    // the class lists (only the dense class translations read them)
-    int64_t ne = (int64_t)nclass * nl;
-    ccnt.resize(ne);
+    const int64_t ne = (int64_t)nclass * nl;
     li.cstart.resize(ne + 1);
     k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ops.far_class.p, ccnt.p);
     FFMM_LAUNCH_CHECK();
-    exclusive_scan_i32(ctx, ccnt.p, li.cstart.p, ne);
-    k_set_i32<<<1, 1, 0, s>>>(li.cstart.p + ne, (int32_t)np);
+    // the class-major scan; its total is np (slot 0 is free here)
+    exclusive_scan_i32_async(ctx, ccnt.p, li.cstart.p, ne, 0);
     li.csrc.resize(np);
     li.cfar.resize(np);
     if (np > 0) {
@@ -204,21 +225,23 @@ void build_lists(Plan &plan) {
                                                   ops.far_class.p, li.csrc.p, li.cfar.p);
       FFMM_LAUNCH_CHECK();
     }
-    // the target chunks and their class offsets, one read back
-    const int64_t cap = m2l_chunk_pairs(plan);
     const int64_t nch = (np + cap - 1) / cap;
-    const int64_t nrow = 1 + nclass;
-    bounds.resize((nch + 1) * nrow);
-    k_chunk_bounds<<<nblocks(nch + 1, 64), 64, 0, s>>>(li.ptr.p, nl, cap, nch, li.cstart.p, nclass, bounds.p);
+    k_chunk_bounds<<<nblocks(nch + 1, 64), 64, 0, s>>>(li.ptr.p, nl, cap, nch, li.cstart.p, nclass,
+                                                       bounds.p + boff[l]);
     FFMM_LAUNCH_CHECK();
-    std::vector<int32_t> hb((size_t)((nch + 1) * nrow));
-    bounds.download(hb.data(), hb.size(), s);
-    ctx.sync();
+  }
+  std::vector<int32_t> hb((size_t)std::max<int64_t>(1, boff[plan.L]));
+  bounds.download(hb.data(), (size_t)boff[plan.L], s);
+  ctx.sync();
+  for (int l = 2; l < plan.L; l++) {
+    LevelLists &li = plan.lists[l];
+    const int64_t nch = (li.np + cap - 1) / cap;
+    const int32_t *h = hb.data() + boff[l];
     li.chunk_t.resize(nch + 1);
     li.chunk_cs.resize((size_t)((nch + 1) * nclass));
     for (int64_t k = 0; k <= nch; k++) {
-      li.chunk_t[k] = hb[k * nrow];
-      for (int c = 0; c < nclass; c++) li.chunk_cs[k * nclass + c] = hb[k * nrow + 1 + c];
+      li.chunk_t[k] = h[k * nrow];
+      for (int c = 0; c < nclass; c++) li.chunk_cs[k * nclass + c] = h[k * nrow + 1 + c];
     }
     li.chunk_max = 0;
     for (int64_t k = 0; k < nch; k++) {
@@ -226,10 +249,9 @@ void build_lists(Plan &plan) {
       for (int c = 0; c < nclass; c++) n += li.chunk_cs[(k + 1) * nclass + c] - li.chunk_cs[k * nclass + c];
       li.chunk_max = std::max(li.chunk_max, n);
     }
-    for (int c = 0; c < nclass; c++) li.class_pair[c] = hb[1 + c];
-    li.class_pair[nclass] = np;
+    for (int c = 0; c < nclass; c++) li.class_pair[c] = h[1 + c];
+    li.class_pair[nclass] = li.np;
   }
-  ctx.sync();
 }
 
 }  // namespace ffmm

@@ -59,6 +59,31 @@ int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_
   return ctx.pinned[0];
 }
 
+// lane 0 of one wave: the total of an exclusive scan into out[n] and into pinned slot host[0]
+__global__ void k_scan_total_slot(int32_t *__restrict__ out, const int32_t *__restrict__ in, int64_t n,
+                                  int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) {
+    const int64_t t = n > 0 ? (int64_t)out[n - 1] + (int64_t)in[n - 1] : 0;
+    out[n] = (int32_t)t;
+    host[0] = t;
+  }
+}
+
+// the exclusive scan of in[0, n) into out[0, n], out[n] = the total (32-bit), and the total
+// (64-bit) into pinned readback slot `slot`: no synchronisation
+void exclusive_scan_i32_async(Context &ctx, const int32_t *in, int32_t *out, int64_t n, int slot) {
+  if (n > 0) {
+    size_t bytes = 0;
+    FFMM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, int32_t(0), (size_t)n, rocprim::plus<int32_t>(),
+                                     ctx.stream));
+    void *t = scratch(ctx, bytes);
+    FFMM_HIP(rocprim::exclusive_scan(t, bytes, in, out, int32_t(0), (size_t)n, rocprim::plus<int32_t>(),
+                                     ctx.stream));
+  }
+  k_scan_total_slot<<<1, 64, 0, ctx.stream>>>(out, in, n, ctx.pinned_dev + slot);
+  FFMM_HIP(hipGetLastError());
+}
+
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n) {
   if (n <= 0) return 0;
   rocprim::counting_iterator<int32_t> it(0);
