@@ -258,11 +258,12 @@ int ffmm_opcache_destroy(ffmm_opcache cache) {
 }
 
 // the permutations of the class M2L with their signs folded into the top bit (signs are
-// +1 or -1): gsel[f, i] = idx[f, i] | sign(sgn[f, i]); ssel[f, j] = inv[f, j] | sign(sgn_t[f, inv[f, j]])
+// +1 or -1): gsel[f, i] = idx[f, i] | sign(sgn[f, i]); ssel[f, j] = inv[f, j] | sign(sgn_t[f, inv[f, j]]);
+// fsel[f, i] = j | sign(sgn_t[f, i]) for inv[f, j] = i
 static void upload_signed_tables(OpCache &o, const int32_t *idx, const int8_t *sgn, const int32_t *inv,
                                  const int8_t *sgn_t, int N, hipStream_t s) {
   const size_t n = (size_t)NFAR * N;
-  std::vector<int32_t> g(n), t(n);
+  std::vector<int32_t> g(n), t(n), fw(n);
   for (int f = 0; f < NFAR; f++)
     for (int i = 0; i < N; i++) {
       const size_t e = (size_t)f * N + i;
@@ -271,9 +272,11 @@ static void upload_signed_tables(OpCache &o, const int32_t *idx, const int8_t *s
       g[e] = idx[e] | (sgn[e] < 0 ? INT32_MIN : 0);
       const int32_t k = inv[e];
       t[e] = k | (sgn_t[(size_t)f * N + k] < 0 ? INT32_MIN : 0);
+      fw[(size_t)f * N + k] = i | (sgn_t[(size_t)f * N + k] < 0 ? INT32_MIN : 0);
     }
   o.gsel.upload(g.data(), n, s);
   o.ssel.upload(t.data(), n, s);
+  o.fsel.upload(fw.data(), n, s);
   FFMM_HIP(hipStreamSynchronize(s));   // before the host tables go
 }
 

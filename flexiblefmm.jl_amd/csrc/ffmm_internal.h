@@ -260,6 +260,8 @@ struct OpCache {
   // the same permutations with their signs in the top bit: gsel[f, i] = idx[f, i] (sign
   // sgn[f, i]); ssel[f, j] = i = inv[f, j] (sign sgn_t[f, i]); one load per entry
   DBuf<int32_t> gsel, ssel;
+  // fsel[f, i] = the target-frame position j of product entry i (inv[f, j] = i), sign sgn_t[f, i]
+  DBuf<int32_t> fsel;
   DBuf<int8_t> sgn;
   DBuf<double> tables;                       // 316 x nfreq complex (interleaved)
   DBuf<float> tables32;

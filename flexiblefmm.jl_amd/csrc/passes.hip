@@ -236,6 +236,7 @@ struct ChunkArgs {
   int nclass;
   int32_t co[MAX_CLASSES + 1];
   int32_t cs[MAX_CLASSES];
+  int32_t to[MAX_CLASSES + 1];   // the fused kernel: first column tile of each class
 };
 
 // the class of chunk column q: the last c with co[c] <= q
@@ -295,6 +296,176 @@ __global__ void k_chunk_gather(T *__restrict__ Gc, const double *__restrict__ Mp
         }
       }
     }
+  }
+}
+
+typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
+constexpr int FM_TB = 512;      // 8 waves: wave w holds the operator's row tile w (rows 16 w .. 16 w + 15)
+constexpr int FM_COLS = 128;    // product columns per tile: 8 column tiles of 16
+constexpr int FM_MP = 128;      // operator rows and K, padded: N <= 128
+constexpr int FM_BS = 132;      // LDS stride of a gathered column (16-byte aligned, K contiguous)
+
+// The class M2L of the harmonic methods in the mixed form, fused: for each tile of a
+// class's pairs in the chunk, the gathered source expansions (P_f^T m per block, signed,
+// the Stokes coupling folded into the fourth block, src/device_passes.jl:449-473) go
+// straight into LDS as FP32 and are multiplied by the class operator on FP32 MFMA
+// (v_mfma_f32_16x16x4f32).  Each wave keeps its 16 operator rows in registers; the four
+// K slots of an MFMA step take four consecutive k, so a lane's operands for 16 k come
+// in one 16-byte load.  The products are written in each pair's target frame (entry j =
+// fsel position, sign applied), so the scatter adds contiguous columns.  Workgroups take
+// contiguous runs of tiles, so the operator rows are loaded again only where the class
+// changes.  N = (p + 1)^2 <= 128.
+template <int NB>
+__global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4))) k_class_m2l32(float *__restrict__ Yc, const double *__restrict__ Mp,
+                                                          int Ns, int N, const float *__restrict__ reps32,
+                                                          const int32_t *__restrict__ csrc,
+                                                          const int32_t *__restrict__ cfar,
+                                                          const int32_t *__restrict__ gsel,
+                                                          const int32_t *__restrict__ fsel,
+                                                          const double *__restrict__ offs, ChunkArgs ca,
+                                                          int32_t ntiles) {
+  constexpr int PPT = FM_COLS / NB;   // pairs per tile
+  __shared__ float Bs[FM_COLS * FM_BS];
+  __shared__ int32_t pf[PPT], ps[PPT];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nkb = (N + 15) >> 4;      // K blocks of 16
+  const bool rows = 16 * w < N;       // this wave's row tile holds operator rows
+  const int32_t t0 = (int32_t)((int64_t)blockIdx.x * ntiles / gridDim.x);
+  const int32_t t1 = (int32_t)((int64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
+  // the K padding of the gathered columns: zero (the gather and the staging write k < N only)
+  const int kpad = 16 * nkb - N;
+  for (int e = tid; e < FM_COLS * kpad; e += FM_TB) Bs[(e / kpad) * FM_BS + N + e % kpad] = 0.f;
+  // operator row 16 w + li, k = 16 kb + 4 lk + s, for kb < 8, s < 4
+  fm_f32x4 a4[8];
+  int cur = -1;
+  for (int32_t tile = t0; tile < t1; tile++) {
+    int c = 0, h = ca.nclass;
+    while (h - c > 1) {
+      const int m = (c + h) >> 1;
+      if (ca.to[m] <= tile) c = m;
+      else h = m;
+    }
+    const int32_t lp0 = (tile - ca.to[c]) * PPT;                       // first class-local pair
+    const int npt = min(PPT, ca.co[c + 1] - ca.co[c] - lp0);           // pairs in the tile
+    if (c != cur) {
+      const float *__restrict__ A = reps32 + (size_t)c * N * N;
+      const int i = 16 * w + li;
+#pragma unroll
+      for (int kb = 0; kb < 8; kb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int k = 16 * kb + 4 * lk + q;
+          a4[kb][q] = (i < N && k < N) ? A[i + (size_t)N * k] : 0.f;
+        }
+      cur = c;
+    }
+    if (tid < npt) {
+      const int32_t p = ca.cs[c] + lp0 + tid;
+      pf[tid] = cfar[p];
+      ps[tid] = csrc[p];
+    }
+    __syncthreads();
+    // gather: one item per (pair, coefficient), all blocks
+    for (int e = tid; e < npt * N; e += FM_TB) {
+      const int pl = e / N, k = e - pl * N;
+      const int32_t f = pf[pl];
+      const int32_t g = gsel[f * N + k];
+      const double *__restrict__ m = Mp + (int64_t)ps[pl] * Ns + (g & 0x7fffffff);
+      const double sg = g < 0 ? -1.0 : 1.0;
+      if (NB == 4) {
+        const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N];
+        float *__restrict__ b = Bs + (pl * 4) * FM_BS + k;
+        b[0] = (float)(sg * m0);
+        b[FM_BS] = (float)(sg * m1);
+        b[2 * FM_BS] = (float)(sg * m2);
+        b[3 * FM_BS] = (float)(sg * (m3 + offs[3 * f] * m0 + offs[3 * f + 1] * m1 + offs[3 * f + 2] * m2));
+      } else {
+        Bs[pl * FM_BS + k] = (float)(sg * m[0]);
+      }
+    }
+    __syncthreads();
+    // products: wave w, row tile w, all 8 column tiles
+    fm_f32x4 acc[8];
+#pragma unroll
+    for (int ct = 0; ct < 8; ct++) acc[ct] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
+    if (rows) {
+      const float *__restrict__ bb = Bs + li * FM_BS + 4 * lk;
+#pragma unroll
+      for (int kb = 0; kb < 8; kb++) {
+        if (kb < nkb) {
+#pragma unroll
+          for (int ct = 0; ct < 8; ct++) {
+            const fm_f32x4 b4 = *reinterpret_cast<const fm_f32x4 *>(bb + 16 * ct * FM_BS + 16 * kb);
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[kb][q], b4[q], acc[ct], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();   // every wave's reads of Bs are done: Bs becomes the staging of the products
+    if (rows) {
+#pragma unroll
+      for (int ct = 0; ct < 8; ct++) {
+        const int col = 16 * ct + li, pl = col / NB;
+        if (pl < npt) {
+          const int32_t f = pf[pl];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int i = 16 * w + 4 * lk + r;
+            if (i < N) {
+              const int32_t fs = fsel[f * N + i];
+              Bs[col * FM_BS + (fs & 0x7fffffff)] = fs < 0 ? -acc[ct][r] : acc[ct][r];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // coalesced copy of the tile's columns: pair column q = co[c] + lp0 + pl, NB N entries
+    float *__restrict__ out = Yc + ((int64_t)ca.co[c] + lp0) * NB * N;
+    for (int e = tid; e < npt * NB * N; e += FM_TB) {
+      const int col = e / N, j = e - col * N;
+      out[e] = Bs[col * FM_BS + j];
+    }
+    __syncthreads();   // before the next tile's pair list and gather (the K padding is never written)
+  }
+}
+
+// The scatter of products already in the target frame: pL[e, t] += alpha sum over the
+// target's pairs (class by class, in pair order) of Y'[e, q]
+__global__ void k_chunk_scatter_framed(double *__restrict__ pL, const float *__restrict__ Yc, int Ns, int nbN,
+                                       const int32_t *__restrict__ cstart, int64_t nl, int32_t t0, double alpha,
+                                       ChunkArgs ca) {
+  __shared__ int32_t pq[SCATTER_MAXP];
+  __shared__ int32_t npair;
+  const int32_t t = t0 + (int32_t)blockIdx.x;
+  if (threadIdx.x < 64) {
+    const int c = threadIdx.x;
+    int32_t p0 = 0, n = 0;
+    if (c < ca.nclass) {
+      const int64_t ec = (int64_t)c * nl + t;
+      p0 = cstart[ec];
+      n = cstart[ec + 1] - p0;
+    }
+    int32_t v = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t u = __shfl_up(v, off, 64);
+      if (c >= off) v += u;
+    }
+    const int32_t start = v - n;
+    const int32_t qoff = c < ca.nclass ? ca.co[c] - ca.cs[c] : 0;
+    for (int32_t k = 0; k < n; k++) pq[start + k] = p0 + k + qoff;
+    if (c == 63) npair = v;
+  }
+  __syncthreads();
+  const int32_t np = npair;
+  for (int e = threadIdx.x; e < nbN; e += blockDim.x) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int32_t k = 0; k < np; k++) acc += (double)Yc[(int64_t)pq[k] * nbN + e];
+    pL[(int64_t)t * Ns + e] += alpha * acc;
   }
 }
 
@@ -499,6 +670,9 @@ void m2l_dense_level(Plan &plan, int l) {
   const unsigned bt = (unsigned)std::min(1024, (nbN + 63) / 64 * 64);   // scatter: a lane per entry
   const unsigned btg = (unsigned)std::min(1024, (N + 63) / 64 * 64);    // gather: a lane per coefficient
   const int64_t nch = (int64_t)li.chunk_t.size() - 1;
+  // the harmonic methods in the mixed form with at most 128 coefficients per block: the
+  // fused gather + product kernel
+  const bool fused = f32 && harm && N <= FM_MP && (nb == 1 || nb == 4);
   for (int64_t k = 0; k < nch; k++) {
     ChunkArgs ca{};
     ca.nclass = o.nclass;
@@ -509,6 +683,23 @@ void m2l_dense_level(Plan &plan, int l) {
     const int32_t npk = ca.co[o.nclass];
     const int32_t t0 = li.chunk_t[k], nt = li.chunk_t[k + 1] - t0;
     if (npk == 0 || nt == 0) continue;
+    if (fused) {
+      const int ppt = FM_COLS / nb;
+      for (int c = 0; c < o.nclass; c++) ca.to[c + 1] = ca.to[c] + (ca.co[c + 1] - ca.co[c] + ppt - 1) / ppt;
+      const int32_t ntiles = ca.to[o.nclass];
+      const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 4 * 256);
+      if (nb == 4)
+        k_class_m2l32<4><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32, li.csrc.p, li.cfar.p, o.gsel.p,
+                                                o.fsel.p, o.offs.p, ca, ntiles);
+      else
+        k_class_m2l32<1><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32, li.csrc.p, li.cfar.p, o.gsel.p,
+                                                o.fsel.p, o.offs.p, ca, ntiles);
+      FFMM_LAUNCH_CHECK();
+      k_chunk_scatter_framed<<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, nbN, li.cstart.p, plan.nl[l], t0, alpha,
+                                                         ca);
+      FFMM_LAUNCH_CHECK();
+      continue;
+    }
     const unsigned gg = (unsigned)((npk + GATHER_PAIRS - 1) / GATHER_PAIRS);
     if (f32) {
       k_chunk_gather<float><<<gg, btg, 0, s>>>(plan.Gc32.p, plan.Mp.p, Ns, N, nb, li.csrc.p, li.cfar.p, npk, o.gsel.p,
