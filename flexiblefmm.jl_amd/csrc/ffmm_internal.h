@@ -12,6 +12,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <cstdlib>
 
 #include "../../include/ffmm.h"
 
@@ -374,12 +375,19 @@ void node_leafslots(Plan &plan);
 void build_lists(Plan &plan);
 void fill_level_pairs(Plan &plan, int l);   // src/far of level l, when not yet written
 // far pairs per target chunk of the dense class M2L: its gathered columns and products
-// (two buffers of one column per pair and block) take about 2 GiB
+// (two buffers of one column per pair and block) take about 2 GiB.  The environment
+// variable FFMM_M2L_CHUNK_PAIRS, read when a plan's lists are built, sets a smaller cap
+// (at least 64 pairs), to bound the memory of many resident plans.
 inline int64_t m2l_chunk_pairs(const Plan &p) {
   const OpCache &o = *p.ops;
   const int64_t col = o.method == METHOD_HARMONIC ? (int64_t)o.nb * o.hN : (int64_t)o.Ns;
   const int64_t bytes = 2 * col * (p.precision == FFMM_PREC_MIXED ? 4 : 8);
-  return std::max<int64_t>(4096, (2ll << 30) / bytes);
+  int64_t cap = std::max<int64_t>(4096, (2ll << 30) / bytes);
+  if (const char *e = std::getenv("FFMM_M2L_CHUNK_PAIRS")) {
+    const long long v = std::atoll(e);
+    if (v > 0) cap = std::max<int64_t>(64, std::min<int64_t>(cap, v));
+  }
+  return cap;
 }
 
 // ---- passes (passes.hip) ----

@@ -167,3 +167,41 @@ def test_device_stokes_direct_sum(nd):
     assert rel(ffmm.direct(K, s, d, t), HO.direct("stokes", s, d, t)) < 1e-13
     rows = np.arange(0, 6000, 17)
     assert rel(ffmm.direct_self(K, s, d, rows=rows), HO.direct("stokes", s, d, s[rows], skip=rows)) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["stokes-mixed", "yukawa-mixed", "laplace-f64", "cube-mixed"])
+def test_class_m2l_chunks(case, monkeypatch):
+    """The dense class M2L runs by target chunks (all classes of a chunk at once; in the
+    mixed harmonic form, the fused gather + product kernel in tiles of 128 columns).
+    A plan whose chunks are capped at 100 pairs (FFMM_M2L_CHUNK_PAIRS, read at plan
+    creation) gives the result of the single-chunk plan: each pair's product is formed
+    alone and each target sums its pairs in the same order (rel 1e-12).  The cube
+    method's mixed form runs its FP32 class GEMMs in the library, which picks other
+    kernels (another K order) for other column counts; the DC2E pseudo-inverse amplifies
+    that FP32 rounding, so that case is held to the mixed form's 1e-6."""
+    import ffmm
+    rng = np.random.default_rng(16)
+    n = 6000
+    pts = mixed(rng, n)
+    kw = dict(ppl=40, bbox=BOX)
+    if case == "stokes-mixed":
+        m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=10)
+        dens = rng.standard_normal((n, 9))
+        dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+        out, kw["precision"] = ffmm.Velocity, "mixed"
+    elif case == "yukawa-mixed":
+        m = ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10)
+        dens, out, kw["precision"] = rng.standard_normal(n), ffmm.PotentialGradient, "mixed"
+    elif case == "laplace-f64":
+        m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=8)
+        dens, out = rng.standard_normal(n), ffmm.PotentialGradient
+    else:
+        m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+        dens, out = rng.standard_normal(n), ffmm.PotentialGradient
+        kw.update(precision="mixed", m2l="dense")
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] class M2L chunks {case}", flush=True)
+    one = ffmm.DevicePlan(m, pts, cache=cache, **kw).matvec(dens, out).cpu().numpy()
+    monkeypatch.setenv("FFMM_M2L_CHUNK_PAIRS", "100")
+    many = ffmm.DevicePlan(m, pts, cache=cache, **kw).matvec(dens, out).cpu().numpy()
+    assert rel(many, one) < (1e-6 if case == "cube-mixed" else 1e-12)
