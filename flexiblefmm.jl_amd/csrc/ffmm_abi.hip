@@ -554,11 +554,10 @@ int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has) {
 // ---- the plan's life cycle (shared by the C entry points and the server) ----
 namespace ffmm {
 
-void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
-               double width, int e0, int max_levels, int precision, int m2l) {
+// the checks of a plan's parameters against its cache (shared by creation and restore)
+static void plan_check(Context &ctx, OpCache &o, int64_t n, int ppl, double width, int e0, int precision, int m2l) {
   if (!o.ready) fail(FFMM_ERR_STATE, "the operator cache holds no operators");
   if (n <= 0) fail(FFMM_ERR_ARGUMENT, "no points");
-  if (!xyz) fail(FFMM_ERR_ARGUMENT, "null points");
   if (ppl < 1) fail(FFMM_ERR_ARGUMENT, "ppl must be positive");
   if (!(width > 0)) fail(FFMM_ERR_ARGUMENT, "box width must be positive");
   if (precision != FFMM_PREC_F64 && precision != FFMM_PREC_MIXED) fail(FFMM_ERR_ARGUMENT, "unknown precision");
@@ -571,8 +570,10 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   if (m2l == FFMM_M2L_SPECTRAL && !o.has_tables) fail(FFMM_ERR_ARGUMENT, "the cache holds no spectral tables");
   if (std::ldexp(o.w_ref, e0) != width) fail(FFMM_ERR_ARGUMENT, "the box width is not w_ref * 2^e0");
   if (ctx.device != o.ctx->device) fail(FFMM_ERR_ARGUMENT, "the cache lives on another device");
-  set_device(ctx);
-  AllocScope scope(ctx);
+}
+
+static void plan_fields(Plan &p, Context &ctx, OpCache &o, int64_t n, int ppl, const double origin[3], double width,
+                        int e0, int max_levels, int precision, int m2l) {
   p.ctx = &ctx;
   p.ops = &o;
   p.N = n;
@@ -584,12 +585,15 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   p.W0 = width;
   p.e0 = e0;
   for (auto &e : p.ev) FFMM_HIP(hipEventCreate(&e));
-  {
-    StageTimer tm(p, 0);
-    build_tree(p, xyz);
-  }
+}
+
+// everything after the tree: level scales, the harmonic levels, node slots, lists, pass
+// buffers, the spectral tiles
+static void plan_finish(Plan &p) {
+  Context &ctx = *p.ctx;
+  OpCache &o = *p.ops;
   p.lam.resize(p.L);
-  for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, e0 - l);
+  for (int l = 0; l < p.L; l++) p.lam[l] = std::ldexp(1.0, p.e0 - l);
   p.lam_d.upload(p.lam.data(), p.L, ctx.stream);
   if (o.method == METHOD_HARMONIC) {
     // per level: 1/w_l and kappa w_l (the extras' scale and kappa-hat, src/operators/harmonics.jl:327-336),
@@ -599,7 +603,7 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
     p.alpha.assign(p.L, 1.0);
     p.hl.assign(p.L, nullptr);
     for (int l = 0; l < p.L; l++) {
-      const double w = std::ldexp(width, -l);
+      const double w = std::ldexp(p.W0, -l);
       hs[l] = 1.0 / w;
       kh[l] = o.kappa * w;
       if (o.kernel != FFMM_KERNEL_YUKAWA) {
@@ -618,6 +622,167 @@ void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, 
   allocate_pass_buffers(p);
   if (p.m2l == FFMM_M2L_SPECTRAL) spectral_prepare(p);
   ctx.sync();
+}
+
+void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
+               double width, int e0, int max_levels, int precision, int m2l) {
+  plan_check(ctx, o, n, ppl, width, e0, precision, m2l);
+  if (!xyz) fail(FFMM_ERR_ARGUMENT, "null points");
+  set_device(ctx);
+  AllocScope scope(ctx);
+  plan_fields(p, ctx, o, n, ppl, origin, width, e0, max_levels, precision, m2l);
+  {
+    StageTimer tm(p, 0);
+    build_tree(p, xyz);
+  }
+  plan_finish(p);
+}
+
+// ---- plan data: the tree of a plan, to rebuild the plan without its tree build ----
+// Layout (host byte order): the magic, the version, the cache's identity (method, kernel,
+// p, nb, Ns, Nl, w_ref, kappa), the plan's parameters and sizes, the per-level node
+// offsets and node-list offsets, then each array as an int64 count and its elements.
+static constexpr char PLAN_MAGIC[8] = {'F', 'F', 'M', 'M', 'T', 'R', 'E', 'E'};
+static constexpr uint32_t PLAN_VERSION = 1;
+
+struct PlanWriter {
+  uint8_t *buf;
+  size_t cap, n = 0;
+  void put(const void *src, size_t b) {
+    if (buf && n + b <= cap) std::memcpy(buf + n, src, b);
+    n += b;
+  }
+  template <class T>
+  void val(T v) { put(&v, sizeof v); }
+  template <class T>
+  void host(const std::vector<T> &v) {
+    val<int64_t>((int64_t)v.size());
+    put(v.data(), v.size() * sizeof(T));
+  }
+  template <class T>
+  void dev(Context &ctx, const DBuf<T> &d, int64_t count) {
+    val<int64_t>(count);
+    const size_t b = (size_t)count * sizeof(T);
+    if (buf && n + b <= cap && count > 0) FFMM_HIP(hipMemcpyAsync(buf + n, d.p, b, hipMemcpyDeviceToHost, ctx.stream));
+    n += b;
+  }
+};
+
+struct PlanReader {
+  const uint8_t *buf;
+  size_t size, n = 0;
+  void get(void *dst, size_t b) {
+    if (n + b > size) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
+    std::memcpy(dst, buf + n, b);
+    n += b;
+  }
+  template <class T>
+  T val() {
+    T v;
+    get(&v, sizeof v);
+    return v;
+  }
+  template <class T>
+  void host(std::vector<T> &v, int64_t expect) {
+    const int64_t c = val<int64_t>();
+    if (c != expect) fail(FFMM_ERR_ARGUMENT, "plan data: an array of the wrong length");
+    v.resize(c);
+    get(v.data(), c * sizeof(T));
+  }
+  template <class T>
+  void dev(Context &ctx, DBuf<T> &d, int64_t expect) {
+    const int64_t c = val<int64_t>();
+    if (c != expect) fail(FFMM_ERR_ARGUMENT, "plan data: an array of the wrong length");
+    const size_t b = (size_t)c * sizeof(T);
+    if (n + b > size) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
+    d.upload(reinterpret_cast<const T *>(buf + n), c, ctx.stream);
+    n += b;
+  }
+};
+
+template <class IO, class PlanT>
+static void plan_tree_arrays(IO &io, Context &ctx, PlanT &p) {
+  const int64_t nn = p.nnodes, N = p.N;
+  io.dev(ctx, p.cell, 3 * nn);
+  io.dev(ctx, p.level, nn);
+  io.dev(ctx, p.parent, nn);
+  io.dev(ctx, p.child_first, nn);
+  io.dev(ctx, p.plo, nn);
+  io.dev(ctx, p.phi, nn);
+  io.dev(ctx, p.col, (int64_t)NCOLL * nn);
+  io.dev(ctx, p.perm, N);
+  io.dev(ctx, p.point_node, N);
+  io.dev(ctx, p.leaf, nn);
+  io.dev(ctx, p.ghost, nn);
+  io.dev(ctx, p.xlist, nn);
+  io.dev(ctx, p.has_source, nn);
+  io.dev(ctx, p.sx, N);
+  io.dev(ctx, p.sy, N);
+  io.dev(ctx, p.sz, N);
+  io.dev(ctx, p.centers, 3 * nn);
+  io.dev(ctx, p.s2m_nodes, p.s2m_off[p.L]);
+  io.dev(ctx, p.leaf_nodes, p.leaf_off[p.L]);
+}
+
+// the plan's tree into buf (when buf is not null and cap is large enough); the size in bytes
+size_t plan_save(Plan &p, void *buf, size_t cap) {
+  Context &ctx = *p.ctx;
+  OpCache &o = *p.ops;
+  set_device(ctx);
+  PlanWriter w{static_cast<uint8_t *>(buf), cap};
+  w.put(PLAN_MAGIC, sizeof PLAN_MAGIC);
+  w.val<uint32_t>(PLAN_VERSION);
+  w.val<int32_t>(o.method); w.val<int32_t>(o.kernel); w.val<int32_t>(o.p); w.val<int32_t>(o.nb);
+  w.val<int32_t>(o.Ns); w.val<int32_t>(o.Nl); w.val<double>(o.w_ref); w.val<double>(o.kappa);
+  w.val<int64_t>(p.N); w.val<int32_t>(p.ppl); w.val<int32_t>(p.max_levels); w.val<int32_t>(p.precision);
+  w.val<int32_t>(p.m2l);
+  for (int d = 0; d < 3; d++) w.val<double>(p.origin[d]);
+  w.val<double>(p.W0); w.val<int32_t>(p.e0); w.val<int32_t>(p.L); w.val<int64_t>(p.nnodes);
+  w.host(p.lo); w.host(p.nl); w.host(p.s2m_off); w.host(p.leaf_off);
+  plan_tree_arrays(w, ctx, p);
+  ctx.sync();
+  return w.n;
+}
+
+// a plan from plan data made by plan_save with a cache of the same operators
+void plan_restore(Plan &p, Context &ctx, OpCache &o, const void *buf, size_t size) {
+  if (!buf) fail(FFMM_ERR_ARGUMENT, "null plan data");
+  PlanReader r{static_cast<const uint8_t *>(buf), size};
+  char magic[8];
+  r.get(magic, sizeof magic);
+  if (std::memcmp(magic, PLAN_MAGIC, sizeof magic) != 0) fail(FFMM_ERR_ARGUMENT, "not plan data");
+  if (r.val<uint32_t>() != PLAN_VERSION) fail(FFMM_ERR_ARGUMENT, "plan data of another version");
+  const int method = r.val<int32_t>(), kernel = r.val<int32_t>(), pdeg = r.val<int32_t>(), nb = r.val<int32_t>();
+  const int Ns = r.val<int32_t>(), Nl = r.val<int32_t>();
+  const double w_ref = r.val<double>(), kappa = r.val<double>();
+  if (method != o.method || kernel != o.kernel || pdeg != o.p || nb != o.nb || Ns != o.Ns || Nl != o.Nl ||
+      w_ref != o.w_ref || kappa != o.kappa)
+    fail(FFMM_ERR_ARGUMENT, "the plan data was made with another operator cache (method, order, kernel or width)");
+  const int64_t N = r.val<int64_t>();
+  const int ppl = r.val<int32_t>(), max_levels = r.val<int32_t>(), precision = r.val<int32_t>();
+  const int m2l = r.val<int32_t>();
+  double origin[3];
+  for (int d = 0; d < 3; d++) origin[d] = r.val<double>();
+  const double W0 = r.val<double>();
+  const int e0 = r.val<int32_t>(), L = r.val<int32_t>();
+  const int64_t nnodes = r.val<int64_t>();
+  plan_check(ctx, o, N, ppl, W0, e0, precision, m2l);
+  if (L < 1 || L > MAX_KEY_LEVELS + 1 || nnodes < 1) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  set_device(ctx);
+  AllocScope scope(ctx);
+  plan_fields(p, ctx, o, N, ppl, origin, W0, e0, max_levels, precision, m2l);
+  p.L = L;
+  p.nnodes = nnodes;
+  r.host(p.lo, L);
+  r.host(p.nl, L);
+  r.host(p.s2m_off, L + 1);
+  r.host(p.leaf_off, L + 1);
+  for (int l = 0; l < L; l++)
+    if (p.lo[l] < 0 || p.nl[l] < 0 || p.lo[l] + p.nl[l] > nnodes) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  if (p.s2m_off[L] > nnodes || p.leaf_off[L] > nnodes) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  plan_tree_arrays(r, ctx, p);
+  if (r.n != size) fail(FFMM_ERR_ARGUMENT, "plan data: trailing bytes");
+  plan_finish(p);
 }
 
 void plan_release(Plan &p) {
@@ -724,6 +889,31 @@ int ffmm_plan_destroy(ffmm_plan plan) {
     if (!plan) return;
     plan_release(plan->c);
     delete plan;
+  });
+}
+
+int ffmm_plan_save(ffmm_plan plan, void *buf, int64_t cap, int64_t *size) {
+  return guard([&] {
+    if (!plan || !size) fail(FFMM_ERR_ARGUMENT, "null argument");
+    if (cap < 0) fail(FFMM_ERR_ARGUMENT, "negative capacity");
+    const size_t need = plan_save(plan->c, nullptr, 0);
+    *size = (int64_t)need;
+    if (buf) {
+      if ((size_t)cap < need) fail(FFMM_ERR_ARGUMENT, "the buffer is smaller than the plan data");
+      plan_save(plan->c, buf, (size_t)cap);
+    }
+  });
+}
+
+int ffmm_plan_load(ffmm_opcache cache, const void *buf, int64_t size, ffmm_plan *out) {
+  return guard([&] {
+    if (!cache || !buf || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    if (size < 0) fail(FFMM_ERR_ARGUMENT, "negative size");
+    *out = nullptr;
+    OpCache &o = cache->c;
+    auto h = std::make_unique<ffmm_plan_s>();
+    plan_restore(h->c, *o.ctx, o, buf, (size_t)size);
+    *out = h.release();
   });
 }
 

@@ -484,6 +484,10 @@ void context_release(Context &c);
 void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
                double width, int e0, int max_levels, int precision, int m2l);
 void plan_release(Plan &p);
+// the plan's tree as plan data (into buf when not null and cap suffices); its size in bytes
+size_t plan_save(Plan &p, void *buf, size_t cap);
+// a plan from plan data of a cache with the same operators: the tree uploaded, the rest rebuilt
+void plan_restore(Plan &p, Context &ctx, OpCache &o, const void *buf, size_t size);
 void plan_build(Plan &p, const double *dens, int nd, int sorted);
 void plan_eval_sources(Plan &p, double *out, int output);
 void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output);

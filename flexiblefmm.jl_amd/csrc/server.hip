@@ -125,6 +125,10 @@ struct ffmm_server_s {
   int64_t completed = 0, built = 0, reused = 0, evictions = 0;
   int running = 0;
   bool stop = false;
+  // plan data in and out of the resident set: a context of its own (created at the first
+  // use), so that no worker's stream or BLAS handle is shared
+  std::mutex io_mu;
+  std::unique_ptr<ffmm::Context> io;
 };
 
 namespace ffmm {
@@ -391,6 +395,97 @@ int ffmm_server_stats(ffmm_server S, int64_t *stats, int n) {
   });
 }
 
+static ffmm::Context &server_io(ffmm_server_s &S) {
+  if (!S.io) {
+    auto c = std::make_unique<ffmm::Context>();
+    ffmm::context_init(*c, S.device, nullptr);
+    S.io = std::move(c);
+  }
+  return *S.io;
+}
+
+int ffmm_server_export(ffmm_server S, ffmm_opcache cache, int64_t geometry, void *buf, int64_t cap, int64_t *size) {
+  return guard([&] {
+    if (!S || !cache || !size) fail(FFMM_ERR_ARGUMENT, "null argument");
+    if (cap < 0) fail(FFMM_ERR_ARGUMENT, "negative capacity");
+    std::lock_guard<std::mutex> io_lk(S->io_mu);
+    Resident *r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      r = find_resident(*S, &cache->c, geometry);
+      if (!r) fail(FFMM_ERR_ARGUMENT, "no resident plan of this cache and geometry");
+      if (r->busy) fail(FFMM_ERR_STATE, "the resident plan is in use");
+      r->busy = true;   // no slot takes it meanwhile, nor is it evicted
+    }
+    struct Release {
+      ffmm_server_s *S;
+      Resident *r;
+      ~Release() {
+        {
+          std::lock_guard<std::mutex> lk(S->mu);
+          r->busy = false;
+        }
+        S->cv_work.notify_all();
+      }
+    } rel{S, r};
+    Context &io = server_io(*S);
+    r->plan->ctx = &io;
+    AllocScope scope(io);
+    const size_t need = plan_save(*r->plan, nullptr, 0);
+    *size = (int64_t)need;
+    if (buf) {
+      if ((size_t)cap < need) fail(FFMM_ERR_ARGUMENT, "the buffer is smaller than the plan data");
+      plan_save(*r->plan, buf, (size_t)cap);
+    }
+  });
+}
+
+int ffmm_server_import(ffmm_server S, ffmm_opcache cache, int64_t geometry, const void *buf, int64_t size) {
+  return guard([&] {
+    if (!S || !cache || !buf) fail(FFMM_ERR_ARGUMENT, "null argument");
+    if (geometry < 0) fail(FFMM_ERR_ARGUMENT, "a resident plan needs a geometry key >= 0");
+    if (size < 0) fail(FFMM_ERR_ARGUMENT, "negative size");
+    OpCache *c = &cache->c;
+    if (c->ctx->device != S->device) fail(FFMM_ERR_ARGUMENT, "the cache lives on another device");
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (find_resident(*S, c, geometry)) fail(FFMM_ERR_ARGUMENT, "a plan of this cache and geometry is resident");
+    }
+    std::lock_guard<std::mutex> io_lk(S->io_mu);
+    auto rr = std::make_unique<Resident>();
+    rr->cache = c;
+    rr->geometry = geometry;
+    rr->plan = std::make_unique<Plan>();
+    Context &io = server_io(*S);
+    try {
+      plan_restore(*rr->plan, io, *c, buf, (size_t)size);
+    } catch (...) {
+      plan_release(*rr->plan);
+      throw;
+    }
+    rr->bytes = plan_bytes(*rr->plan);
+    std::vector<std::unique_ptr<Resident>> graveyard;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (find_resident(*S, c, geometry)) {
+        plan_release(*rr->plan);
+        fail(FFMM_ERR_ARGUMENT, "a plan of this cache and geometry is resident");
+      }
+      while (S->in_use + (int64_t)rr->bytes > S->budget && evict_one(*S, graveyard)) {}
+      if (S->in_use + (int64_t)rr->bytes > S->budget) {
+        plan_release(*rr->plan);
+        fail(FFMM_ERR_MEMORY, "the plan does not fit the server's memory budget");
+      }
+      rr->last_use = ++S->clock;
+      S->in_use += (int64_t)rr->bytes;
+      S->peak = std::max(S->peak, S->in_use);
+      S->resident.push_back(std::move(rr));
+    }
+    for (auto &g : graveyard) plan_release(*g->plan);
+    S->cv_work.notify_all();
+  });
+}
+
 int ffmm_server_destroy(ffmm_server S) {
   return guard([&] {
     if (!S) return;
@@ -403,6 +498,7 @@ int ffmm_server_destroy(ffmm_server S) {
       if (w->th.joinable()) w->th.join();
     for (auto &r : S->resident) plan_release(*r->plan);
     S->resident.clear();
+    if (S->io) context_release(*S->io);
     for (auto &w : S->workers) {
       w->xyz.release();
       w->dens.release();

@@ -58,6 +58,8 @@ SIGNATURES = {
                                _vp, _vp]),
     "ffmm_plan_download_tree": (_i32, [_vp] + [_vp] * 14),
     "ffmm_plan_perm": (_i32, [_vp, _vp]),
+    "ffmm_plan_save": (_i32, [_vp, _vp, _i64, ctypes.POINTER(_i64)]),
+    "ffmm_plan_load": (_i32, [_vp, _vp, _i64, ctypes.POINTER(_vp)]),
     "ffmm_plan_download_lists": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "ffmm_build_expansions": (_i32, [_vp, _vp, _i32]),
     "ffmm_build_expansions_nd": (_i32, [_vp, _vp, _i32, _i32]),
@@ -70,6 +72,8 @@ SIGNATURES = {
     "ffmm_server_submit": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),
     "ffmm_server_wait": (_i32, [_vp, _i64, ctypes.POINTER(_d)]),
     "ffmm_server_stats": (_i32, [_vp, _vp, _i32]),
+    "ffmm_server_export": (_i32, [_vp, _vp, _i64, _vp, _i64, ctypes.POINTER(_i64)]),
+    "ffmm_server_import": (_i32, [_vp, _vp, _i64, _vp, _i64]),
     "ffmm_server_destroy": (_i32, [_vp]),
 }
 

@@ -29,6 +29,7 @@ only); every computation runs in libffmm_hip.
 from __future__ import annotations
 
 import ctypes
+import json
 import math
 from dataclasses import dataclass
 
@@ -547,6 +548,59 @@ class DevicePlan:
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
             # the per-width Yukawa operators of the levels with translations (2 .. L-1)
             cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+
+    # -- plan data: warm restarts without the tree build --
+    _FILE_MAGIC = b"FFMMPLAN\n"
+
+    def save(self, path) -> None:
+        """Write the plan's tree (``ffmm_plan_save``) to ``path``, after a one-line JSON
+        header with the host-side fields (box, groups, forms).  ``DevicePlan.load`` makes
+        the same plan from it without building the tree again."""
+        size = ctypes.c_int64()
+        call("ffmm_plan_save", self._h, None, 0, ctypes.byref(size))
+        buf = (ctypes.c_uint8 * size.value)()
+        call("ffmm_plan_save", self._h, buf, size.value, ctypes.byref(size))
+        head = dict(N=self.N, ppl=self.ppl, origin=[float(v) for v in self.origin], W0=self.W0, e0=self.e0,
+                    precision=self.precision, m2l=self.m2l, group_sizes=[int(g) for g in self.group_sizes])
+        with open(path, "wb") as f:
+            f.write(self._FILE_MAGIC)
+            f.write(json.dumps(head).encode() + b"\n")
+            f.write(bytes(buf))
+
+    @classmethod
+    def load(cls, method, path, *, cache: OperatorCache = None, device: int = 0) -> "DevicePlan":
+        """A plan from a file written by ``save``, with a cache of the same method (the
+        lists and buffers are rebuilt; the tree is uploaded)."""
+        with open(path, "rb") as f:
+            if f.readline() != cls._FILE_MAGIC:
+                raise ArgumentError("not a plan file")
+            head = json.loads(f.readline())
+            data = f.read()
+        cache = OperatorCache(method) if cache is None else cache
+        if cache.method is not method:
+            raise ArgumentError("the cache was built for a different method")
+        self = cls.__new__(cls)
+        self.method, self.cache, self.device = method, cache, device
+        self.precision, self.m2l = head["precision"], head["m2l"]
+        self.origin, self.W0, self.e0 = np.asarray(head["origin"], dtype=np.float64), float(head["W0"]), int(head["e0"])
+        self.N, self.ppl = int(head["N"]), int(head["ppl"])
+        w_ref = math.ldexp(self.W0, -self.e0)   # the cache's reference width when the plan was made
+        if not cache.has_base_width():
+            cache.W0 = w_ref
+        elif cache.W0 != w_ref:
+            raise ArgumentError("the plan data was made with another reference width")
+        oc = cache.device(device)
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        h = ctypes.c_void_p()
+        call("ffmm_plan_load", oc, buf, len(data), ctypes.byref(h))
+        self._h = h
+        self.group_sizes = [int(g) for g in head["group_sizes"]]
+        self.group_offsets = np.concatenate([[0], np.cumsum(self.group_sizes)]).astype(np.int64)
+        self._gperm = None
+        self._read_sizes()
+        if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
+            cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+        return self
 
     def __del__(self):
         h = getattr(self, "_h", None)

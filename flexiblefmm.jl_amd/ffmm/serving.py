@@ -12,6 +12,7 @@ passes only (``build_expansions!``, src/fmm.jl:264-298).
 from __future__ import annotations
 
 import ctypes
+import struct
 import math
 
 import numpy as np
@@ -98,6 +99,27 @@ class Server:
         _lib.check(rc)
         _, _, _, out_s, out_t, _ = keep
         return out_s, out_t, lat.value
+
+    def export_plan(self, cache: OperatorCache, geometry: int) -> bytes:
+        """The plan data of the resident plan of (cache, geometry) (``ffmm_server_export``)."""
+        oc = cache.device(self.device)
+        size = ctypes.c_int64()
+        call("ffmm_server_export", self._h, oc, int(geometry), None, 0, ctypes.byref(size))
+        buf = (ctypes.c_uint8 * size.value)()
+        call("ffmm_server_export", self._h, oc, int(geometry), buf, size.value, ctypes.byref(size))
+        return bytes(buf)
+
+    def import_plan(self, cache: OperatorCache, geometry: int, data: bytes) -> None:
+        """Make plan data (``export_plan``, ``DevicePlan.save``'s payload) the resident plan
+        of (cache, geometry) (``ffmm_server_import``): a warm restart of the resident set."""
+        w_ref = struct.unpack_from("<d", data, 36)[0]   # the cache identity's reference width
+        if not cache.has_base_width():
+            cache.W0 = w_ref
+        elif cache.W0 != w_ref:
+            raise ArgumentError("the plan data was made with another reference width")
+        oc = cache.device(self.device)
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        call("ffmm_server_import", self._h, oc, int(geometry), buf, len(data))
 
     def stats(self) -> dict:
         v = (ctypes.c_int64 * 8)()

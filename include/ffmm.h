@@ -151,6 +151,17 @@ int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, 
                      double width, int e0, int max_levels, int precision, int m2l, ffmm_plan *out);
 int ffmm_plan_destroy(ffmm_plan plan);
 
+/* Plan data: the plan's tree (the flat DeviceTree fields, src/tree/device_builder.jl:
+ * 21-48: cells, levels, parents, children, point ranges, colleagues, flags, the point
+ * permutation and sorted points, the node lists) with the plan's parameters and its
+ * cache's identity, in host byte order.  No counterpart in the reference: it restarts a
+ * serving cache's resident plans without their tree builds.  ffmm_plan_save writes it
+ * into buf (when buf is not NULL; cap must be at least the size) and sets *size;
+ * ffmm_plan_load makes a plan from it with a cache of the same operators (method,
+ * order, kernel, reference width), rebuilding the lists and buffers. */
+int ffmm_plan_save(ffmm_plan plan, void *buf, int64_t cap, int64_t *size);
+int ffmm_plan_load(ffmm_opcache cache, const void *buf, int64_t size, ffmm_plan *out);
+
 /* Tree and list sizes: nlevels, nnodes, npoints, and per level (arrays of length
  * nlevels, any may be NULL) the first node, the node count, the far pairs, the childless
  * sourced nodes and the real leaves. */
@@ -257,6 +268,15 @@ int ffmm_server_wait(ffmm_server server, int64_t ticket, double *latency_ms);
 /* [0] completed, [1] plans built, [2] resident plans reused, [3] resident plans,
  * [4] device bytes held or reserved, [5] their peak, [6] evictions, [7] queued. */
 int ffmm_server_stats(ffmm_server server, int64_t *stats, int n);
+/* Plan data of the resident plan of (cache, geometry), as ffmm_plan_save writes it (buf
+ * may be NULL to ask for the size).  No counterpart in the reference: with
+ * ffmm_server_import it restarts a server's resident set warm.  The plan must be idle. */
+int ffmm_server_export(ffmm_server server, ffmm_opcache cache, int64_t geometry, void *buf, int64_t cap,
+                       int64_t *size);
+/* A resident plan of (cache, geometry) from plan data: later requests with that key run
+ * the passes only.  Counted against the memory budget (idle resident plans are evicted to
+ * make room). */
+int ffmm_server_import(ffmm_server server, ffmm_opcache cache, int64_t geometry, const void *buf, int64_t size);
 /* Finish every queued request, then release the slots and the resident plans. */
 int ffmm_server_destroy(ffmm_server server);
 
