@@ -134,6 +134,7 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
         srv._prepare(c, np.zeros(3), np.ones(3), box)
     tickets = [None] * nreq
     t_sub = [0.0] * nreq
+    d_sub = [0.0] * nreq
 
     def submitter():
         t0 = time.perf_counter()
@@ -146,6 +147,7 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
             t_sub[i] = time.perf_counter()
             tickets[i] = srv.submit(caches[key], r["pts"], r["tau"], geometry=r["gid"], ppl=64, bbox=box,
                                     precision=prec, m2l=m2l)
+            d_sub[i] = time.perf_counter() - t_sub[i]
 
     t0 = time.perf_counter()
     th = threading.Thread(target=submitter)
@@ -162,6 +164,8 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
             first[key] = (i, u)
     t_served = time.perf_counter() - t0
     th.join()
+    # how far the submissions ran behind the arrival schedule (the submitting thread)
+    lag = np.array([t_sub[i] - t0 - reqs[i]["t"] for i in range(nreq)])
     st = srv.stats()
     srv.close()
     # the same requests one after another, plain plans, no reuse and no arrival gaps
@@ -213,6 +217,9 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
         acc[names[key]] = {"rel_l2": err, "gate": gates[key], "ok": err <= gates[key], "points": len(pts),
                            "form": "/".join(form[key])}
     return {"requests": nreq, "points": total, "slots": nslots, "arrival_rate_per_s": rate, "accuracy": acc,
+            "arrival_span_ms": 1e3 * reqs[-1]["t"], "submit_lag_ms": {"mean": 1e3 * float(lag.mean()),
+                                                                       "max": 1e3 * float(lag.max())},
+            "submit_call_ms": {"mean": 1e3 * float(np.mean(d_sub)), "max": 1e3 * float(np.max(d_sub))},
             "n_range": [nmin, nmax], "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)), "served_points_per_s": total / t_served,
             "served_wall_s": t_served, "sequential_points_per_s": total / t_seq, "sequential_wall_s": t_seq,

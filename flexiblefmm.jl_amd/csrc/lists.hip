@@ -1,12 +1,12 @@
 // The interaction lists of every level, built on the device from the flat tree.
 //
-// Per level (0-based levels >= 2): one work-item per target counts its far pairs --
-// the sourced children of its parent's colleagues at Chebyshev offset >= 2, in the
-// enumeration order of the colleagues and their children -- a scan places them and a
-// fill writes (source, far offset).  Then the pairs are counted per (target, class),
-// placed class-major (the pairs of a class with its targets in node order, each
-// target's consecutive) and filled, and the runs (one per (target, class) with pairs)
-// come from a compaction of the counts.
+// Per level (0-based levels >= 2): 32 lanes per target (one per colleague of its parent)
+// count its far pairs -- the sourced children of the colleagues at Chebyshev offset >= 2,
+// in the enumeration order of the colleagues and their children -- a scan places them
+// and a fill writes (source, far offset).  Then one wave per target counts its pairs per
+// class, and they are placed class-major (the pairs of a class with its targets in node
+// order, each target's consecutive, in pair order) and filled; the class-major scan and
+// the target chunks of the dense class M2L come with them.
 //
 // Follows src/device_lists.jl:261-344, 425-484 (= src/tree/lists.jl:73-124 for the
 // pairs and src/operators/symmetry.jl:207-236 for the class lists); 0-based.
@@ -28,64 +28,99 @@ __device__ __forceinline__ bool far_offset(int32_t o0, int32_t o1, int32_t o2) {
   return max(abs(o0), max(abs(o1), abs(o2))) >= 2;
 }
 
+// The pairs of target q: 32 lanes, lane k < 27 for colleague k of the parent; a lane
+// counts the sourced far children of its colleague, and the pairs are placed in
+// colleague order, then child order (the reference's enumeration order).
+constexpr int PL = 32;   // lanes per target of the pair kernels
+
+__device__ __forceinline__ int32_t pair_lane_count(const TreePtrs &t, int64_t i, bool valid, int k, int32_t &s0,
+                                                  int32_t &ci0, int32_t &ci1, int32_t &ci2) {
+  s0 = -1;
+  ci0 = ci1 = ci2 = 0;
+  if (!valid || k >= NCOLL) return 0;
+  const int32_t pc = t.col[(int64_t)k * t.nn + t.parent[i]];
+  if (pc < 0) return 0;
+  s0 = t.child_first[pc];
+  if (s0 < 0) return 0;
+  ci0 = t.cell[i];
+  ci1 = t.cell[t.nn + i];
+  ci2 = t.cell[2 * t.nn + i];
+  int32_t c = 0;
+  for (int32_t s = s0; s < s0 + NCHILD; s++)
+    if (t.has_source[s] && far_offset(t.cell[s] - ci0, t.cell[t.nn + s] - ci1, t.cell[2 * t.nn + s] - ci2)) c++;
+  return c;
+}
+
 // _pair_count! (src/device_lists.jl:263-285)
 __global__ void k_pair_count(TreePtrs t, int64_t lo, int64_t nl, int32_t *__restrict__ cnt) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nl) return;
-  int64_t i = lo + q;
-  int32_t c = 0;
-  if (!t.ghost[i]) {
-    int32_t p = t.parent[i];
-    int32_t ci0 = t.cell[i], ci1 = t.cell[t.nn + i], ci2 = t.cell[2 * t.nn + i];
-    for (int k = 0; k < NCOLL; k++) {
-      int32_t pc = t.col[(int64_t)k * t.nn + p];
-      if (pc < 0) continue;
-      int32_t s0 = t.child_first[pc];
-      if (s0 < 0) continue;
-      for (int32_t s = s0; s < s0 + NCHILD; s++) {
-        if (!t.has_source[s]) continue;
-        if (far_offset(t.cell[s] - ci0, t.cell[t.nn + s] - ci1, t.cell[2 * t.nn + s] - ci2)) c++;
-      }
-    }
-  }
-  cnt[q] = c;
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PL;
+  const int k = threadIdx.x % PL;
+  const int64_t i = lo + q;
+  const bool valid = q < nl && !t.ghost[i];
+  int32_t s0, ci0, ci1, ci2;
+  int32_t n = pair_lane_count(t, i, valid, k, s0, ci0, ci1, ci2);
+#pragma unroll
+  for (int off = 1; off < PL; off <<= 1) n += __shfl_xor(n, off, PL);
+  if (k == 0 && q < nl) cnt[q] = n;
 }
 
 // _pair_fill! (src/device_lists.jl:286-312)
 __global__ void k_pair_fill(TreePtrs t, int64_t lo, int64_t nl, const int32_t *__restrict__ ptr,
                             int32_t *__restrict__ src, int32_t *__restrict__ far) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nl) return;
-  int64_t i = lo + q;
-  if (t.ghost[i]) return;
-  int32_t pos = ptr[q];
-  int32_t p = t.parent[i];
-  int32_t ci0 = t.cell[i], ci1 = t.cell[t.nn + i], ci2 = t.cell[2 * t.nn + i];
-  for (int k = 0; k < NCOLL; k++) {
-    int32_t pc = t.col[(int64_t)k * t.nn + p];
-    if (pc < 0) continue;
-    int32_t s0 = t.child_first[pc];
-    if (s0 < 0) continue;
-    for (int32_t s = s0; s < s0 + NCHILD; s++) {
-      if (!t.has_source[s]) continue;
-      int32_t o0 = t.cell[s] - ci0, o1 = t.cell[t.nn + s] - ci1, o2 = t.cell[2 * t.nn + s] - ci2;
-      if (!far_offset(o0, o1, o2)) continue;
-      src[pos] = s;
-      far[pos] = c_far_index[(o0 + 3) + 7 * (o1 + 3) + 49 * (o2 + 3)];
-      pos++;
-    }
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PL;
+  const int k = threadIdx.x % PL;
+  const int64_t i = lo + q;
+  const bool valid = q < nl && !t.ghost[i];
+  int32_t s0, ci0, ci1, ci2;
+  const int32_t n = pair_lane_count(t, i, valid, k, s0, ci0, ci1, ci2);
+  int32_t v = n;   // inclusive scan over the target's lanes
+#pragma unroll
+  for (int off = 1; off < PL; off <<= 1) {
+    const int32_t u = __shfl_up(v, off, PL);
+    if (k >= off) v += u;
+  }
+  if (n == 0) return;
+  int32_t pos = ptr[q] + v - n;
+  for (int32_t s = s0; s < s0 + NCHILD; s++) {
+    if (!t.has_source[s]) continue;
+    const int32_t o0 = t.cell[s] - ci0, o1 = t.cell[t.nn + s] - ci1, o2 = t.cell[2 * t.nn + s] - ci2;
+    if (!far_offset(o0, o1, o2)) continue;
+    src[pos] = s;
+    far[pos] = c_far_index[(o0 + 3) + 7 * (o1 + 3) + 49 * (o2 + 3)];
+    pos++;
+  }
+}
+
+// The class lists of target q: one wave, the target's pairs 64 at a time (lane j: pair
+// base + j, class c_j); lane c keeps class c's count or next position.  A pair's rank
+// among the batch's earlier pairs of its class comes from one sweep over the lanes, so
+// each class's pairs keep their order.
+__device__ __forceinline__ void class_batch(int32_t c, int lane, int32_t &rank, int32_t &mine) {
+  rank = 0;
+  mine = 0;   // the batch's pairs of class `lane`
+  for (int j = 0; j < 64; j++) {
+    const int32_t cj = __shfl(c, j, 64);
+    rank += (cj == c && j < lane) ? 1 : 0;
+    mine += cj == lane ? 1 : 0;
   }
 }
 
 // _class_count! (src/device_lists.jl:314-324), class-major: cnt[c * nl + q]
 __global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__restrict__ ptr, int64_t nl, int nclass,
                               const int32_t *__restrict__ far_class, int32_t *__restrict__ cnt) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nl) return;
-  int32_t cc[MAX_CLASSES];
-  for (int c = 0; c < nclass; c++) cc[c] = 0;
-  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) cc[far_class[far[p]]]++;
-  for (int c = 0; c < nclass; c++) cnt[(int64_t)c * nl + q] = cc[c];
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (q >= nl) return;   // whole waves: q is wave-uniform
+  const int32_t p0 = ptr[q], p1 = ptr[q + 1];
+  int32_t total = 0;
+  for (int32_t base = p0; base < p1; base += 64) {
+    const int32_t p = base + lane;
+    const int32_t c = p < p1 ? far_class[far[p]] : -1;
+    int32_t rank, mine;
+    class_batch(c, lane, rank, mine);
+    total += mine;
+  }
+  if (lane < nclass) cnt[(int64_t)lane * nl + q] = total;
 }
 
 // _class_fill! (src/device_lists.jl:325-334)
@@ -93,16 +128,23 @@ __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__r
                              const int32_t *__restrict__ ptr, const int32_t *__restrict__ cstart, int64_t nl,
                              int nclass, const int32_t *__restrict__ far_class, int32_t *__restrict__ csrc,
                              int32_t *__restrict__ cfar) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nl) return;
-  int32_t cur[MAX_CLASSES];
-  for (int c = 0; c < nclass; c++) cur[c] = cstart[(int64_t)c * nl + q];
-  for (int32_t p = ptr[q]; p < ptr[q + 1]; p++) {
-    int32_t f = far[p];
-    int c = far_class[f];
-    int32_t pos = cur[c]++;
-    csrc[pos] = src[p];
-    cfar[pos] = f;
+  const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (q >= nl) return;   // whole waves: q is wave-uniform
+  const int32_t p0 = ptr[q], p1 = ptr[q + 1];
+  int32_t next = lane < nclass ? cstart[(int64_t)lane * nl + q] : 0;   // class `lane`'s next position
+  for (int32_t base = p0; base < p1; base += 64) {
+    const int32_t p = base + lane;
+    const int32_t f = p < p1 ? far[p] : 0;
+    const int32_t c = p < p1 ? far_class[f] : -1;
+    int32_t rank, mine;
+    class_batch(c, lane, rank, mine);
+    const int32_t at = __shfl(next, c < 0 ? 0 : c, 64);
+    if (p < p1) {
+      csrc[at + rank] = src[p];
+      cfar[at + rank] = f;
+    }
+    next += mine;
   }
 }
 
@@ -138,7 +180,7 @@ void fill_level_pairs(Plan &plan, int l) {
   li.src.resize(li.np);
   li.far.resize(li.np);
   if (li.np > 0) {
-    k_pair_fill<<<nblocks(plan.nl[l], TB), TB, 0, plan.ctx->stream>>>(t, plan.lo[l], plan.nl[l], li.ptr.p, li.src.p,
+    k_pair_fill<<<nblocks(PL * plan.nl[l], TB), TB, 0, plan.ctx->stream>>>(t, plan.lo[l], plan.nl[l], li.ptr.p, li.src.p,
                                                                       li.far.p);
     FFMM_LAUNCH_CHECK();
   }
@@ -181,7 +223,7 @@ void build_lists(Plan &plan) {
       li.filled = true;
       continue;
     }
-    k_pair_count<<<nblocks(nl, TB), TB, 0, s>>>(t, lo, nl, cnt.p + lo);
+    k_pair_count<<<nblocks(PL * nl, TB), TB, 0, s>>>(t, lo, nl, cnt.p + lo);
     FFMM_LAUNCH_CHECK();
     exclusive_scan_i32_async(ctx, cnt.p + lo, li.ptr.p, nl, l);
   }
@@ -214,14 +256,14 @@ void build_lists(Plan &plan) {
     fill_level_pairs(plan, l);
     const int64_t ne = (int64_t)nclass * nl;
     li.cstart.resize(ne + 1);
-    k_class_count<<<nblocks(nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ops.far_class.p, ccnt.p);
+    k_class_count<<<nblocks(64 * nl, TB), TB, 0, s>>>(li.far.p, li.ptr.p, nl, nclass, ops.far_class.p, ccnt.p);
     FFMM_LAUNCH_CHECK();
     // the class-major scan; its total is np (slot 0 is free here)
     exclusive_scan_i32_async(ctx, ccnt.p, li.cstart.p, ne, 0);
     li.csrc.resize(np);
     li.cfar.resize(np);
     if (np > 0) {
-      k_class_fill<<<nblocks(nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, li.cstart.p, nl, nclass,
+      k_class_fill<<<nblocks(64 * nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, li.cstart.p, nl, nclass,
                                                   ops.far_class.p, li.csrc.p, li.cfar.p);
       FFMM_LAUNCH_CHECK();
     }

@@ -75,7 +75,9 @@ class Server:
         nd = 1 if d.ndim == 1 else d.shape[1]
         o = _output(output)
         nc = 3 if isinstance(m, StokesHarmonics) else o.ncomp
-        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        # per coordinate: numpy's axis-0 reduction of an N x 3 array is several times slower
+        lo = np.array([pts[:, d].min() for d in range(3)])
+        hi = np.array([pts[:, d].max() for d in range(3)])
         if bbox is None:
             bbox = square_bbox(lo, hi)
         oc, origin, W, e0 = self._prepare(cache, lo, hi, bbox)
