@@ -70,7 +70,9 @@ class StokesFMMPlan:
         tgts = None if targets is None else _points(targets)
         if bbox is None:
             allp = pts if tgts is None else np.vstack([pts, tgts])
-            bbox = square_bbox(allp.min(0), allp.max(0))
+            # per coordinate: numpy's axis-0 reduction of an N x 3 array is several times slower
+            bbox = square_bbox(np.array([allp[:, d].min() for d in range(3)]),
+                               np.array([allp[:, d].max() for d in range(3)]))
         m = StokesHarmonics(Stokeslet3D(), p=p)
         if cache is None:
             cache = OperatorCache(m)
