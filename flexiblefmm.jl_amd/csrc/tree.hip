@@ -390,10 +390,15 @@ static void tag(Context &ctx, DevLevel &lev, DevLevel &par) {
   FFMM_LAUNCH_CHECK();
 }
 
-static void depths(Context &ctx, std::vector<DevLevel> &levels) {
+// the depths of levels `from` .. 0 from those below (the deepest level's are zero); a
+// split at level l changes the depths of levels l + 1 .. 0 only
+static void depths(Context &ctx, std::vector<DevLevel> &levels, int from) {
   int L = (int)levels.size();
-  k_zero_i32<<<nblocks(levels[L - 1].n, TB), TB, 0, ctx.stream>>>(levels[L - 1].depth.p, levels[L - 1].n);
-  for (int l = L - 2; l >= 0; l--)
+  if (from >= L - 1) {
+    k_zero_i32<<<nblocks(levels[L - 1].n, TB), TB, 0, ctx.stream>>>(levels[L - 1].depth.p, levels[L - 1].n);
+    from = L - 2;
+  }
+  for (int l = from; l >= 0; l--)
     k_depth<<<nblocks(levels[l].n, TB), TB, 0, ctx.stream>>>(levels[l].child_first.p, levels[l].n,
                                                              levels[l + 1].depth.p, levels[l].depth.p);
   FFMM_LAUNCH_CHECK();
@@ -460,7 +465,7 @@ void build_tree(Plan &plan, const double *xyz) {
   k_root_col<<<1, 32, 0, s>>>(levels[0].col.p);
   for (int l = 1; l < L; l++) tag(ctx, levels[l], levels[l - 1]);
   // 2:1 restriction (src/tree/device_builder.jl:444-460): 0-based levels 1 .. L-3
-  depths(ctx, levels);
+  depths(ctx, levels, L - 1);
   for (;;) {
     int64_t nsplit = 0;
     for (int l = 1; l <= L - 3; l++) {
@@ -473,7 +478,7 @@ void build_tree(Plan &plan, const double *xyz) {
       if (nb == 0) continue;
       split(ctx, levels[l], which.p, nb, l, levels[l + 1], skeys.p, KL, false);
       tag(ctx, levels[l + 1], levels[l]);
-      depths(ctx, levels);
+      depths(ctx, levels, l + 1);
       nsplit += nb;
     }
     if (nsplit == 0) break;
