@@ -303,8 +303,8 @@ def main():
                          "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus N with N > 1 is not supported "
-                                   "(the multi-GPU sharding is a design only; see the design notes)"}))
+        print(json.dumps({"error": "bench.py measures one GPU: --gpus N with N > 1 is not supported "
+                                   "(the library runs one GPU's share; sharding over several GPUs is not built)"}))
         sys.exit(2)
 
     import torch

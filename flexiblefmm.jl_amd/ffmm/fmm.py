@@ -112,7 +112,7 @@ class CubeKernelIndependent:
             raise ArgumentError("d must lie in (0, 2/3): the large surface, at (3 − 2d)w/2, must lie outside the "
                                 "small one, at (1 + d)w/2")
         if not isinstance(kernel, Laplace3D):
-            raise ArgumentError("the device passes of this build take Laplace3D")
+            raise ArgumentError("the device library takes Laplace3D")
         self.kernel, self.p, self.pc, self.d, self.pinv_rtol = kernel, int(p), int(pc), float(d), float(pinv_rtol)
 
     @property
@@ -147,9 +147,9 @@ class SphericalHarmonics:
         if not 0 <= tol < 1:
             raise ArgumentError("tol must lie in [0, 1)")
         if tol > 0:
-            raise ArgumentError("the device passes of this build take the dense M2L (tol = 0)")
+            raise ArgumentError("the device library takes the dense M2L (tol = 0)")
         if p > 60:
-            raise ArgumentError("the device passes of this build take p <= 60")
+            raise ArgumentError("the device library takes p <= 60")
         self.kernel, self.p, self.tol = kernel, int(p), float(tol)
 
     nblocks = 1
@@ -176,9 +176,9 @@ class StokesHarmonics(SphericalHarmonics):
         if not 0 <= tol < 1:
             raise ArgumentError("tol must lie in [0, 1)")
         if tol > 0:
-            raise ArgumentError("the device passes of this build take the dense M2L (tol = 0)")
+            raise ArgumentError("the device library takes the dense M2L (tol = 0)")
         if p > 60:
-            raise ArgumentError("the device passes of this build take p <= 60")
+            raise ArgumentError("the device library takes p <= 60")
         self.kernel, self.p, self.tol = kernel, int(p), float(tol)
 
     def __repr__(self):

@@ -92,7 +92,7 @@ def _check_output(output):
 
 def _check_same(given, planned, what: str):
     if given is not None and not np.array_equal(_points(given), planned):
-        raise ArgumentError(f"the {what} differ from the plan's (this build has no dual-number positions)")
+        raise ArgumentError(f"the {what} differ from the plan's (the device library takes no dual-number positions)")
 
 
 def stokes_fmm_eval_(velocities, plan: StokesFMMPlan, *, stokeslets=None, stresslets=None, stressvecs=None,
