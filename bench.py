@@ -127,47 +127,61 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
     reqs = serving_stream(nreq, nmin, nmax, rate)
     total = float(sum(len(r["pts"]) for r in reqs))
     log(f"[bench] serving: {nreq} requests, {total:.3e} points, {nslots} slots")
-    srv = Server(nslots=nslots, hbm_budget=96 << 30)
-    # warm the operator caches (host precompute and device copies, Yukawa widths of 14 levels)
-    for k, c in caches.items():
-        c.fit_box(np.zeros(3), np.ones(3), box)
-        srv._prepare(c, np.zeros(3), np.ones(3), box)
-    tickets = [None] * nreq
-    t_sub = [0.0] * nreq
-    d_sub = [0.0] * nreq
 
-    def submitter():
+    def serve(time_scale: float):
+        """The stream through a fresh server, arrivals at the stream's times x time_scale
+        replayed by a submitting thread; per request the latency, the submission lag and
+        the submit call's time, and the first served result of each class."""
+        srv = Server(nslots=nslots, hbm_budget=96 << 30)
+        # warm the operator caches (host precompute and device copies, Yukawa widths of 14 levels)
+        for k, c in caches.items():
+            c.fit_box(np.zeros(3), np.ones(3), box)
+            srv._prepare(c, np.zeros(3), np.ones(3), box)
+        tickets = [None] * nreq
+        t_sub = [0.0] * nreq
+        d_sub = [0.0] * nreq
+
+        def submitter():
+            t0 = time.perf_counter()
+            for i, r in enumerate(reqs):
+                dt = time_scale * r["t"] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                key = (r["lap"], r["fine"])
+                prec, m2l = form[key]
+                t_sub[i] = time.perf_counter()
+                tickets[i] = srv.submit(caches[key], r["pts"], r["tau"], geometry=r["gid"], ppl=64, bbox=box,
+                                        precision=prec, m2l=m2l)
+                d_sub[i] = time.perf_counter() - t_sub[i]
+
         t0 = time.perf_counter()
-        for i, r in enumerate(reqs):
-            dt = r["t"] - (time.perf_counter() - t0)
-            if dt > 0:
-                time.sleep(dt)
-            key = (r["lap"], r["fine"])
-            prec, m2l = form[key]
-            t_sub[i] = time.perf_counter()
-            tickets[i] = srv.submit(caches[key], r["pts"], r["tau"], geometry=r["gid"], ppl=64, bbox=box,
-                                    precision=prec, m2l=m2l)
-            d_sub[i] = time.perf_counter() - t_sub[i]
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = []
+        first = {}   # class -> (request index, served potential at the sources)
+        for i in range(nreq):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            u, _, ms = srv.wait(tickets[i])
+            lat.append(ms)
+            key = (reqs[i]["lap"], reqs[i]["fine"])
+            if key not in first:
+                first[key] = (i, u)
+        t_served = time.perf_counter() - t0
+        th.join()
+        # how far the submissions ran behind the arrival schedule (the submitting thread)
+        lag = np.array([t_sub[i] - t0 - time_scale * reqs[i]["t"] for i in range(nreq)])
+        st = srv.stats()
+        srv.close()
+        return t_served, np.array(lat), lag, np.array(d_sub), first, st
 
-    t0 = time.perf_counter()
-    th = threading.Thread(target=submitter)
-    th.start()
-    lat = []
-    first = {}   # class -> (request index, served potential at the sources)
-    for i in range(nreq):
-        while tickets[i] is None:
-            time.sleep(0.0005)
-        u, _, ms = srv.wait(tickets[i])
-        lat.append(ms)
-        key = (reqs[i]["lap"], reqs[i]["fine"])
-        if key not in first:
-            first[key] = (i, u)
-    t_served = time.perf_counter() - t0
-    th.join()
-    # how far the submissions ran behind the arrival schedule (the submitting thread)
-    lag = np.array([t_sub[i] - t0 - reqs[i]["t"] for i in range(nreq)])
-    st = srv.stats()
-    srv.close()
+    # a lighter load first (a fifth of the rate: the latency of requests that do not queue),
+    # then the stream at its rate (the throughput)
+    light_t, light_lat, light_lag, _, _, _ = serve(5.0)
+    light = {"arrival_rate_per_s": rate / 5.0, "p50_latency_ms": float(np.percentile(light_lat, 50)),
+             "p99_latency_ms": float(np.percentile(light_lat, 99)), "served_points_per_s": total / light_t,
+             "submit_lag_ms": {"mean": 1e3 * float(light_lag.mean()), "max": 1e3 * float(light_lag.max())}}
+    t_served, lat, lag, d_sub, first, st = serve(1.0)
     # the same requests one after another, plain plans, no reuse and no arrival gaps
     torch = __import__("torch")
     torch.cuda.synchronize()
@@ -191,7 +205,6 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
         del plan
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t1
-    lat = np.array(lat)
     # the accuracy of the served results: the first request of each class against the
     # FP64 direct sum at 300 sampled sources (Laplace on the device; Yukawa in numpy)
     acc = {}
@@ -217,7 +230,7 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
         acc[names[key]] = {"rel_l2": err, "gate": gates[key], "ok": err <= gates[key], "points": len(pts),
                            "form": "/".join(form[key])}
     return {"requests": nreq, "points": total, "slots": nslots, "arrival_rate_per_s": rate, "accuracy": acc,
-            "arrival_span_ms": 1e3 * reqs[-1]["t"], "submit_lag_ms": {"mean": 1e3 * float(lag.mean()),
+            "light_load": light, "arrival_span_ms": 1e3 * reqs[-1]["t"], "submit_lag_ms": {"mean": 1e3 * float(lag.mean()),
                                                                        "max": 1e3 * float(lag.max())},
             "submit_call_ms": {"mean": 1e3 * float(np.mean(d_sub)), "max": 1e3 * float(np.max(d_sub))},
             "n_range": [nmin, nmax], "p50_latency_ms": float(np.percentile(lat, 50)),
