@@ -3,12 +3,12 @@
 // colleagues) for SphericalHarmonics with Laplace3D or Yukawa3D charges and for
 // StokesHarmonics with stokeslets and stresslets.
 //
-// S2M: one wave per node; lane (m, r) runs the order-m chain of the Schmidt
-// recurrences (_fold_chain, src/operators/harmonics.jl:681-704) for the sources
-// r, r + R, ... of the node (R = 64 / next_pow2(p + 1) replicas) and owns the entries
-// of order m in replica r's column in LDS; the replicas are summed at the end
-// (_s2m_harmonic_kernel!, src/device_passes.jl:343-373, which runs one work-item per
-// node and order).
+// S2M: one wave per node, over tiles of the node's sources; the lanes of a source run
+// the order chains of the Schmidt recurrences (_fold_chain,
+// src/operators/harmonics.jl:681-704) in pairs of orders (m, p - m) and store the
+// harmonics in LDS, then each lane folds the tile into the entries of the multipole
+// that it owns (_s2m_harmonic_kernel!, src/device_passes.jl:343-373, which runs one
+// work-item per node and order).
 //
 // Evaluation: one wave per leaf; as in the cube evaluation, lanes take targets and,
 // when a leaf has fewer than 64, R replicas of each target split the orders of the
@@ -73,53 +73,23 @@ __device__ void yukawa_u(double z2, int nmax, double *u) {
   for (int n = 0; n <= nmax; n++) u[n] *= s;
 }
 
-__device__ __forceinline__ void acc_entry(double *col, int n, int m, double re, double im) {
-  if (m == 0) {
-    col[n * n] += re;
-  } else {
-    col[n * n + 2 * m - 1] += re;
-    col[n * n + 2 * m] += im;
-  }
+// the LDS index of the real (im = 0) or imaginary (im = 1) part of the harmonic of
+// degree n, order m, in a row of (p + 1)^2 reals (the multipole layout: n^2 for m = 0,
+// n^2 + 2m - 1 and n^2 + 2m for m >= 1); the imaginary part of order 0 maps to the
+// row's zero slot z
+__device__ __forceinline__ int hidx(int n, int m, int im, int z) {
+  return m == 0 ? (im ? z : n * n) : n * n + 2 * m - 1 + im;
 }
 
-// the entries of order m of a dipole d at x (folds to p - 1 of the chains m - 1, m,
-// m + 1): _accumulate_order! (src/operators/harmonics.jl:718-752)
-template <int ND>
-__device__ __forceinline__ void dipole_order(const HTab &tb, double x, double y, double z, int p, int m,
-                                             const double (&d)[ND][3], double *col, int N) {
-  if (m >= 1)
-    chain(tb, x, y, z, m - 1, p - 1, [&](int n1, double ur, double ui) {
-      const int n = n1 + 1;
-      const double c = tb.cm[tri(n, m)] / 2;
-#pragma unroll
-      for (int k = 0; k < ND; k++)
-        acc_entry(col + k * N, n, m, -c * (d[k][0] * ur - d[k][1] * ui), c * (d[k][0] * ui + d[k][1] * ur));
-    });
-  if (m <= p - 1)
-    chain(tb, x, y, z, m, p - 1, [&](int n1, double ur, double ui) {
-      const int n = n1 + 1;
-#pragma unroll
-      for (int k = 0; k < ND; k++) {
-        const double c = tb.cz[tri(n, m)] * d[k][2];
-        acc_entry(col + k * N, n, m, c * ur, -c * ui);
-      }
-    });
-  if (m + 1 <= p - 1)
-    chain(tb, x, y, z, m + 1, p - 1, [&](int n1, double ur, double ui) {
-      const int n = n1 + 1;
-      if (m >= 1) {
-        const double c = tb.cp[tri(n, m)] / 2;
-#pragma unroll
-        for (int k = 0; k < ND; k++)
-          acc_entry(col + k * N, n, m, c * (d[k][0] * ur + d[k][1] * ui), c * (d[k][1] * ur - d[k][0] * ui));
-      } else {
-        const double c = tb.cp[tri(n, 0)];
-#pragma unroll
-        for (int k = 0; k < ND; k++) acc_entry(col + k * N, n, 0, c * (d[k][0] * ur + d[k][1] * ui), 0.0);
-      }
-    });
-}
-
+// S2M in tiles of S sources of the node. Phase 1: the LS = 64 / S lanes of a source
+// split its orders in pairs (m, p - m), whose chains are about p + 2 steps together,
+// and store the regular harmonics (Yukawa: times u_n) in the source's row of the
+// tile Y. Phase 2: lane t folds the tile into the entries e = t, t + 64, ... that it
+// owns, in registers, and keeps them in an LDS column between tiles:
+//   charges, stokeslets: M_b(e) += w_b conj(Y(e)) per block b;
+//   stresslet dipoles d_k (src/operators/harmonics.jl:718-752): entry (n, m) takes
+//   Y of degree n - 1 and orders m - 1, m, m + 1 against the components of d_k,
+//   M_k(e) += a0 d_k0 + a1 d_k1 + a2 d_k2.
 // KIND: 0 Laplace3D charges, 1 Yukawa3D charges, 2 Stokeslet3D (nd = 3 or 9)
 template <int KIND>
 __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nodes, const double *__restrict__ centers,
@@ -128,82 +98,125 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
                                                  const int32_t *__restrict__ plo, const int32_t *__restrict__ phi,
                                                  const double *__restrict__ sx, const double *__restrict__ sy,
                                                  const double *__restrict__ sz, const double *__restrict__ dens, int nd,
-                                                 HTab tb, int p, int MP, double *__restrict__ Mp) {
+                                                 HTab tb, int p, int S, double *__restrict__ Mp) {
   extern __shared__ double sm[];
-  const int N = (p + 1) * (p + 1);
+  const int N = (p + 1) * (p + 1), NY = N + 1;   // a row of the tile ends in a zero slot
   constexpr int nb = KIND == 2 ? 4 : 1;
-  const int NB = nb * N;
-  const int R = HB / MP;
-  double *acc = sm;               // R columns of NB
-  double *uw = sm + R * NB;       // R x (p + 2) radial weights (Yukawa)
-  const int t = threadIdx.x, m = t % MP, r = t / MP;
+  constexpr int NW = KIND == 2 ? 16 : 1;          // per source: w_0..3, then d_k (4 x 3)
+  double *acc = sm;                               // nb x N
+  double *Y = acc + nb * N;                       // S x NY
+  double *W = Y + (size_t)S * NY;                 // S x NW
+  double *U = W + S * NW;                         // S x (p + 2), Yukawa radial weights
+  const int t = threadIdx.x, LS = HB / S, sl = t / LS, gl = t % LS, ng = p / 2 + 1;
   const int node = nodes[blockIdx.x];
-  for (int i = t; i < R * NB; i += HB) acc[i] = 0.0;
   const double c0 = centers[node], c1 = centers[nn + node], c2 = centers[2 * nn + node];
   const int lv = level[node];
   const double s = hscale[lv];
   const int32_t j0 = plo[node], j1 = phi[node];
-  double *col = acc + r * NB;
-  __syncthreads();
-  for (int32_t jb = j0; jb <= j1; jb += R) {
-    const int32_t j = jb + r;
-    const bool valid = j <= j1;
+  const bool dip = KIND == 2 && nd == 9;
+  for (int i = t; i < nb * N; i += HB) acc[i] = 0.0;
+  for (int i = t; i < S; i += HB) Y[(size_t)i * NY + N] = 0.0;
+  for (int32_t jb = j0; jb <= j1; jb += S) {
+    const int ns = min(S, (int)(j1 - jb + 1));
+    const int32_t j = jb + sl;
+    const bool valid = sl < ns;
     const double x = valid ? (sx[j] - c0) * s : 0.0;
     const double y = valid ? (sy[j] - c1) * s : 0.0;
     const double z = valid ? (sz[j] - c2) * s : 0.0;
-    if (KIND == 1) {
-      if (valid && m == 0) {
-        const double kh = khat[lv];
-        yukawa_u(kh * kh * (x * x + y * y + z * z), p + 1, uw + r * (p + 2));
-      }
-      __syncthreads();
-    }
-    if (valid && m <= p) {
-      if (KIND == 0) {
-        const double q = dens[j];
-        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) { acc_entry(col, n, m, q * vr, -q * vi); });
-      } else if (KIND == 1) {
-        const double q = dens[j];
-        const double *u = uw + r * (p + 2);
-        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) {
-          const double qq = q * u[n];
-          acc_entry(col, n, m, qq * vr, -qq * vi);
-        });
-      } else {
+    if (valid && gl == 0) {
+      double *w = W + sl * NW;
+      if (KIND == 2) {
         const double *f = dens + (int64_t)nd * j;
-        const double f0 = f[0], f1 = f[1], f2 = f[2];
-        const double fy = x * f0 + y * f1 + z * f2;
-        chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) {
-          acc_entry(col, n, m, f0 * vr, -f0 * vi);
-          acc_entry(col + N, n, m, f1 * vr, -f1 * vi);
-          acc_entry(col + 2 * N, n, m, f2 * vr, -f2 * vi);
-          acc_entry(col + 3 * N, n, m, fy * vr, -fy * vi);
-        });
-        if (nd == 9) {
+        w[0] = f[0];
+        w[1] = f[1];
+        w[2] = f[2];
+        w[3] = x * f[0] + y * f[1] + z * f[2];
+        if (dip) {
           // the stresslet (g, nu): Phi_k from Dipole(-(g_k nu + nu_k g) s), Psi from
           // Dipole(-((nu.y) g + (g.y) nu) s) (src/operators/harmonics_stokes.jl:204-211)
           const double g[3] = {f[3], f[4], f[5]}, nu[3] = {f[6], f[7], f[8]};
-          double d[4][3];
-#pragma unroll
-          for (int k = 0; k < 3; k++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) d[k][c] = -(g[k] * nu[c] + nu[k] * g[c]) * s;
           const double nuy = nu[0] * x + nu[1] * y + nu[2] * z;
           const double gy = g[0] * x + g[1] * y + g[2] * z;
 #pragma unroll
-          for (int c = 0; c < 3; c++) d[3][c] = -(nuy * g[c] + gy * nu[c]) * s;
-          dipole_order<4>(tb, x, y, z, p, m, d, col, N);
+          for (int c = 0; c < 3; c++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) w[4 + 3 * k + c] = -(g[k] * nu[c] + nu[k] * g[c]) * s;
+            w[13 + c] = -(nuy * g[c] + gy * nu[c]) * s;
+          }
+        }
+      } else {
+        w[0] = dens[j];
+        if (KIND == 1) {
+          const double kh = khat[lv];
+          yukawa_u(kh * kh * (x * x + y * y + z * z), p + 1, U + sl * (p + 2));
         }
       }
     }
     if (KIND == 1) __syncthreads();
+    if (valid) {
+      double *row = Y + (size_t)sl * NY;
+      const double *u = U + sl * (p + 2);
+      for (int g = gl; g < ng; g += LS)
+        for (int which = 0; which < 2; which++) {
+          const int m = which == 0 ? g : p - g;
+          if (which == 1 && m == g) break;
+          chain(tb, x, y, z, m, p, [&](int n, double vr, double vi) {
+            if (KIND == 1) {
+              vr *= u[n];
+              vi *= u[n];
+            }
+            if (m == 0) {
+              row[n * n] = vr;
+            } else {
+              row[n * n + 2 * m - 1] = vr;
+              row[n * n + 2 * m] = vi;
+            }
+          });
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < N; e += HB) {
+      const int n = (int)sqrt((double)e), r = e - n * n, m = (r + 1) / 2, im = r > 0 && (r & 1) == 0;
+      const double sg = im ? -1.0 : 1.0;
+      double a[nb];
+#pragma unroll
+      for (int b = 0; b < nb; b++) a[b] = acc[b * N + e];
+      for (int q = 0; q < ns; q++) {
+        const double v = sg * Y[(size_t)q * NY + e];
+#pragma unroll
+        for (int b = 0; b < nb; b++) a[b] += W[q * NW + b] * v;
+      }
+      if (dip) {
+        // entry (n, m) from Y_{n-1}^{m-1} (A, m >= 1), Y_{n-1}^m (B, n >= m + 1) and
+        // Y_{n-1}^{m+1} (C, n >= m + 2), indices into a row and their weights
+        const int z0 = N;
+        double kA = m >= 1 ? tb.cm[tri(n, m)] / 2 : 0.0;
+        const double kB = n >= m + 1 ? tb.cz[tri(n, m)] : 0.0;
+        const double kC = n >= m + 2 ? (m >= 1 ? tb.cp[tri(n, m)] / 2 : tb.cp[tri(n, 0)]) : 0.0;
+        const int n1 = n - 1;
+        const int Ar = m >= 1 ? hidx(n1, m - 1, 0, z0) : z0, Ai = m >= 1 ? hidx(n1, m - 1, 1, z0) : z0;
+        const int Br = n >= m + 1 ? hidx(n1, m, 0, z0) : z0, Bi = n >= m + 1 ? hidx(n1, m, 1, z0) : z0;
+        const int Cr = n >= m + 2 ? hidx(n1, m + 1, 0, z0) : z0, Ci = n >= m + 2 ? hidx(n1, m + 1, 1, z0) : z0;
+        // re: a0 = -kA Ar + kC Cr, a1 = kA Ai + kC Ci, a2 = kB Br
+        // im: a0 = kA Ai - kC Ci, a1 = kA Ar + kC Cr, a2 = -kB Bi
+        const int i00 = im ? Ai : Ar, i01 = im ? Ci : Cr, i10 = im ? Ar : Ai, i11 = im ? Cr : Ci, i2 = im ? Bi : Br;
+        const double k00 = im ? kA : -kA, k01 = im ? -kC : kC, k10 = kA, k11 = kC, k2 = im ? -kB : kB;
+        for (int q = 0; q < ns; q++) {
+          const double *yr = Y + (size_t)q * NY;
+          const double a0 = k00 * yr[i00] + k01 * yr[i01];
+          const double a1 = k10 * yr[i10] + k11 * yr[i11];
+          const double a2 = k2 * yr[i2];
+          const double *d = W + q * NW + 4;
+#pragma unroll
+          for (int k = 0; k < 4; k++) a[k] += a0 * d[3 * k] + a1 * d[3 * k + 1] + a2 * d[3 * k + 2];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < nb; b++) acc[b * N + e] = a[b];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  for (int i = t; i < NB; i += HB) {
-    double v = 0.0;
-    for (int rr = 0; rr < R; rr++) v += acc[rr * NB + i];
-    Mp[(int64_t)node * NB + i] = v;
-  }
+  for (int i = t; i < nb * N; i += HB) Mp[(int64_t)node * nb * N + i] = acc[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -497,24 +510,22 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   }
 }
 
-static int next_pow2(int v) {
-  int m = 1;
-  while (m < v) m <<= 1;
-  return m;
-}
-
 void harmonic_s2m_level(Plan &plan, int l) {
   OpCache &o = *plan.ops;
   const int64_t nq = plan.s2m_off[l + 1] - plan.s2m_off[l];
   if (nq == 0) return;
-  const int p = o.p, MP = next_pow2(p + 1), R = HB / MP;
-  const int NB = o.Ns;
-  const size_t lds = sizeof(double) * ((size_t)R * NB + (size_t)R * (p + 2));
+  const int p = o.p, N = (p + 1) * (p + 1);
+  // sources per tile: the most (a power of two, at most 64) whose harmonics fit 8 KiB
+  int S = 64;
+  while (S > 1 && (size_t)S * (N + 1) * sizeof(double) > 8192) S /= 2;
+  const int nw = o.kernel == FFMM_KERNEL_STOKESLET ? 16 : 1;
+  const size_t lds = sizeof(double) * ((size_t)o.Ns + (size_t)S * (N + 1) + (size_t)S * nw +
+                                       (o.kernel == FFMM_KERNEL_YUKAWA ? (size_t)S * (p + 2) : 0));
   const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];
   hipStream_t s = plan.ctx->stream;
 #define FFMM_S2M_ARGS                                                                                              \
   nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.hscale.p, plan.khat.p, plan.plo.p, plan.phi.p, plan.sx.p, \
-      plan.sy.p, plan.sz.p, plan.tau.p, plan.nd, o.tab(), p, MP, plan.Mp.p
+      plan.sy.p, plan.sz.p, plan.tau.p, plan.nd, o.tab(), p, S, plan.Mp.p
   if (o.kernel == FFMM_KERNEL_LAPLACE) k_s2m_harm<0><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
   else if (o.kernel == FFMM_KERNEL_YUKAWA) k_s2m_harm<1><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
   else k_s2m_harm<2><<<(unsigned)nq, HB, lds, s>>>(FFMM_S2M_ARGS);
