@@ -283,7 +283,8 @@ def stokes_bench(n: int, ppl: int, nerr: int):
         u = u_d.cpu().numpy()[rows]
         err = float(np.linalg.norm(u - ud) / np.linalg.norm(ud))
         res[prec] = {"ms_per_solve": ms, "points_per_s": n / (ms * 1e-3), "rel_l2_velocity": err,
-                     "stages_ms": plan.stage_times(), "levels": plan.nlevels}
+                     "stages_ms": plan.stage_times(), "levels": plan.nlevels, "nodes": plan.nnodes,
+                     "far_pairs": int(plan.level_pairs.sum())}
         log(f"[bench] stokes {prec}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} points/s, rel L2 {err:.2e}")
         del plan
     res["mixed_over_f64"] = res["f64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]

@@ -366,24 +366,24 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
       ps[tid] = csrc[p];
     }
     __syncthreads();
-    // gather: one item per (pair, coefficient), all blocks
-    for (int e = tid; e < npt * N; e += FM_TB) {
-      const int pl = e / N, k = e - pl * N;
-      const int32_t f = pf[pl];
-      const int32_t g = gsel[f * N + k];
-      const double *__restrict__ m = Mp + (int64_t)ps[pl] * Ns + (g & 0x7fffffff);
-      const double sg = g < 0 ? -1.0 : 1.0;
-      if (NB == 4) {
-        const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N];
-        float *__restrict__ b = Bs + (pl * 4) * FM_BS + k;
-        b[0] = (float)(sg * m0);
-        b[FM_BS] = (float)(sg * m1);
-        b[2 * FM_BS] = (float)(sg * m2);
-        b[3 * FM_BS] = (float)(sg * (m3 + offs[3 * f] * m0 + offs[3 * f + 1] * m1 + offs[3 * f + 2] * m2));
-      } else {
-        Bs[pl * FM_BS + k] = (float)(sg * m[0]);
+    // gather: a wave per pair, a lane per coefficient, all blocks
+    for (int pl = w; pl < npt; pl += FM_TB / 64)
+      for (int k = lane; k < N; k += 64) {
+        const int32_t f = pf[pl];
+        const int32_t g = gsel[f * N + k];
+        const double *__restrict__ m = Mp + (int64_t)ps[pl] * Ns + (g & 0x7fffffff);
+        const double sg = g < 0 ? -1.0 : 1.0;
+        if (NB == 4) {
+          const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N];
+          float *__restrict__ b = Bs + (pl * 4) * FM_BS + k;
+          b[0] = (float)(sg * m0);
+          b[FM_BS] = (float)(sg * m1);
+          b[2 * FM_BS] = (float)(sg * m2);
+          b[3 * FM_BS] = (float)(sg * (m3 + offs[3 * f] * m0 + offs[3 * f + 1] * m1 + offs[3 * f + 2] * m2));
+        } else {
+          Bs[pl * FM_BS + k] = (float)(sg * m[0]);
+        }
       }
-    }
     __syncthreads();
     // products: wave w, row tile w, all 8 column tiles
     fm_f32x4 acc[8];
@@ -424,10 +424,8 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
     __syncthreads();
     // coalesced copy of the tile's columns: pair column q = co[c] + lp0 + pl, NB N entries
     float *__restrict__ out = Yc + ((int64_t)ca.co[c] + lp0) * NB * N;
-    for (int e = tid; e < npt * NB * N; e += FM_TB) {
-      const int col = e / N, j = e - col * N;
-      out[e] = Bs[col * FM_BS + j];
-    }
+    for (int col = w; col < npt * NB; col += FM_TB / 64)
+      for (int j = lane; j < N; j += 64) out[(int64_t)col * N + j] = Bs[col * FM_BS + j];
     __syncthreads();   // before the next tile's pair list and gather (the K padding is never written)
   }
 }
@@ -466,6 +464,56 @@ __global__ void k_chunk_scatter_framed(double *__restrict__ pL, const float *__r
 #pragma unroll 8
     for (int32_t k = 0; k < np; k++) acc += (double)Yc[(int64_t)pq[k] * nbN + e];
     pL[(int64_t)t * Ns + e] += alpha * acc;
+  }
+}
+
+// As k_chunk_scatter_framed for product columns whose length nbN is a multiple of 4:
+// four consecutive entries per lane, 16-byte loads (same summation order per entry)
+__global__ void k_chunk_scatter_framed4(double *__restrict__ pL, const float4 *__restrict__ Yc, int Ns, int nbN4,
+                                        const int32_t *__restrict__ cstart, int64_t nl, int32_t t0, double alpha,
+                                        ChunkArgs ca) {
+  __shared__ int32_t pq[SCATTER_MAXP];
+  __shared__ int32_t npair;
+  const int32_t t = t0 + (int32_t)blockIdx.x;
+  if (threadIdx.x < 64) {
+    const int c = threadIdx.x;
+    int32_t p0 = 0, n = 0;
+    if (c < ca.nclass) {
+      const int64_t ec = (int64_t)c * nl + t;
+      p0 = cstart[ec];
+      n = cstart[ec + 1] - p0;
+    }
+    int32_t v = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t u = __shfl_up(v, off, 64);
+      if (c >= off) v += u;
+    }
+    const int32_t start = v - n;
+    const int32_t qoff = c < ca.nclass ? ca.co[c] - ca.cs[c] : 0;
+    for (int32_t k = 0; k < n; k++) pq[start + k] = p0 + k + qoff;
+    if (c == 63) npair = v;
+  }
+  __syncthreads();
+  const int32_t np = npair;
+  for (int e = threadIdx.x; e < nbN4; e += blockDim.x) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+    for (int32_t k = 0; k < np; k++) {
+      const float4 y = Yc[(int64_t)pq[k] * nbN4 + e];
+      a0 += (double)y.x;
+      a1 += (double)y.y;
+      a2 += (double)y.z;
+      a3 += (double)y.w;
+    }
+    double2 *o = reinterpret_cast<double2 *>(pL + (int64_t)t * Ns + 4 * e);
+    double2 u = o[0], w = o[1];
+    u.x += alpha * a0;
+    u.y += alpha * a1;
+    w.x += alpha * a2;
+    w.y += alpha * a3;
+    o[0] = u;
+    o[1] = w;
   }
 }
 
@@ -695,8 +743,14 @@ void m2l_dense_level(Plan &plan, int l) {
         k_class_m2l32<1><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32, li.csrc.p, li.cfar.p, o.gsel.p,
                                                 o.fsel.p, o.offs.p, ca, ntiles);
       FFMM_LAUNCH_CHECK();
-      k_chunk_scatter_framed<<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, nbN, li.cstart.p, plan.nl[l], t0, alpha,
-                                                         ca);
+      if (nbN % 4 == 0) {
+        const unsigned bt4 = (unsigned)std::min(1024, (nbN / 4 + 63) / 64 * 64);
+        k_chunk_scatter_framed4<<<(unsigned)nt, bt4, 0, s>>>(pL, reinterpret_cast<const float4 *>(plan.Yc32.p), Ns,
+                                                              nbN / 4, li.cstart.p, plan.nl[l], t0, alpha, ca);
+      } else {
+        k_chunk_scatter_framed<<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, nbN, li.cstart.p, plan.nl[l], t0,
+                                                           alpha, ca);
+      }
       FFMM_LAUNCH_CHECK();
       continue;
     }
