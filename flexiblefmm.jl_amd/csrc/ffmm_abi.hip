@@ -534,6 +534,14 @@ int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const doub
     lv->reps.upload(reps, (size_t)o.nclass * N * N, s);
     std::vector<float> r32(reps, reps + (size_t)o.nclass * N * N);   // the mixed form's class GEMMs
     lv->reps32.upload(r32.data(), r32.size(), s);
+    std::vector<float> rr;   // held until the copies are done
+    if (N <= 128) {
+      rr.assign((size_t)o.nclass * 128 * 128, 0.f);
+      for (int c = 0; c < o.nclass; c++)
+        for (int k = 0; k < N; k++)
+          for (int i = 0; i < N; i++) rr[((size_t)c * 128 + i) * 128 + k] = r32[(size_t)c * N * N + i + (size_t)N * k];
+      lv->reps32r.upload(rr.data(), rr.size(), s);
+    }
     ctx.sync();
     std::lock_guard<std::mutex> lk(o.hmu);
     for (auto &h : o.hlev)

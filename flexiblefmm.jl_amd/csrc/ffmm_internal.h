@@ -214,6 +214,7 @@ struct HarmonicLevel {
   DBuf<double> M2M, L2L;   // NB x 8NB, 8NB x NB (NB = nb N; the dense Stokes forms for nb = 4)
   DBuf<double> reps;       // nclass x (N x N)
   DBuf<float> reps32;      // the same in FP32 (the mixed form's M2L)
+  DBuf<float> reps32r;     // FP32, row-major, zero-padded to 128 x 128 per class (N <= 128: the fused mixed M2L)
 };
 
 // device pointers of the recurrence tables (HarmonicTables, src/operators/harmonics.jl:163-187)

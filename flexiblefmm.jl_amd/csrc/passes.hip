@@ -309,15 +309,16 @@ constexpr int FM_BS = 132;      // LDS stride of a gathered column (16-byte alig
 // class's pairs in the chunk, the gathered source expansions (P_f^T m per block, signed,
 // the Stokes coupling folded into the fourth block, src/device_passes.jl:449-473) go
 // straight into LDS as FP32 and are multiplied by the class operator on FP32 MFMA
-// (v_mfma_f32_16x16x4f32).  Each wave keeps its 16 operator rows in registers; the four
-// K slots of an MFMA step take four consecutive k, so a lane's operands for 16 k come
-// in one 16-byte load.  The products are written in each pair's target frame (entry j =
-// fsel position, sign applied), so the scatter adds contiguous columns.  Workgroups take
-// contiguous runs of tiles, so the operator rows are loaded again only where the class
-// changes.  N = (p + 1)^2 <= 128.
+// (v_mfma_f32_16x16x4f32).  Each wave loads its 16 operator rows per tile from a
+// row-major copy of the class operators padded to 128 x 128 (L2-resident), while the
+// other waves finish the gather, so that no register is held across the tile's phases;
+// the four K slots of an MFMA step take four consecutive k, so a lane's operands for
+// 16 k come in one 16-byte load, of the operator and of the gathered column alike.  The
+// products are written in each pair's target frame (entry j = fsel position, sign
+// applied), so the scatter adds contiguous columns.  N = (p + 1)^2 <= 128.
 template <int NB>
 __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4))) k_class_m2l32(float *__restrict__ Yc, const double *__restrict__ Mp,
-                                                          int Ns, int N, const float *__restrict__ reps32,
+                                                          int Ns, int N, const float *__restrict__ reps32r,
                                                           const int32_t *__restrict__ csrc,
                                                           const int32_t *__restrict__ cfar,
                                                           const int32_t *__restrict__ gsel,
@@ -336,9 +337,6 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
   // the K padding of the gathered columns: zero (the gather and the staging write k < N only)
   const int kpad = 16 * nkb - N;
   for (int e = tid; e < FM_COLS * kpad; e += FM_TB) Bs[(e / kpad) * FM_BS + N + e % kpad] = 0.f;
-  // operator row 16 w + li, k = 16 kb + 4 lk + s, for kb < 8, s < 4
-  fm_f32x4 a4[8];
-  int cur = -1;
   for (int32_t tile = t0; tile < t1; tile++) {
     int c = 0, h = ca.nclass;
     while (h - c > 1) {
@@ -348,18 +346,6 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
     }
     const int32_t lp0 = (tile - ca.to[c]) * PPT;                       // first class-local pair
     const int npt = min(PPT, ca.co[c + 1] - ca.co[c] - lp0);           // pairs in the tile
-    if (c != cur) {
-      const float *__restrict__ A = reps32 + (size_t)c * N * N;
-      const int i = 16 * w + li;
-#pragma unroll
-      for (int kb = 0; kb < 8; kb++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int k = 16 * kb + 4 * lk + q;
-          a4[kb][q] = (i < N && k < N) ? A[i + (size_t)N * k] : 0.f;
-        }
-      cur = c;
-    }
     if (tid < npt) {
       const int32_t p = ca.cs[c] + lp0 + tid;
       pf[tid] = cfar[p];
@@ -384,6 +370,14 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
           Bs[pl * FM_BS + k] = (float)(sg * m[0]);
         }
       }
+    // operator row 16 w + li, k = 16 kb + 4 lk + q (q < 4): one 16-byte load per kb from
+    // the padded row-major table (zero beyond N), in flight while the other waves gather
+    fm_f32x4 a4[8];
+    if (rows) {
+      const float *__restrict__ Ar = reps32r + ((size_t)c * FM_MP + 16 * w + li) * FM_MP + 4 * lk;
+#pragma unroll
+      for (int kb = 0; kb < 8; kb++) a4[kb] = *reinterpret_cast<const fm_f32x4 *>(Ar + 16 * kb);
+    }
     __syncthreads();
     // products: wave w, row tile w, all 8 column tiles
     fm_f32x4 acc[8];
@@ -588,6 +582,12 @@ static void gemm32(Context &ctx, int m, int n, int k, const float *A, int lda, c
                           C, ldc));
 }
 
+// the harmonic methods in the mixed form with at most 128 coefficients per block take
+// the fused gather + product kernel
+static bool fused_m2l(const OpCache &o) {
+  return o.method == METHOD_HARMONIC && o.hN <= FM_MP && (o.nb == 1 || o.nb == 4);
+}
+
 void allocate_pass_buffers(Plan &plan) {
   OpCache &o = *plan.ops;
   const int Ns = o.Ns, Nl = o.Nl;
@@ -619,8 +619,8 @@ void allocate_pass_buffers(Plan &plan) {
   for (auto &li : plan.lists) pmax = std::max(pmax, li.chunk_max);
   if (plan.m2l == FFMM_M2L_DENSE && plan.precision == FFMM_PREC_MIXED) {
     // the mixed form: the class GEMMs in FP32 (FP32 MFMA), gathered from and scattered
-    // into the FP64 expansions
-    plan.Gc32.resize((size_t)Ns * pmax);
+    // into the FP64 expansions (the fused form gathers straight into LDS)
+    if (!fused_m2l(o)) plan.Gc32.resize((size_t)Ns * pmax);
     plan.Yc32.resize((size_t)Ns * pmax);
   } else if (plan.m2l == FFMM_M2L_DENSE) {
     plan.Gc.resize((size_t)Ns * pmax);
@@ -718,9 +718,7 @@ void m2l_dense_level(Plan &plan, int l) {
   const unsigned bt = (unsigned)std::min(1024, (nbN + 63) / 64 * 64);   // scatter: a lane per entry
   const unsigned btg = (unsigned)std::min(1024, (N + 63) / 64 * 64);    // gather: a lane per coefficient
   const int64_t nch = (int64_t)li.chunk_t.size() - 1;
-  // the harmonic methods in the mixed form with at most 128 coefficients per block: the
-  // fused gather + product kernel
-  const bool fused = f32 && harm && N <= FM_MP && (nb == 1 || nb == 4);
+  const bool fused = f32 && plan.m2l == FFMM_M2L_DENSE && fused_m2l(o);
   for (int64_t k = 0; k < nch; k++) {
     ChunkArgs ca{};
     ca.nclass = o.nclass;
@@ -732,15 +730,16 @@ void m2l_dense_level(Plan &plan, int l) {
     const int32_t t0 = li.chunk_t[k], nt = li.chunk_t[k + 1] - t0;
     if (npk == 0 || nt == 0) continue;
     if (fused) {
+      const float *reps32r = plan.hl[l]->reps32r.p;
       const int ppt = FM_COLS / nb;
       for (int c = 0; c < o.nclass; c++) ca.to[c + 1] = ca.to[c] + (ca.co[c + 1] - ca.co[c] + ppt - 1) / ppt;
       const int32_t ntiles = ca.to[o.nclass];
       const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 4 * 256);
       if (nb == 4)
-        k_class_m2l32<4><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32, li.csrc.p, li.cfar.p, o.gsel.p,
+        k_class_m2l32<4><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32r, li.csrc.p, li.cfar.p, o.gsel.p,
                                                 o.fsel.p, o.offs.p, ca, ntiles);
       else
-        k_class_m2l32<1><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32, li.csrc.p, li.cfar.p, o.gsel.p,
+        k_class_m2l32<1><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32r, li.csrc.p, li.cfar.p, o.gsel.p,
                                                 o.fsel.p, o.offs.p, ca, ntiles);
       FFMM_LAUNCH_CHECK();
       if (nbN % 4 == 0) {
