@@ -93,6 +93,41 @@ def test_stokes_harmonics_vs_oracle_and_direct(nd):
     assert rel(v, HO.direct("stokes", pts, dens, tgt)) < 1e-3
 
 
+@pytest.mark.parametrize("case", ["laplace-p3", "yukawa-p13", "stokes9-p16"])
+def test_s2m_tiles_vs_oracle(case):
+    """The S2M runs over tiles of S sources with 64 / S lanes per source, S set by p (the
+    harmonics of a tile fit 8 KiB): p = 3 gives 32 sources of 2 lanes, p = 13 (odd: the
+    order pairs (m, p - m) end without a middle order) 4 sources of 16 lanes with the
+    Yukawa radial weights, p = 16 2 sources of 32 lanes with stresslets.  The multipoles
+    (S2M, then M2M) against the oracle's, FP64 both."""
+    import ffmm
+    kind, p = case.split("-p")
+    p = int(p)
+    rng = np.random.default_rng(13)
+    n = 3000
+    pts = mixed(rng, n)
+    if kind == "laplace":
+        m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=p)
+        dens = rng.standard_normal(n)
+    elif kind == "yukawa":
+        m = ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=p)
+        dens = rng.standard_normal(n)
+    else:
+        m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=p)
+        dens = rng.standard_normal((n, 9))
+        dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] s2m tiles {case} n={n}", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    plan.matvec(dens, ffmm.Velocity if kind == "stokes" else ffmm.Potential)
+    M, _, _ = plan.download_expansions()
+    tree = plan.download_tree()
+    f, _, _ = oracle_for(m, cache, pts, 40, tree.perm)
+    assert f.tree.nnodes == plan.nnodes
+    Mo = f.upward(np.ascontiguousarray(dens[f.tree.perm]))
+    assert np.max(np.abs(M.T - Mo)) <= 1e-11 * np.max(np.abs(Mo))
+
+
 def test_harmonic_sorted_densities_linearity_and_errors():
     import ffmm
     rng = np.random.default_rng(13)
