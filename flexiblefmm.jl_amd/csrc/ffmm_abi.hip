@@ -812,7 +812,13 @@ void plan_build(Plan &p, const double *tau, int nd, int sorted) {
   const bool stokes = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET;
   if (stokes && nd != 3 && nd != 9)
     fail(FFMM_ERR_ARGUMENT, "StokesHarmonics takes stokeslets (3 components) or stokeslets and stresslets (9)");
-  if (!stokes && nd != 1) fail(FFMM_ERR_ARGUMENT, "this method takes one charge per point");
+  // SphericalHarmonics(Laplace3D()) also takes a charge and a dipole per point
+  // (supports_density, src/operators/harmonics.jl:317-320)
+  const bool dipoles = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_LAPLACE;
+  if (!stokes && nd != 1 && !(dipoles && nd == 4))
+    fail(FFMM_ERR_ARGUMENT, dipoles ? "SphericalHarmonics(Laplace3D()) takes charges (1 per point) or charges and "
+                                      "dipoles (4 per point: q, d)"
+                                    : "this method takes one charge per point");
   set_device(*p.ctx);
   AllocScope scope(*p.ctx);
   if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_YUKAWA) {

@@ -87,10 +87,12 @@ __device__ __forceinline__ int hidx(int n, int m, int im, int z) {
 // tile Y. Phase 2: lane t folds the tile into the entries e = t, t + 64, ... that it
 // owns, in registers, and keeps them in an LDS column between tiles:
 //   charges, stokeslets: M_b(e) += w_b conj(Y(e)) per block b;
-//   stresslet dipoles d_k (src/operators/harmonics.jl:718-752): entry (n, m) takes
+//   Laplace dipoles and stresslet dipoles d_k (src/operators/harmonics.jl:711-752,
+//   one block, or four): entry (n, m) takes
 //   Y of degree n - 1 and orders m - 1, m, m + 1 against the components of d_k,
 //   M_k(e) += a0 d_k0 + a1 d_k1 + a2 d_k2.
-// KIND: 0 Laplace3D charges, 1 Yukawa3D charges, 2 Stokeslet3D (nd = 3 or 9)
+// KIND: 0 Laplace3D charges (nd = 1) or charges + dipoles (nd = 4), 1 Yukawa3D charges,
+// 2 Stokeslet3D (nd = 3 or 9)
 template <int KIND>
 __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nodes, const double *__restrict__ centers,
                                                  int64_t nn, const int32_t *__restrict__ level,
@@ -102,7 +104,8 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
   extern __shared__ double sm[];
   const int N = (p + 1) * (p + 1), NY = N + 1;   // a row of the tile ends in a zero slot
   constexpr int nb = KIND == 2 ? 4 : 1;
-  constexpr int NW = KIND == 2 ? 16 : 1;          // per source: w_0..3, then d_k (4 x 3)
+  // per source: w_0..3, then d_k (4 x 3); Laplace: the charge, then the dipole at 4..6
+  constexpr int NW = KIND == 2 ? 16 : KIND == 0 ? 8 : 1;
   double *acc = sm;                               // nb x N
   double *Y = acc + nb * N;                       // S x NY
   double *W = Y + (size_t)S * NY;                 // S x NW
@@ -113,7 +116,7 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
   const int lv = level[node];
   const double s = hscale[lv];
   const int32_t j0 = plo[node], j1 = phi[node];
-  const bool dip = KIND == 2 && nd == 9;
+  const bool dip = (KIND == 2 && nd == 9) || (KIND == 0 && nd == 4);
   for (int i = t; i < nb * N; i += HB) acc[i] = 0.0;
   for (int i = t; i < S; i += HB) Y[(size_t)i * NY + N] = 0.0;
   for (int32_t jb = j0; jb <= j1; jb += S) {
@@ -145,7 +148,12 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
           }
         }
       } else {
-        w[0] = dens[j];
+        w[0] = dens[(int64_t)nd * j];
+        if (KIND == 0 && nd == 4) {
+          // a dipole d: Dipole(d s) (_scaled_density, src/operators/harmonics.jl:436)
+#pragma unroll
+          for (int c = 0; c < 3; c++) w[4 + c] = dens[4 * (int64_t)j + 1 + c] * s;
+        }
         if (KIND == 1) {
           const double kh = khat[lv];
           yukawa_u(kh * kh * (x * x + y * y + z * z), p + 1, U + sl * (p + 2));
@@ -208,7 +216,7 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
           const double a2 = k2 * yr[i2];
           const double *d = W + q * NW + 4;
 #pragma unroll
-          for (int k = 0; k < 4; k++) a[k] += a0 * d[3 * k] + a1 * d[3 * k + 1] + a2 * d[3 * k + 2];
+          for (int k = 0; k < nb; k++) a[k] += a0 * d[3 * k] + a1 * d[3 * k + 1] + a2 * d[3 * k + 2];
         }
       }
 #pragma unroll
@@ -304,6 +312,27 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
   }
   const double q = f[0];
   double v, dv;
+  if (KIND == 0 && nd == 4) {
+    // a dipole d (source_field and target_field of Dipole, src/sources.jl:138, 151):
+    // potential d.r / (4 pi r^3), gradient (d / r^3 - 3 (d.r) r / r^5) / (4 pi)
+    const double ir3 = ir * ir * ir, dr = f[1] * d0 + f[2] * d1 + f[3] * d2;
+    const double gs = -3.0 * HC4PI * dr * ir3 * ir * ir;
+    const double pv = HC4PI * dr * ir3;
+    const double g0 = HC4PI * f[1] * ir3 + gs * d0, g1 = HC4PI * f[2] * ir3 + gs * d1,
+                 g2 = HC4PI * f[3] * ir3 + gs * d2;
+    if (OUT == FFMM_POTENTIAL) {
+      acc[0] += keep ? pv : 0.0;
+    } else if (OUT == FFMM_GRADIENT) {
+      acc[0] += keep ? g0 : 0.0;
+      acc[1] += keep ? g1 : 0.0;
+      acc[2] += keep ? g2 : 0.0;
+    } else {
+      acc[0] += keep ? pv : 0.0;
+      acc[1] += keep ? g0 : 0.0;
+      acc[2] += keep ? g1 : 0.0;
+      acc[3] += keep ? g2 : 0.0;
+    }
+  }
   if (KIND == 0) {
     v = HC4PI * ir;
     dv = -HC4PI * ir * ir * ir;
@@ -329,7 +358,7 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
 template <int KIND, int OUT>
 __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   extern __shared__ double sm[];
-  constexpr int TW = KIND == 2 ? 12 : 4;   // doubles per tile entry: position and density components
+  constexpr int TW = KIND == 2 ? 12 : 7;   // doubles per tile entry: position and density components
   __shared__ double tile[HT_CK * TW];
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
@@ -518,7 +547,7 @@ void harmonic_s2m_level(Plan &plan, int l) {
   // sources per tile: the most (a power of two, at most 64) whose harmonics fit 8 KiB
   int S = 64;
   while (S > 1 && (size_t)S * (N + 1) * sizeof(double) > 8192) S /= 2;
-  const int nw = o.kernel == FFMM_KERNEL_STOKESLET ? 16 : 1;
+  const int nw = o.kernel == FFMM_KERNEL_STOKESLET ? 16 : o.kernel == FFMM_KERNEL_LAPLACE ? 8 : 1;
   const size_t lds = sizeof(double) * ((size_t)o.Ns + (size_t)S * (N + 1) + (size_t)S * nw +
                                        (o.kernel == FFMM_KERNEL_YUKAWA ? (size_t)S * (p + 2) : 0));
   const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l];

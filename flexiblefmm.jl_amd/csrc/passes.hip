@@ -603,7 +603,10 @@ void allocate_pass_buffers(Plan &plan) {
   plan.Mp.resize((size_t)Ns * plan.nnodes);
   plan.Pl.resize((size_t)Ns * plan.nnodes);
   plan.Lc.resize((size_t)Nl * std::max<int64_t>(nleaves, 1));
-  plan.tau.resize((size_t)plan.N * (o.method == METHOD_HARMONIC && o.nb == 4 ? 9 : 1));
+  // the densities, at the widest the method takes: Stokes 9, SphericalHarmonics(Laplace3D())
+  // 4 (a charge and a dipole), else 1
+  const int ndmax = o.method != METHOD_HARMONIC ? 1 : o.nb == 4 ? 9 : o.kernel == FFMM_KERNEL_LAPLACE ? 4 : 1;
+  plan.tau.resize((size_t)plan.N * ndmax);
   if (o.method == METHOD_CUBE) {
     plan.pM.resize((size_t)Nl * nq);
     plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);

@@ -683,7 +683,8 @@ class DevicePlan:
     def _join_densities(self, tau, sorted: bool):
         """One density array over all points from the groups' (``None``: targets only,
         zero densities); a Stokes group of stokeslets alone (nd = 3) joins others with
-        stresslets (nd = 9) with a zero stresslet part."""
+        stresslets (nd = 9) with a zero stresslet part, and a group of charges (nd = 1)
+        joins groups of charges and dipoles (nd = 4) with a zero dipole."""
         torch = _torch()
         G = self.ngroups
         if not isinstance(tau, (tuple, list)):
@@ -697,7 +698,7 @@ class DevicePlan:
         if not nds:
             nds = {3 if isinstance(self.method, StokesHarmonics) else 1}
         nd = 9 if nds == {3, 9} else max(nds)
-        if len(nds) > 1 and nds != {3, 9}:
+        if len(nds) > 1 and nds not in ({3, 9}, {1, 4}):
             raise ArgumentError("the groups' densities do not combine")
         for g, t in enumerate(parts):
             if t is not None and t.shape[0] != self.group_sizes[g]:
@@ -710,6 +711,8 @@ class DevicePlan:
                 continue
             if t.ndim == 2 and t.shape[1] == 3 and nd == 9:
                 t = torch.cat([t, torch.zeros((t.shape[0], 6), dtype=t.dtype, device=t.device)], 1)
+            if t.ndim == 1 and nd == 4:
+                t = torch.cat([t[:, None], torch.zeros((t.shape[0], 3), dtype=t.dtype, device=t.device)], 1)
             if sorted:
                 out[pos[g]] = t
             else:
@@ -724,9 +727,11 @@ class DevicePlan:
 
     # -- the passes (device tensors in and out) --
     def build_expansions(self, tau, sorted: bool = False):
-        """Densities: N charges, or for Stokes (N, 3) stokeslets or (N, 9) rows
-        (f, g, nu) of a stokeslet f plus a stresslet (g, nu); with groups, a tuple of
-        them, one per group (``None`` for a group of targets only)."""
+        """Densities: N charges; for SphericalHarmonics(Laplace3D()) also (N, 4) rows
+        (q, d) of a charge q and a dipole d (``Dipole{3}``, src/sources.jl:17-29; a
+        dipole alone has q = 0); for Stokes (N, 3) stokeslets or (N, 9) rows (f, g, nu)
+        of a stokeslet f plus a stresslet (g, nu); with groups, a tuple of them, one per
+        group (``None`` for a group of targets only)."""
         t = self._join_densities(tau, sorted and self.ngroups > 1)
         if t.shape[0] != self.N or t.ndim > 2:
             raise DimensionMismatch(f"expected {self.N} entries in group 1")

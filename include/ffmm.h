@@ -194,7 +194,10 @@ int ffmm_build_expansions(ffmm_plan plan, const double *tau, int sorted);
 /* The same from densities of nd components per point (dens: nd x N, point major):
  * nd = 1 charges (any method); for Stokes nd = 3 stokeslets f, or nd = 9 a stokeslet f
  * plus a stresslet (f, nu) at every point (the tuple densities of the reference,
- * (SVector{3}, Stresslet{3}), src/sources.jl:55-66; src/operators/harmonics_stokes.jl:197-217). */
+ * (SVector{3}, Stresslet{3}), src/sources.jl:55-66; src/operators/harmonics_stokes.jl:197-217);
+ * for SphericalHarmonics(Laplace3D()) nd = 4, a charge q and a dipole d per point, rows
+ * (q, d) (Dipole{3} densities, src/sources.jl:17-29, 138, 151; supports_density,
+ * src/operators/harmonics.jl:317-320). */
 int ffmm_build_expansions_nd(ffmm_plan plan, const double *dens, int nd, int sorted);
 
 /* Output at every source, in the user's order, from all other sources:

@@ -263,7 +263,14 @@ void oh_s2m(int kind, int p, const double *ta, const double *tbb, const double *
                        z = (pts[3 * j + 2] - centers[3 * i + 2]) * s;
                 fold(&tb, x, y, z, p, R, I);
                 if (kind == 0) {
-                    acc_charge(col, &tb, R, I, p, dens[j], NULL);
+                    /* nd = 4: a charge and a dipole per point, the dipole scaled by s
+                     * (_scaled_density, src/operators/harmonics.jl:436) */
+                    acc_charge(col, &tb, R, I, p, dens[(int64_t)nd * j], NULL);
+                    if (nd == 4) {
+                        double d[3];
+                        for (int c = 0; c < 3; c++) d[c] = dens[4 * (int64_t)j + 1 + c] * s;
+                        acc_dipole(col, &tb, R, I, p, d, NULL);
+                    }
                 } else if (kind == 1) {
                     double kh = khat[level[i]];
                     yuk_u(kh * kh * (x * x + y * y + z * z), p + 1, u);
@@ -353,6 +360,19 @@ void oh_l2t(int kind, int p, const double *ta, const double *tbb, const double *
     }
 }
 
+/* the Laplace field of a dipole d at r = target - source (source_field and
+ * target_field of Dipole, src/sources.jl:138, 151): potential d.r / (4 pi r^3),
+ * gradient (d / r^3 - 3 (d.r) r / r^5) / (4 pi); output 0 potential, 1 gradient, 2 both */
+static void dipole_field(const double *d, double dx, double dy, double dz, double ir, int output, double acc[4]) {
+    const double c4 = 1.0 / (4.0 * PI);
+    const double ir3 = ir * ir * ir, dr = d[0] * dx + d[1] * dy + d[2] * dz;
+    const double pv = c4 * dr * ir3, gs = -3.0 * c4 * dr * ir3 * ir * ir;
+    const double g0 = c4 * d[0] * ir3 + gs * dx, g1 = c4 * d[1] * ir3 + gs * dy, g2 = c4 * d[2] * ir3 + gs * dz;
+    if (output == 0) acc[0] += pv;
+    else if (output == 1) { acc[0] += g0; acc[1] += g1; acc[2] += g2; }
+    else { acc[0] += pv; acc[1] += g0; acc[2] += g1; acc[3] += g2; }
+}
+
 /* near field over the 27 colleagues, added into out; tskip: sorted source index to
  * exclude (-1 none) */
 void oh_near(int kind, double kappa, const double *tx, const int32_t *tnode, const int32_t *tskip, int64_t nt,
@@ -390,8 +410,10 @@ void oh_near(int kind, double kappa, const double *tx, const int32_t *tnode, con
                     }
                     continue;
                 }
-                double t = dens[j];
+                double t = dens[(int64_t)nd * j];
                 double v, dv;      /* value and radial derivative (times 1/r) */
+                if (kind == 0 && nd == 4)
+                    dipole_field(dens + 4 * (int64_t)j + 1, dx, dy, dz, ir, output, acc);
                 if (kind == 0) {
                     v = c4 * ir;
                     dv = -c4 * ir * ir * ir;
@@ -439,7 +461,8 @@ void oh_direct(int kind, double kappa, const double *src, const double *dens, in
                 }
                 continue;
             }
-            double t = dens[j], v, dv;
+            double t = dens[(int64_t)nd * j], v, dv;
+            if (kind == 0 && nd == 4) dipole_field(dens + 4 * j + 1, dx, dy, dz, ir, output, acc);
             if (kind == 0) { v = c4 * ir; dv = -c4 * ir * ir * ir; }
             else {
                 double e = exp(-kappa * r2 * ir);

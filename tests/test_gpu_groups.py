@@ -109,3 +109,25 @@ def test_stokes_groups_of_different_densities():
     d[len(xs1):] = d2
     w = one.matvec(d, ffmm.Velocity).cpu().numpy()
     assert np.array_equal(v1.cpu().numpy(), w[:len(xs1)]) and np.array_equal(v2.cpu().numpy(), w[len(xs1):])
+
+
+def test_laplace_groups_of_charges_and_dipoles():
+    """A group of charges (N) beside a group of charges and dipoles (N x 4, rows (q, d)):
+    joined with a zero dipole on the first, as one plan with both (the reference's
+    groups of different density types, src/device_passes.jl:113-134)."""
+    import ffmm
+    rng = np.random.default_rng(34)
+    xs1, xs2 = mixed(rng, 2500), mixed(rng, 2000)
+    q1 = rng.standard_normal(len(xs1))
+    d2 = rng.standard_normal((len(xs2), 4))
+    m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=6)
+    cache = ffmm.OperatorCache(m)
+    print("[groups] laplace 1 + 4", flush=True)
+    plan = ffmm.DevicePlan(m, (xs1, xs2), ppl=40, bbox=BOX, cache=cache)
+    v1, v2 = plan.matvec((q1, d2), ffmm.PotentialGradient)
+    one = ffmm.DevicePlan(m, np.vstack([xs1, xs2]), ppl=40, bbox=BOX, cache=cache)
+    d = np.zeros((len(xs1) + len(xs2), 4))
+    d[:len(xs1), 0] = q1
+    d[len(xs1):] = d2
+    w = one.matvec(d, ffmm.PotentialGradient).cpu().numpy()
+    assert np.array_equal(v1.cpu().numpy(), w[:len(xs1)]) and np.array_equal(v2.cpu().numpy(), w[len(xs1):])

@@ -128,6 +128,47 @@ def test_s2m_tiles_vs_oracle(case):
     assert np.max(np.abs(M.T - Mo)) <= 1e-11 * np.max(np.abs(Mo))
 
 
+def test_laplace_dipoles_vs_oracle_and_direct():
+    """SphericalHarmonics(Laplace3D()) with a charge and a dipole per point (rows (q, d),
+    Dipole{3} densities, src/sources.jl:17-29): potential and gradient at the sources and
+    at targets, and the multipoles, against the harmonic oracle (FP64 both: agreement to
+    rounding) and against FP64 direct sums (the truncation of p = 10); a dipole-only
+    density is the dipole part of the sum; Yukawa refuses dipoles."""
+    import ffmm
+    rng = np.random.default_rng(14)
+    n = 4000
+    pts = mixed(rng, n)
+    dens = rng.standard_normal((n, 4))
+    tgt = uniform(rng, 300)
+    m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=10)
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] laplace dipoles n={n} p=10", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    u = plan.matvec(dens, ffmm.PotentialGradient).cpu().numpy()
+    v = plan.evaluate(tgt, ffmm.PotentialGradient).cpu().numpy()
+    M, _, _ = plan.download_expansions()
+    f, _, _ = oracle_for(m, cache, pts, 40, plan.download_tree().perm)
+    uo, vo = f.matvec(dens, output=2, targets=tgt)
+    assert rel(u, uo) < 1e-11 and rel(v, vo) < 1e-11
+    assert np.max(np.abs(M.T - f.multipoles)) <= 1e-11 * np.max(np.abs(f.multipoles))
+    ud = HO.direct("laplace", pts, dens, pts, 2, skip=np.arange(n))
+    vd = HO.direct("laplace", pts, dens, tgt, 2)
+    assert rel(u, ud) < 3e-5 and rel(v, vd) < 3e-5
+    # linearity in the density kind: the charge part plus the dipole part
+    dq = dens.copy()
+    dq[:, 1:] = 0.0
+    dd = dens.copy()
+    dd[:, 0] = 0.0
+    uq = plan.matvec(dq, ffmm.PotentialGradient).cpu().numpy()
+    ud2 = plan.matvec(dd, ffmm.PotentialGradient).cpu().numpy()
+    uc = plan.matvec(dens[:, 0].copy(), ffmm.PotentialGradient).cpu().numpy()
+    assert rel(uq + ud2, u) < 1e-13 and rel(uq, uc) < 1e-13
+    my = ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=6)
+    py = ffmm.DevicePlan(my, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(my))
+    with pytest.raises(ffmm.ArgumentError, match="one charge per point"):
+        py.matvec(dens, ffmm.Potential)
+
+
 def test_harmonic_sorted_densities_linearity_and_errors():
     import ffmm
     rng = np.random.default_rng(13)

@@ -288,3 +288,42 @@ def test_oracle_fmm_against_direct_sums(lap, kind):
         vd = HO.direct(kind, pts, tau, tgt, 2, kappa=kappa)
         assert rel(u[:, 0], ud[:, 0]) < 2e-5 and rel(v[:, 0], vd[:, 0]) < 2e-5
         assert rel(u[:, 1:], ud[:, 1:]) < 1e-4 and rel(v[:, 1:], vd[:, 1:]) < 1e-4
+
+
+def test_oracle_dipole_kernel_is_the_charge_kernel_differenced():
+    """The Laplace field of a dipole d at s (potential d . grad_s K, src/sources.jl:138;
+    gradient -(hess_t K) d, src/sources.jl:151) against the central difference of two
+    charges +-1/(2h) at s +- h d, at sources and separate targets, potential and gradient."""
+    rng = np.random.default_rng(21)
+    src = rng.random((40, 3))
+    tgt = rng.random((30, 3)) + 1.5
+    d = rng.standard_normal((40, 3))
+    h = 1e-4
+    dens = np.hstack([np.zeros((40, 1)), d])
+    got = HO.direct("laplace", src, dens, tgt, 2)
+    pair = np.vstack([src + h * d, src - h * d])
+    q = np.concatenate([np.full(40, 1 / (2 * h)), np.full(40, -1 / (2 * h))])
+    want = HO.direct("laplace", pair, q, tgt, 2)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-9 * np.max(np.abs(want)))
+    # a charge and a dipole per point add: the (q, d) rows equal the two parts summed
+    qd = rng.standard_normal(40)
+    both = HO.direct("laplace", src, np.hstack([qd[:, None], d]), tgt, 2)
+    assert np.allclose(both, got + HO.direct("laplace", src, qd, tgt, 2), rtol=1e-13, atol=1e-14)
+
+
+def test_oracle_fmm_with_dipoles_against_direct_sums(lap):
+    """Laplace charge + dipole densities through the harmonic oracle (S2M by the dipole
+    accumulation of src/operators/harmonics.jl:711-752, the near field by the dipole
+    kernel) against FP64 direct sums: the truncation error of p = 10."""
+    rng = np.random.default_rng(22)
+    n = 2500
+    pts = mixed(rng, n)
+    tgt = uniform(rng, 150)
+    f = HO.HarmonicFMM("laplace", lambda l, w: (lap, 1.0 / w), pts, 40, np.zeros(3), 1.0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)   # noqa: E731
+    dens = rng.standard_normal((n, 4))
+    u, v = f.matvec(dens, output=2, targets=tgt)
+    ud = HO.direct("laplace", pts, dens, pts, 2, skip=np.arange(n))
+    vd = HO.direct("laplace", pts, dens, tgt, 2)
+    assert rel(u[:, 0], ud[:, 0]) < 3e-5 and rel(v[:, 0], vd[:, 0]) < 3e-5
+    assert rel(u[:, 1:], ud[:, 1:]) < 3e-5 and rel(v[:, 1:], vd[:, 1:]) < 3e-5
