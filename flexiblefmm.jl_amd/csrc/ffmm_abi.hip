@@ -852,9 +852,11 @@ void plan_build(Plan &p, const double *tau, int nd, int sorted) {
 
 static void check_output(Plan &p, int output) {
   if (!p.expansions_ready) fail(FFMM_ERR_STATE, "build the expansions first");
-  if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
-  if (p.ops->kernel == FFMM_KERNEL_STOKESLET && output != FFMM_POTENTIAL)
-    fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) on the device");
+  if (output < 0 || output > 3) fail(FFMM_ERR_ARGUMENT, "unknown output");
+  const bool stokes = p.ops->kernel == FFMM_KERNEL_STOKESLET;
+  if (stokes && output != FFMM_POTENTIAL && output != FFMM_PRESSURE)
+    fail(FFMM_ERR_ARGUMENT, "StokesHarmonics evaluates the velocity (Potential) or the pressure on the device");
+  if (!stokes && output == FFMM_PRESSURE) fail(FFMM_ERR_ARGUMENT, "Pressure is an output of the Stokes kernels");
 }
 
 void plan_eval_sources(Plan &p, double *out, int output) {

@@ -293,6 +293,19 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
                                           double kappa, double acc[4]) {
   const double r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
   const double ir = rsqrt_f64(r2);
+  if (KIND == 2 && OUT == FFMM_PRESSURE) {
+    // pressure_kernel . f = (d.f) / (4 pi r^3) (src/kernels.jl:267-271); a stresslet
+    // (g, nu): ((g.nu) / r^3 - 3 (d.g)(d.nu) / r^5) / (2 pi) (src/sources.jl:237-242)
+    const double ir3 = ir * ir * ir;
+    double pr = 2 * HC8PI * ir3 * (d0 * f[0] + d1 * f[1] + d2 * f[2]);
+    if (nd == 9) {
+      const double gn = f[3] * f[6] + f[4] * f[7] + f[5] * f[8];
+      const double dg = d0 * f[3] + d1 * f[4] + d2 * f[5], dn = d0 * f[6] + d1 * f[7] + d2 * f[8];
+      pr += 4 * HC8PI * (gn * ir3 - 3 * dg * dn * ir3 * ir * ir);
+    }
+    acc[0] += keep ? pr : 0.0;
+    return;
+  }
   if (KIND == 2) {
     const double a = HC8PI * ir;
     const double b = a * ir * ir * (d0 * f[0] + d1 * f[1] + d2 * f[2]);
@@ -424,10 +437,15 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
             for (int c = 0; c < 3; c++) gP[c] += wt * (o[2 * c] * vr + o[2 * c + 1] * vi);
           }
         });
+      if (OUT == FFMM_PRESSURE) {
+        // -s tr(grad Phi) (_stokes_output(::Pressure), src/operators/harmonics_stokes.jl:348-351)
+        acc[0] = -sc * (G[0][0] + G[1][1] + G[2][2]);
+      } else {
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double gx = G[c][0] * xh + G[c][1] * yh + G[c][2] * zh;
-        acc[c] = (Phi[c] - gx + gP[c]) / 2;
+        for (int c = 0; c < 3; c++) {
+          const double gx = G[c][0] * xh + G[c][1] * yh + G[c][2] * zh;
+          acc[c] = (Phi[c] - gx + gP[c]) / 2;
+        }
       }
     } else {
       double *up = uw + threadIdx.x * (p + 2), *ugr = ug + threadIdx.x * (p + 2);
@@ -519,7 +537,9 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
       for (int c = 0; c < 4; c++) acc[c] += __shfl_xor(acc[c], off, HB);
     if (valid && rep == 0) {
       const int64_t o = a.outidx[j];
-      if (KIND == 2) {
+      if (KIND == 2 && OUT == FFMM_PRESSURE) {
+        a.out[o] = acc[0];
+      } else if (KIND == 2) {
         a.out[3 * o] = acc[0];
         a.out[3 * o + 1] = acc[1];
         a.out[3 * o + 2] = acc[2];
@@ -570,7 +590,8 @@ void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output) {
   hipStream_t s = plan.ctx->stream;
   const unsigned g = (unsigned)nleaves;
   if (o.kernel == FFMM_KERNEL_STOKESLET) {
-    k_eval_harm<2, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    if (output == FFMM_PRESSURE) k_eval_harm<2, FFMM_PRESSURE><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<2, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
   } else if (o.kernel == FFMM_KERNEL_LAPLACE) {
     if (output == FFMM_POTENTIAL) k_eval_harm<0, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
     else if (output == FFMM_GRADIENT) k_eval_harm<0, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);

@@ -6,7 +6,7 @@ in Python over ctypes, and the host FP64 operator precompute.
 """
 from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 from .fmm import (FMM, CubeKernelIndependent, DevicePlan, FMMState, Gradient, Laplace3D, OperatorCache,  # noqa: F401
-                  Potential, PotentialGradient, SphericalHarmonics, StokesHarmonics, Stokeslet3D, Velocity, Yukawa3D,
+                  Potential, PotentialGradient, Pressure, SphericalHarmonics, StokesHarmonics, Stokeslet3D, Velocity, Yukawa3D,
                   build_expansions, build_expansions_, device_build_expansions_, device_matvec_, direct, direct_self,
                   evaluate, evaluate_at_sources, square_bbox)
 from .stokes_api import (StokesFMMKernel, StokesFMMPlan, stokes_densities, stokes_fmm_eval_,  # noqa: F401

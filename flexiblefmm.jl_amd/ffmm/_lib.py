@@ -14,7 +14,7 @@ LIBPATH = os.path.join(_HERE, "libffmm_hip.so")
 FFMM_OK, FFMM_ERR_ARGUMENT, FFMM_ERR_DIMENSION, FFMM_ERR_DEVICE, FFMM_ERR_STATE, FFMM_ERR_MEMORY = range(6)
 PREC_F64, PREC_MIXED = 0, 1
 M2L_DENSE, M2L_SPECTRAL = 0, 1
-POTENTIAL, GRADIENT, POTENTIAL_GRADIENT = 0, 1, 2
+POTENTIAL, GRADIENT, POTENTIAL_GRADIENT, PRESSURE = 0, 1, 2, 3
 KERNEL_LAPLACE, KERNEL_YUKAWA, KERNEL_STOKESLET = 0, 1, 2
 
 

@@ -89,6 +89,12 @@ class PotentialGradient:
     ncomp = 4
 
 
+class Pressure:
+    """The pressure of the Stokes kernels, one value per point (src/sources.jl:34-42)."""
+    code = _lib.PRESSURE
+    ncomp = 1
+
+
 def _output(o):
     o = o() if isinstance(o, type) else o
     if not hasattr(o, "code"):
@@ -740,7 +746,9 @@ class DevicePlan:
         return self
 
     def _ncomp(self, o) -> int:
-        return 3 if isinstance(self.method, StokesHarmonics) else o.ncomp
+        if isinstance(self.method, StokesHarmonics):
+            return 1 if o.code == _lib.PRESSURE else 3
+        return o.ncomp
 
     def evaluate_at_sources(self, output=Potential, out=None):
         o = _output(output)

@@ -9,8 +9,9 @@ targets; ``StokesFMMKernel`` is the one-shot form (src/stokes_api.jl:160-186).
 
 On the device the 3D method is ``StokesHarmonics`` (``method="harmonics"``, the
 reference's analytic Stokes method; its default p here is the reference's 16).  The
-kernel-independent cube method for the Stokes kernel, doublets, pressure and velocity
-gradients are not built: asking for them raises ``ArgumentError``.  The densities
+kernel-independent cube method for the Stokes kernel, doublets and velocity gradients
+are not built: asking for them raises ``ArgumentError``; ``output=Pressure`` gives the
+pressure.  The densities
 become the device plan's (N, 9) rows [f, g, nu]: stokeslet f, stresslet strength g and
 normal nu, each strength multiplied by the source's weight.
 """
@@ -19,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import ArgumentError, DimensionMismatch
-from .fmm import DevicePlan, OperatorCache, StokesHarmonics, Stokeslet3D, Velocity, _output, square_bbox
+from .fmm import DevicePlan, OperatorCache, Pressure, StokesHarmonics, Stokeslet3D, Velocity, _output, square_bbox
 
 
 def _points(x) -> np.ndarray:
@@ -86,8 +87,11 @@ class StokesFMMPlan:
 
 
 def _check_output(output):
-    if type(_output(output)) is not type(_output(Velocity)):
-        raise ArgumentError("the device Stokes path evaluates the velocity")
+    """Velocity (the default) or Pressure (src/stokes_api.jl:100-104); the output object."""
+    o = _output(output)
+    if type(o) not in (type(_output(Velocity)), type(_output(Pressure))):
+        raise ArgumentError("the device Stokes path evaluates the velocity or the pressure")
+    return o
 
 
 def _check_same(given, planned, what: str):
@@ -97,18 +101,19 @@ def _check_same(given, planned, what: str):
 
 def stokes_fmm_eval_(velocities, plan: StokesFMMPlan, *, stokeslets=None, stresslets=None, stressvecs=None,
                      doublets=None, weights=None, sources=None, targets=None, output=Velocity):
-    """``stokes_fmm_eval!`` (src/stokes_api.jl:107-123): the velocity at the plan's
-    targets, or at the sources for a self plan, into ``velocities`` (n x 3)."""
-    _check_output(output)
+    """``stokes_fmm_eval!`` (src/stokes_api.jl:107-123): the velocity (or, with
+    ``output=Pressure``, the pressure) at the plan's targets, or at the sources for a
+    self plan, into ``velocities`` (n x 3; n for the pressure)."""
+    o = _check_output(output)
     _check_same(sources, plan.sources, "sources")
     d = stokes_densities(plan.n, weights, stokeslets, stresslets, stressvecs, doublets)
     p = plan.plan
     p.build_expansions(d)
     if plan.targets is None and targets is None:
-        v = p.evaluate_at_sources(Velocity).cpu().numpy()
+        v = p.evaluate_at_sources(o).cpu().numpy()
     else:
         tg = plan.targets if targets is None else _points(targets)
-        v = p.evaluate(tg, Velocity).cpu().numpy()
+        v = p.evaluate(tg, o).cpu().numpy()
     if velocities is None:
         return v
     if np.shape(velocities) != v.shape:
@@ -121,17 +126,17 @@ def stokes_fmm_eval_src_trg_(vel_src, vel_trg, plan: StokesFMMPlan, *, stokeslet
                              stressvecs=None, doublets=None, weights=None, sources=None, targets=None,
                              output=Velocity):
     """``stokes_fmm_eval_src_trg!`` (src/stokes_api.jl:131-146): one expansion build,
-    the velocity at the sources (self) and at the plan's targets."""
+    the velocity (or the pressure) at the sources (self) and at the plan's targets."""
     if plan.targets is None and targets is None:
         raise ArgumentError("the plan has no targets")
-    _check_output(output)
+    o = _check_output(output)
     _check_same(sources, plan.sources, "sources")
     d = stokes_densities(plan.n, weights, stokeslets, stresslets, stressvecs, doublets)
     p = plan.plan
     p.build_expansions(d)
-    vs = p.evaluate_at_sources(Velocity).cpu().numpy()
+    vs = p.evaluate_at_sources(o).cpu().numpy()
     tg = plan.targets if targets is None else _points(targets)
-    vt = p.evaluate(tg, Velocity).cpu().numpy()
+    vt = p.evaluate(tg, o).cpu().numpy()
     if vel_src is not None:
         vel_src[...] = vs
     if vel_trg is not None:

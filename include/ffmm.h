@@ -43,6 +43,9 @@ extern "C" {
 #define FFMM_POTENTIAL 0
 #define FFMM_GRADIENT 1
 #define FFMM_POTENTIAL_GRADIENT 2
+/* Pressure() of the Stokes kernels, one value per point (src/sources.jl:34-42;
+ * _stokes_output(::Pressure), src/operators/harmonics_stokes.jl:348-351) */
+#define FFMM_PRESSURE 3
 
 typedef struct ffmm_context_s *ffmm_context;
 typedef struct ffmm_opcache_s *ffmm_opcache;

@@ -39,7 +39,7 @@ def direct(kind: str, sources, dens, targets, output: int = 0, kappa: float = 0.
     d = np.ascontiguousarray(dens, dtype=np.float64)
     nd = 1 if d.ndim == 1 else d.shape[1]
     k = KIND[kind]
-    nout = 3 if k == 2 else (1 if output == 0 else 3 if output == 1 else 4)
+    nout = (1 if output == 3 else 3) if k == 2 else (1 if output == 0 else 3 if output == 1 else 4)
     out = np.empty((len(t), nout))
     sk = None
     if skip is not None:
@@ -167,7 +167,7 @@ class HarmonicFMM:
         t = self.tree
         d = np.ascontiguousarray(dens_sorted, dtype=np.float64)
         nd = 1 if d.ndim == 1 else d.shape[1]
-        nout = 3 if self.k == 2 else (1 if output == 0 else 3 if output == 1 else 4)
+        nout = (1 if output == 3 else 3) if self.k == 2 else (1 if output == 0 else 3 if output == 1 else 4)
         out = np.zeros((len(x), nout))
         O.lib().oh_l2t(self.k, self.p, *self.tabs, x, tnode, len(x), np.ascontiguousarray(Pl), self.centers, t.level,
                        self.scales, self.khat, int(output), out)

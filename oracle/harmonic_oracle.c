@@ -309,7 +309,7 @@ void oh_l2t(int kind, int p, const double *ta, const double *tbb, const double *
     const int N = (p + 1) * (p + 1);
     const int nb = kind == 2 ? 4 : 1;
     const int nt2 = tidx(p + 1, p + 1) + 1;
-    const int nout = kind == 2 ? 3 : (output == 0 ? 1 : output == 1 ? 3 : 4);
+    const int nout = kind == 2 ? (output == 3 ? 1 : 3) : (output == 0 ? 1 : output == 1 ? 3 : 4);
     #pragma omp parallel
     {
         double *R = malloc(sizeof(double) * nt2), *I = malloc(sizeof(double) * nt2);
@@ -323,6 +323,18 @@ void oh_l2t(int kind, int p, const double *ta, const double *tbb, const double *
             const double *Li = L + (int64_t)i * nb * N;
             double *o = out + q * nout;
             fold(&tb, x, y, z, p, R, I);
+            if (kind == 2 && output == 3) {
+                /* the pressure -s tr(grad Phi), each block's gradient by the Laplace
+                 * gradient fold (_stokes_output(::Pressure), harmonics_stokes.jl:348-351) */
+                double tr = 0;
+                for (int k = 0; k < 3; k++) {
+                    double g[3] = {0, 0, 0};
+                    h_gradient(&tb, Li + k * N, R, I, p, NULL, g);
+                    tr += g[k];
+                }
+                o[0] = -s * tr;
+                continue;
+            }
             if (kind == 2) {
                 double xh[3] = {x, y, z};
                 stokes_velocity(&tb, Li, N, R, I, p, xh, o);
@@ -373,13 +385,28 @@ static void dipole_field(const double *d, double dx, double dy, double dz, doubl
     else { acc[0] += pv; acc[1] += g0; acc[2] += g1; acc[3] += g2; }
 }
 
+/* the Stokes pressure at r = target - source of a stokeslet f, (f.r) / (4 pi r^3)
+ * (pressure_kernel, src/kernels.jl:267-271), and with nd = 9 of a stresslet (g, nu),
+ * ((g.nu) / r^3 - 3 (g.r)(nu.r) / r^5) / (2 pi) (src/sources.jl:237-242) */
+static double stokes_pressure(const double *f, int nd, double dx, double dy, double dz, double ir) {
+    const double ir3 = ir * ir * ir;
+    double pr = (dx * f[0] + dy * f[1] + dz * f[2]) * ir3 / (4.0 * PI);
+    if (nd == 9) {
+        const double *g = f + 3, *nu = f + 6;
+        const double gn = g[0] * nu[0] + g[1] * nu[1] + g[2] * nu[2];
+        const double gr = g[0] * dx + g[1] * dy + g[2] * dz, nr = nu[0] * dx + nu[1] * dy + nu[2] * dz;
+        pr += (gn * ir3 - 3.0 * gr * nr * ir3 * ir * ir) / (2.0 * PI);
+    }
+    return pr;
+}
+
 /* near field over the 27 colleagues, added into out; tskip: sorted source index to
  * exclude (-1 none) */
 void oh_near(int kind, double kappa, const double *tx, const int32_t *tnode, const int32_t *tskip, int64_t nt,
              const int32_t *colleagues, const int32_t *plo, const int32_t *phi, const double *pts,
              const double *dens, int nd, int output, double *out) {
     const double c4 = 1.0 / (4.0 * PI), c8 = 1.0 / (8.0 * PI);
-    const int nout = kind == 2 ? 3 : (output == 0 ? 1 : output == 1 ? 3 : 4);
+    const int nout = kind == 2 ? (output == 3 ? 1 : 3) : (output == 0 ? 1 : output == 1 ? 3 : 4);
     #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t q = 0; q < nt; q++) {
         int i = tnode[q];
@@ -393,6 +420,10 @@ void oh_near(int kind, double kappa, const double *tx, const int32_t *tnode, con
                 double dx = x - pts[3 * j], dy = y - pts[3 * j + 1], dz = z - pts[3 * j + 2];
                 double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
                 double ir = 1.0 / sqrt(r2);
+                if (kind == 2 && output == 3) {
+                    acc[0] += stokes_pressure(dens + (int64_t)nd * j, nd, dx, dy, dz, ir);
+                    continue;
+                }
                 if (kind == 2) {
                     const double *f = dens + (int64_t)nd * j;
                     double a = c8 * ir;
@@ -436,7 +467,7 @@ void oh_near(int kind, double kappa, const double *tx, const int32_t *tnode, con
 void oh_direct(int kind, double kappa, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
                int64_t nt, const int64_t *skip, int output, double *out) {
     const double c4 = 1.0 / (4.0 * PI), c8 = 1.0 / (8.0 * PI);
-    const int nout = kind == 2 ? 3 : (output == 0 ? 1 : output == 1 ? 3 : 4);
+    const int nout = kind == 2 ? (output == 3 ? 1 : 3) : (output == 0 ? 1 : output == 1 ? 3 : 4);
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t q = 0; q < nt; q++) {
         double x = tgt[3 * q], y = tgt[3 * q + 1], z = tgt[3 * q + 2];
@@ -447,6 +478,10 @@ void oh_direct(int kind, double kappa, const double *src, const double *dens, in
             double dx = x - src[3 * j], dy = y - src[3 * j + 1], dz = z - src[3 * j + 2];
             double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             double ir = 1.0 / sqrt(r2);
+            if (kind == 2 && output == 3) {
+                acc[0] += stokes_pressure(dens + (int64_t)nd * j, nd, dx, dy, dz, ir);
+                continue;
+            }
             if (kind == 2) {
                 const double *f = dens + (int64_t)nd * j;
                 double a = c8 * ir;

@@ -169,6 +169,41 @@ def test_laplace_dipoles_vs_oracle_and_direct():
         py.matvec(dens, ffmm.Potential)
 
 
+@pytest.mark.parametrize("nd", [3, 9])
+def test_stokes_pressure_vs_oracle_and_direct(nd):
+    """The Stokes pressure (Pressure(), src/sources.jl:34-42): far field -s tr(grad Phi)
+    (src/operators/harmonics_stokes.jl:348-351), near field the stokeslet and stresslet
+    pressure kernels, at the sources and at targets, against the harmonic oracle (FP64
+    both) and the FP64 direct sum; the velocity of the same plan is unchanged."""
+    import ffmm
+    rng = np.random.default_rng(15)
+    n = 4000
+    pts = mixed(rng, n)
+    dens = rng.standard_normal((n, nd))
+    if nd == 9:
+        dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+    tgt = uniform(rng, 300)
+    m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    print(f"[harmonic] stokes pressure nd={nd} n={n} p=8", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
+    u = plan.matvec(dens, ffmm.Pressure).cpu().numpy()
+    v = plan.evaluate(tgt, ffmm.Pressure).cpu().numpy()
+    assert u.shape == (n,) and v.shape == (300,)
+    f, _, _ = oracle_for(m, cache, pts, 40, plan.download_tree().perm)
+    uo, vo = f.matvec(dens, output=3, targets=tgt)
+    assert rel(u, uo) < 1e-11 and rel(v, vo) < 1e-11
+    assert rel(u, HO.direct("stokes", pts, dens, pts, 3, skip=np.arange(n))) < 1e-4
+    assert rel(v, HO.direct("stokes", pts, dens, tgt, 3)) < 1e-4
+    w = plan.evaluate_at_sources(ffmm.Velocity).cpu().numpy()
+    assert rel(w, f.matvec(dens)) < 1e-11
+    ml = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=4)
+    pl = ffmm.DevicePlan(ml, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(ml))
+    pl.build_expansions(dens[:, 0].copy())
+    with pytest.raises(ffmm.ArgumentError, match="Stokes"):
+        pl.evaluate_at_sources(ffmm.Pressure)
+
+
 def test_harmonic_sorted_densities_linearity_and_errors():
     import ffmm
     rng = np.random.default_rng(13)

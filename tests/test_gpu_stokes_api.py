@@ -6,6 +6,7 @@ device path."""
 import numpy as np
 import pytest
 
+import harmonic as HO
 from pointsets import mixed, uniform
 
 pytestmark = pytest.mark.gpu
@@ -42,6 +43,13 @@ def test_stokes_front_end():
     vs, vt2 = ffmm.stokes_fmm_eval_src_trg_(None, None, pt, stokeslets=f)
     assert np.array_equal(vt2, vt)
     assert rel(vs[rows], ffmm.direct_self(K, pts, f, rows=rows)) < 1e-3
+    # the pressure (output=Pressure, src/stokes_api.jl:100-104), against the device plan's
+    # and the oracle's direct sum
+    pr = ffmm.stokes_fmm_eval_(None, plan, stokeslets=f, stresslets=g, stressvecs=nu, weights=w,
+                               output=ffmm.Pressure)
+    assert pr.shape == (n,)
+    assert np.array_equal(pr, plan.plan.matvec(dens, ffmm.Pressure).cpu().numpy())
+    assert rel(pr, HO.direct("stokes", pts, dens, pts, 3, skip=np.arange(n))) < 1e-4
     # one-shot kernel, D x N input
     u = ffmm.StokesFMMKernel(pts.T.copy(), stokeslets=f, p=10, ppl=40)
     assert rel(u[rows], ffmm.direct_self(K, pts, f, rows=rows)) < 1e-3

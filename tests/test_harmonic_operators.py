@@ -327,3 +327,40 @@ def test_oracle_fmm_with_dipoles_against_direct_sums(lap):
     vd = HO.direct("laplace", pts, dens, tgt, 2)
     assert rel(u[:, 0], ud[:, 0]) < 3e-5 and rel(v[:, 0], vd[:, 0]) < 3e-5
     assert rel(u[:, 1:], ud[:, 1:]) < 3e-5 and rel(v[:, 1:], vd[:, 1:]) < 3e-5
+
+
+def test_oracle_stokes_pressure_is_the_laplace_dipole_field():
+    """The Stokes pressure kernels against the (pinned) Laplace dipole field: a stokeslet
+    f's pressure (f.r) / (4 pi r^3) (src/kernels.jl:267-271) is the potential of the
+    dipole f, and a stresslet (g, nu)'s, ((g.nu) / r^3 - 3 (g.r)(nu.r) / r^5) / (2 pi)
+    (src/sources.jl:237-242), is twice nu . the gradient of the dipole g."""
+    rng = np.random.default_rng(23)
+    src = rng.random((30, 3))
+    tgt = rng.random((25, 3)) + 1.5
+    f = rng.standard_normal((30, 3))
+    got = HO.direct("stokes", src, f, tgt, 3)
+    want = HO.direct("laplace", src, np.hstack([np.zeros((30, 1)), f]), tgt, 0)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-15)
+    for j in range(5):
+        g, nu = rng.standard_normal(3), rng.standard_normal(3)
+        dens = np.concatenate([np.zeros(3), g, nu])[None, :]
+        got = HO.direct("stokes", src[j:j + 1], dens, tgt, 3)
+        grad = HO.direct("laplace", src[j:j + 1], np.concatenate([[0.0], g])[None, :], tgt, 1)
+        assert np.allclose(got, 2 * grad @ nu, rtol=1e-12, atol=1e-14)
+
+
+def test_oracle_stokes_pressure_fmm_against_direct_sums(lap):
+    """The pressure of stokeslets + stresslets through the harmonic oracle (far field
+    -s tr(grad Phi), _stokes_output(::Pressure), src/operators/harmonics_stokes.jl:348-351)
+    against the FP64 direct sum, at sources and targets."""
+    rng = np.random.default_rng(24)
+    n = 2500
+    pts = mixed(rng, n)
+    tgt = uniform(rng, 150)
+    f = HO.HarmonicFMM("stokes", lambda l, w: (lap, 1.0 / w), pts, 40, np.zeros(3), 1.0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)   # noqa: E731
+    dens = rng.standard_normal((n, 9))
+    dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+    u, v = f.matvec(dens, output=3, targets=tgt)
+    assert rel(u, HO.direct("stokes", pts, dens, pts, 3, skip=np.arange(n))) < 5e-6
+    assert rel(v, HO.direct("stokes", pts, dens, tgt, 3)) < 5e-6
