@@ -134,7 +134,7 @@ struct ffmm_server_s {
 namespace ffmm {
 
 static int ncomp_of(const OpCache &o, int output) {
-  if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET) return 3;
+  if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET) return output == FFMM_PRESSURE ? 1 : 3;
   return output == FFMM_POTENTIAL ? 1 : output == FFMM_GRADIENT ? 3 : 4;
 }
 
@@ -349,7 +349,9 @@ int ffmm_server_submit(ffmm_server S, const ffmm_request *req, int64_t *ticket) 
     if (req->n <= 0 || !req->points || !req->dens) fail(FFMM_ERR_ARGUMENT, "a request needs points and densities");
     if (req->nt > 0 && (!req->targets || !req->out_targets)) fail(FFMM_ERR_ARGUMENT, "null targets or target output");
     if (!req->out_sources && req->nt <= 0) fail(FFMM_ERR_ARGUMENT, "a request needs an output");
-    if (req->output < 0 || req->output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (req->output < 0 || req->output > 3) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (req->output == FFMM_PRESSURE && !(c->method == METHOD_HARMONIC && c->kernel == FFMM_KERNEL_STOKESLET))
+      fail(FFMM_ERR_ARGUMENT, "Pressure is an output of the Stokes kernels");
     auto q = std::make_unique<Request>();
     q->r = *req;
     q->cache = c;

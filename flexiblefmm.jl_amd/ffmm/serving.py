@@ -74,7 +74,7 @@ class Server:
             raise DimensionMismatch(f"expected {n} densities")
         nd = 1 if d.ndim == 1 else d.shape[1]
         o = _output(output)
-        nc = 3 if isinstance(m, StokesHarmonics) else o.ncomp
+        nc = (1 if o.code == _lib.PRESSURE else 3) if isinstance(m, StokesHarmonics) else o.ncomp
         # per coordinate: numpy's axis-0 reduction of an N x 3 array is several times slower
         lo = np.array([pts[:, d].min() for d in range(3)])
         hi = np.array([pts[:, d].max() for d in range(3)])

@@ -101,3 +101,30 @@ def test_request_errors_are_reported_per_request():
             srv.wait(bad)
         u, _, _ = srv.wait(good)
         assert u.shape == (2000, 3) and np.all(np.isfinite(u))
+
+
+def test_served_stokes_pressure_matches_plan():
+    """A served Stokes request for the pressure (one value per point) against the same
+    solve through DevicePlan; a pressure request on a Laplace cache is refused."""
+    import ffmm
+    from ffmm.serving import Server
+    rng = np.random.default_rng(23)
+    sto = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=6)
+    cache = ffmm.OperatorCache(sto)
+    pts = uniform(rng, 3000)
+    dens = rng.standard_normal((3000, 9))
+    dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
+    tgt = uniform(rng, 200)
+    lap = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=4)
+    lcache = ffmm.OperatorCache(lap)
+    with Server(nslots=2, hbm_budget=4 << 30) as srv:
+        t = srv.submit(cache, pts, dens, tgt, bbox=BOX, output=ffmm.Pressure)
+        u, v, _ = srv.wait(t)
+        with pytest.raises(ffmm.ArgumentError, match="Stokes"):
+            srv.submit(lcache, pts, dens[:, 0].copy(), bbox=BOX, output=ffmm.Pressure)
+    plan = ffmm.DevicePlan(sto, pts, ppl=64, bbox=BOX, cache=cache)
+    pu = plan.matvec(dens, ffmm.Pressure).cpu().numpy()
+    pv = plan.evaluate(tgt, ffmm.Pressure).cpu().numpy()
+    assert u.shape == (3000,) and v.shape == (200,)
+    assert np.allclose(u, pu, rtol=1e-12, atol=1e-12 * np.max(np.abs(pu)))
+    assert np.allclose(v, pv, rtol=1e-12, atol=1e-12 * np.max(np.abs(pv)))
