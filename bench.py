@@ -280,11 +280,17 @@ def stokes_bench(n: int, ppl: int, nerr: int):
         ms = 1e3 * (time.perf_counter() - t)
         if ud is None:
             ud = ffmm.direct_self(ffmm.Stokeslet3D(), pts, dens, rows=rows)
+        stages = plan.stage_times()   # the solve's, before the pressure evaluation below
+        # the pressure at the sources from the same expansions (Pressure(), one value a point)
+        t = time.perf_counter()
+        plan.evaluate_at_sources(ffmm.Pressure)
+        torch.cuda.synchronize()
+        pressure_ms = 1e3 * (time.perf_counter() - t)
         u = u_d.cpu().numpy()[rows]
         err = float(np.linalg.norm(u - ud) / np.linalg.norm(ud))
         res[prec] = {"ms_per_solve": ms, "points_per_s": n / (ms * 1e-3), "rel_l2_velocity": err,
-                     "stages_ms": plan.stage_times(), "levels": plan.nlevels, "nodes": plan.nnodes,
-                     "far_pairs": int(plan.level_pairs.sum())}
+                     "stages_ms": stages, "levels": plan.nlevels, "nodes": plan.nnodes,
+                     "far_pairs": int(plan.level_pairs.sum()), "pressure_eval_ms": pressure_ms}
         log(f"[bench] stokes {prec}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} points/s, rel L2 {err:.2e}")
         del plan
     res["mixed_over_f64"] = res["f64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
