@@ -316,3 +316,37 @@ def test_class_m2l_chunks(case, monkeypatch):
     monkeypatch.setenv("FFMM_M2L_CHUNK_PAIRS", "100")
     many = ffmm.DevicePlan(m, pts, cache=cache, **kw).matvec(dens, out).cpu().numpy()
     assert rel(many, one) < (1e-6 if case == "cube-mixed" else 1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 2, 37])
+def test_dipoles_and_pressure_small_cases(n):
+    """Laplace dipoles and the Stokes pressure with 1, 2 and 37 points (one leaf: every
+    interaction is in the near field) against the FP64 direct sums, at targets and at
+    the sources (a point skips itself)."""
+    import ffmm
+    rng = np.random.default_rng(16 + n)
+    pts = uniform(rng, n)
+    tgt = uniform(rng, 20)
+    print(f"[harmonic] small dipole / pressure n={n}", flush=True)
+    m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=6)
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(m))
+    dens = rng.standard_normal((n, 4))
+    u = plan.matvec(dens, ffmm.PotentialGradient).cpu().numpy().reshape(n, 4)
+    v = plan.evaluate(tgt, ffmm.PotentialGradient).cpu().numpy()
+    vd = HO.direct("laplace", pts, dens, tgt, 2)
+    assert rel(v, vd) < 1e-12
+    if n > 1:
+        assert rel(u, HO.direct("laplace", pts, dens, pts, 2, skip=np.arange(n))) < 1e-12
+    else:
+        assert np.all(u == 0.0)
+    ms = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=4)
+    ps = ffmm.DevicePlan(ms, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(ms))
+    ds = rng.standard_normal((n, 9))
+    ds[:, 6:] /= np.linalg.norm(ds[:, 6:], axis=1)[:, None]
+    pu = ps.matvec(ds, ffmm.Pressure).cpu().numpy().reshape(n)
+    pv = ps.evaluate(tgt, ffmm.Pressure).cpu().numpy()
+    assert rel(pv, HO.direct("stokes", pts, ds, tgt, 3)) < 1e-12
+    if n > 1:
+        assert rel(pu, HO.direct("stokes", pts, ds, pts, 3, skip=np.arange(n))) < 1e-12
+    else:
+        assert np.all(pu == 0.0)
