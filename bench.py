@@ -80,66 +80,50 @@ def cpu_baseline(n_sample: int, ppl: int):
                       f"potential at the sources, FP64, {dt:.1f} s"}
 
 
-def serving_stream(nreq: int, nmin: int, nmax: int, rate: float, seed: int = 5):
-    """The served mixed-size stream: Poisson arrivals at ``rate`` per second, point
-    counts log-uniform in [nmin, nmax], 70% Laplace3D cube p = 8 / 30% Yukawa3D(5)
-    harmonics, uniform or Gaussian-mixture layouts, charges N(0, 1), 40% repeats of an
-    earlier geometry with new charges, half in the 1e-6 class (cube p = 8 mixed
-    spectral, Yukawa p = 10) and half in the 1e-3 class (cube p = 4, Yukawa p = 6)."""
-    from ffmm import workloads as W
-    rng = np.random.default_rng(seed)
-    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    reqs, geoms = [], []
-    for k in range(nreq):
-        lap = rng.random() < 0.7
-        fine = rng.random() < 0.5
-        if geoms and rng.random() < 0.4:
-            g = geoms[rng.integers(len(geoms))]
-            lap, fine, pts, gid = g["lap"], g["fine"], g["pts"], g["gid"]
-        else:
-            n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
-            pts = W.gaussian_mixture(n, seed=int(rng.integers(1 << 30))) if rng.random() < 0.5 else \
-                W.uniform(n, seed=int(rng.integers(1 << 30)))
-            gid = len(geoms)
-            geoms.append(dict(lap=lap, fine=fine, pts=pts, gid=gid))
-        reqs.append(dict(t=float(t[k]), lap=lap, fine=fine, pts=pts, gid=gid,
-                         tau=rng.standard_normal(len(pts))))
-    return reqs
+def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, plain_every: int):
+    """The served stream (ffmm.workloads.served_stream): Poisson arrivals, point counts
+    log-uniform in [nmin, nmax], 70% Laplace3D cube p = 8 (mixed, spectral M2L) / 30%
+    Yukawa3D(5) harmonics p = 10 (mixed, dense class M2L), uniform or Gaussian-mixture
+    layouts, 40% repeated geometries with new charges, the 1e-6 and 1e-3 classes.
 
-
-def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
-    """Serve the stream through the server (arrivals replayed in real time by a
-    submitting thread) and solve the same requests one after another with plain plans
-    (a new DevicePlan per request, no reuse); report p50/p99 latency and throughput."""
+    Two passes through a fresh server: arrivals replayed in real time at ``rate`` (the
+    latencies at a rate the server sustains) and all requests at once (the saturated
+    throughput).  Against them, the same requests solved one after another with a new
+    plan each (no resident plans, no overlap): in the served forms (every request), and
+    in the plain FP64 forms with the same M2L algorithms (every ``plain_every``-th
+    request)."""
     import threading
     import ffmm
+    from ffmm import workloads as W
     from ffmm.serving import Server
+    torch = __import__("torch")
     box = (np.zeros(3), 1.0)
-    methods = {(True, True): ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8),
-               (True, False): ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=4),
-               (False, True): ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10),
-               (False, False): ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=6)}
+    methods = {True: ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8),
+               False: ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10)}
     caches = {k: ffmm.OperatorCache(m) for k, m in methods.items()}
-    # the served forms: mixed precision for both accuracy classes (FP32 arithmetic is well
-    # inside the 1e-3 class); the cube method's batched-FFT M2L is built for p = 8
-    form = {(True, True): ("mixed", "spectral"), (True, False): ("mixed", "dense"),
-            (False, True): ("mixed", "dense"), (False, False): ("mixed", "dense")}
-    reqs = serving_stream(nreq, nmin, nmax, rate)
-    total = float(sum(len(r["pts"]) for r in reqs))
-    log(f"[bench] serving: {nreq} requests, {total:.3e} points, {nslots} slots")
+    form = {True: ("mixed", "spectral"), False: ("mixed", "dense")}
+    plain_form = {True: ("f64", "spectral"), False: ("f64", "dense")}
+    reqs = W.served_stream(nreq, nmin, nmax, rate, seed=5)
+    t = time.perf_counter()
+    geo, data = {}, []
+    for r in reqs:
+        if r["gid"] not in geo:
+            geo[r["gid"]] = W.request_points(r)
+        data.append((geo[r["gid"]], W.request_charges(r)))
+    total = float(sum(r["n"] for r in reqs))
+    log(f"[bench] serving: {nreq} requests, {total:.3e} points, {len(geo)} geometries, {nslots} slots "
+        f"(generated in {time.perf_counter() - t:.1f} s)")
 
     def serve(time_scale: float):
         """The stream through a fresh server, arrivals at the stream's times x time_scale
-        replayed by a submitting thread; per request the latency, the submission lag and
-        the submit call's time, and the first served result of each class."""
-        srv = Server(nslots=nslots, hbm_budget=96 << 30)
-        # warm the operator caches (host precompute and device copies, Yukawa widths of 14 levels)
-        for k, c in caches.items():
+        (0: all at once) replayed by a submitting thread; per request the latency and
+        the submission lag, and the first served result of each (kernel, class)."""
+        srv = Server(nslots=nslots, hbm_budget=200 << 30)
+        for k, c in caches.items():      # warm the operator caches (host precompute, device copies)
             c.fit_box(np.zeros(3), np.ones(3), box)
             srv._prepare(c, np.zeros(3), np.ones(3), box)
         tickets = [None] * nreq
         t_sub = [0.0] * nreq
-        d_sub = [0.0] * nreq
 
         def submitter():
             t0 = time.perf_counter()
@@ -147,76 +131,67 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
                 dt = time_scale * r["t"] - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
-                key = (r["lap"], r["fine"])
-                prec, m2l = form[key]
                 t_sub[i] = time.perf_counter()
-                tickets[i] = srv.submit(caches[key], r["pts"], r["tau"], geometry=r["gid"], ppl=64, bbox=box,
-                                        precision=prec, m2l=m2l)
-                d_sub[i] = time.perf_counter() - t_sub[i]
+                prec, m2l = form[r["lap"]]
+                tickets[i] = srv.submit(caches[r["lap"]], data[i][0], data[i][1], geometry=r["gid"], ppl=64,
+                                        bbox=box, precision=prec, m2l=m2l)
 
         t0 = time.perf_counter()
         th = threading.Thread(target=submitter)
         th.start()
-        lat = []
-        first = {}   # class -> (request index, served potential at the sources)
+        lat, first = [], {}
         for i in range(nreq):
             while tickets[i] is None:
-                time.sleep(0.0005)
+                time.sleep(0.0002)
             u, _, ms = srv.wait(tickets[i])
             lat.append(ms)
             key = (reqs[i]["lap"], reqs[i]["fine"])
             if key not in first:
                 first[key] = (i, u)
-        t_served = time.perf_counter() - t0
+        wall = time.perf_counter() - t0
         th.join()
-        # how far the submissions ran behind the arrival schedule (the submitting thread)
         lag = np.array([t_sub[i] - t0 - time_scale * reqs[i]["t"] for i in range(nreq)])
         st = srv.stats()
         srv.close()
-        return t_served, np.array(lat), lag, np.array(d_sub), first, st
+        lat = np.array(lat)
+        return {"wall_s": wall, "served_points_per_s": total / wall, "p50_latency_ms": float(np.percentile(lat, 50)),
+                "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
+                "submit_lag_ms": {"mean": 1e3 * float(lag.mean()), "max": 1e3 * float(lag.max())},
+                "plans_built": st["plans_built"], "plans_reused": st["plans_reused"],
+                "evictions": st["evictions"], "peak_device_bytes": st["peak_bytes"]}, first
 
-    # a lighter load first (a fifth of the rate: the latency of requests that do not queue),
-    # then the stream at its rate (the throughput)
-    light_t, light_lat, light_lag, _, _, _ = serve(5.0)
-    light = {"arrival_rate_per_s": rate / 5.0, "p50_latency_ms": float(np.percentile(light_lat, 50)),
-             "p99_latency_ms": float(np.percentile(light_lat, 99)), "served_points_per_s": total / light_t,
-             "submit_lag_ms": {"mean": 1e3 * float(light_lag.mean()), "max": 1e3 * float(light_lag.max())}}
-    t_served, lat, lag, d_sub, first, st = serve(1.0)
-    # the same requests one after another, plain plans, no reuse and no arrival gaps
-    torch = __import__("torch")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for r in reqs:
-        key = (r["lap"], r["fine"])
-        prec, m2l = form[key]
-        plan = ffmm.DevicePlan(methods[key], r["pts"], ppl=64, bbox=box, cache=caches[key], precision=prec, m2l=m2l)
-        plan.matvec(r["tau"]).cpu()
-        del plan
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t1
-    # and in the plain form (every request FP64 with dense class M2L, the reference
-    # device plan's form), one after another
-    t1 = time.perf_counter()
-    for r in reqs:
-        key = (r["lap"], r["fine"])
-        plan = ffmm.DevicePlan(methods[key], r["pts"], ppl=64, bbox=box, cache=caches[key], precision="f64",
-                               m2l="dense")
-        plan.matvec(r["tau"]).cpu()
-        del plan
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t1
-    # the accuracy of the served results: the first request of each class against the
-    # FP64 direct sum at 300 sampled sources (Laplace on the device; Yukawa in numpy)
+    sustained, first = serve(1.0)
+    log(f"[bench] serving at {rate}/s: " + json.dumps(sustained))
+    saturated, _ = serve(0.0)
+    log("[bench] serving, all at once: " + json.dumps(saturated))
+
+    def sequential(fm, every):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pts_done = 0
+        for i in range(0, nreq, every):
+            r = reqs[i]
+            prec, m2l = fm[r["lap"]]
+            plan = ffmm.DevicePlan(methods[r["lap"]], data[i][0], ppl=64, bbox=box, cache=caches[r["lap"]],
+                                   precision=prec, m2l=m2l)
+            plan.matvec(data[i][1]).cpu()
+            del plan
+            pts_done += r["n"]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        return {"wall_s": dt, "points": pts_done, "points_per_s": pts_done / dt,
+                "requests": len(range(0, nreq, every))}
+
+    seq = sequential(form, 1)
+    log("[bench] sequential, served forms: " + json.dumps(seq))
+    seq_plain = sequential(plain_form, plain_every)
+    log("[bench] sequential, plain FP64 forms: " + json.dumps(seq_plain))
+    # the accuracy of the served results: the first request of each (kernel, class)
+    # against the FP64 direct sum at 300 sampled sources (Laplace on the device; Yukawa in numpy)
     acc = {}
-    names = {(True, True): "laplace_cube_p8", (True, False): "laplace_cube_p4",
-             (False, True): "yukawa_harmonic_p10", (False, False): "yukawa_harmonic_p6"}
-    # the classes' gates: 1e-6 and 1e-3; Yukawa at p = 10 is held to the reference's own
-    # end-to-end Yukawa tolerance (1e-5, test/test_harmonics_yukawa.jl:107-109), its
-    # truncation error at k = 5 being a few 1e-6
     gates = {(True, True): 1e-6, (True, False): 1e-3, (False, True): 1e-5, (False, False): 1e-3}
     for key, (i, u) in sorted(first.items()):
-        r = reqs[i]
-        pts, tau = r["pts"], r["tau"]
+        pts, tau = data[i]
         rows = np.random.default_rng(5).choice(len(pts), min(300, len(pts)), replace=False)
         if key[0]:
             ref = ffmm.direct(ffmm.Laplace3D(), pts, tau, pts[rows], skip=rows)
@@ -227,22 +202,20 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int):
                 d[j] = np.inf
                 ref[a] = np.sum(tau * np.exp(-5.0 * d) / (4 * np.pi * d))
         err = float(np.linalg.norm(u[rows] - ref) / np.linalg.norm(ref))
-        acc[names[key]] = {"rel_l2": err, "gate": gates[key], "ok": err <= gates[key], "points": len(pts),
-                           "form": "/".join(form[key])}
-    return {"requests": nreq, "points": total, "slots": nslots, "arrival_rate_per_s": rate, "accuracy": acc,
-            "light_load": light, "arrival_span_ms": 1e3 * reqs[-1]["t"], "submit_lag_ms": {"mean": 1e3 * float(lag.mean()),
-                                                                       "max": 1e3 * float(lag.max())},
-            "submit_call_ms": {"mean": 1e3 * float(np.mean(d_sub)), "max": 1e3 * float(np.max(d_sub))},
-            "n_range": [nmin, nmax], "p50_latency_ms": float(np.percentile(lat, 50)),
-            "p99_latency_ms": float(np.percentile(lat, 99)), "served_points_per_s": total / t_served,
-            "served_wall_s": t_served, "sequential_points_per_s": total / t_seq, "sequential_wall_s": t_seq,
-            "speedup_vs_sequential": t_seq / t_served, "sequential_plain_points_per_s": total / t_plain,
-            "sequential_plain_wall_s": t_plain, "speedup_vs_sequential_plain": t_plain / t_served,
-            "plans_built": st["plans_built"],
-            "plans_reused": st["plans_reused"], "peak_device_bytes": st["peak_bytes"],
-            "note": "arrivals replayed in real time (the served wall time includes the arrival gaps); "
-                    "the sequential legs solve back to back with a new plan per request, in the served "
-                    "requests' forms and in the plain form (FP64, dense class M2L)"}
+        name = ("laplace_cube_p8" if key[0] else "yukawa_harmonic_p10") + ("_1e-6" if key[1] else "_1e-3")
+        acc[name] = {"rel_l2": err, "gate": gates[key], "ok": err <= gates[key], "points": len(pts),
+                     "form": "/".join(form[key[0]])}
+    return {"requests": nreq, "points": total, "geometries": len(geo), "slots": nslots, "n_range": [nmin, nmax],
+            "arrival_rate_per_s": rate, "arrival_span_s": reqs[-1]["t"], "sustained": sustained,
+            "saturated": saturated, "p50_latency_ms": sustained["p50_latency_ms"],
+            "p99_latency_ms": sustained["p99_latency_ms"], "served_points_per_s": saturated["served_points_per_s"],
+            "sequential_same_forms": seq, "sequential_plain_forms": seq_plain,
+            "speedup_vs_sequential_same_forms": saturated["served_points_per_s"] / seq["points_per_s"],
+            "speedup_vs_sequential_plain": saturated["served_points_per_s"] / seq_plain["points_per_s"],
+            "accuracy": acc,
+            "note": "served forms: cube p=8 mixed/spectral, Yukawa p=10 mixed/dense for both classes (the orders "
+                    "are fixed by the workload); plain forms: FP64 with the same M2L algorithms; sequential legs "
+                    "build a new plan per request; the plain leg runs every %d-th request" % plain_every}
 
 
 def stokes_bench(n: int, ppl: int, nerr: int):
@@ -280,18 +253,23 @@ def stokes_bench(n: int, ppl: int, nerr: int):
         ms = 1e3 * (time.perf_counter() - t)
         if ud is None:
             ud = ffmm.direct_self(ffmm.Stokeslet3D(), pts, dens, rows=rows)
+            pd = ffmm.direct_self(ffmm.Stokeslet3D(), pts, dens, ffmm.Pressure, rows=rows)
         stages = plan.stage_times()   # the solve's, before the pressure evaluation below
         # the pressure at the sources from the same expansions (Pressure(), one value a point)
         t = time.perf_counter()
-        plan.evaluate_at_sources(ffmm.Pressure)
+        pr = plan.evaluate_at_sources(ffmm.Pressure)
         torch.cuda.synchronize()
         pressure_ms = 1e3 * (time.perf_counter() - t)
         u = u_d.cpu().numpy()[rows]
         err = float(np.linalg.norm(u - ud) / np.linalg.norm(ud))
+        pr = pr.cpu().numpy()[rows]
+        perr = float(np.linalg.norm(pr - pd) / np.linalg.norm(pd))
         res[prec] = {"ms_per_solve": ms, "points_per_s": n / (ms * 1e-3), "rel_l2_velocity": err,
                      "stages_ms": stages, "levels": plan.nlevels, "nodes": plan.nnodes,
-                     "far_pairs": int(plan.level_pairs.sum()), "pressure_eval_ms": pressure_ms}
-        log(f"[bench] stokes {prec}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} points/s, rel L2 {err:.2e}")
+                     "far_pairs": int(plan.level_pairs.sum()), "pressure_eval_ms": pressure_ms,
+                     "rel_l2_pressure": perr}
+        log(f"[bench] stokes {prec}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} points/s, rel L2 {err:.2e} "
+            f"(pressure {perr:.2e})")
         del plan
     res["mixed_over_f64"] = res["f64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
     return res
@@ -309,13 +287,15 @@ def main():
     ap.add_argument("--m2l", default="spectral", choices=["dense", "spectral"])
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 dense step")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 steps")
     ap.add_argument("--nerr", type=int, default=10_000)
-    ap.add_argument("--serve-requests", type=int, default=80, help="served-stream requests (0: skip the leg)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="served-stream requests (0: skip the leg)")
     ap.add_argument("--serve-nmin", type=int, default=10_000)
-    ap.add_argument("--serve-nmax", type=int, default=200_000)
-    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-nmax", type=int, default=2_000_000)
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-plain-every", type=int, default=8,
+                    help="the plain-form sequential leg runs every k-th request")
     ap.add_argument("--stokes-n", type=int, default=2_000_000,
                     help="points of the Stokes surface leg (0: skip the leg)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -392,36 +372,38 @@ def main():
     torch.cuda.synchronize()
     matvec_ms = 1e3 * (time.perf_counter() - t) / args.steps
 
-    # the plain FP64 form of the reference device plan (dense class GEMMs) on the same
-    # workload: one warm-up and one timed step, for the mixed-over-plain ratio
+    # the plain FP64 forms on the same workload, one warm-up and one timed step each: the
+    # same spectral M2L in FP64 (the like-for-like ratio of the mixed form) and the dense
+    # class GEMMs (the reference device plan's form)
     plain = None
-    if not args.no_plain and (args.precision, args.m2l) != ("f64", "dense"):
-        def plain_step():
-            p = ffmm.DevicePlan(m, xs_d, ppl=args.ppl, bbox=box, cache=cache, precision="f64", m2l="dense")
-            p.build_expansions(tau_d)
-            p.evaluate_at_sources(ffmm.Potential, out=u_d)
-            p.evaluate(tgt_d, ffmm.PotentialGradient, out=v_d)
-            return p
-        plan = None
-        p0 = plain_step()
-        del p0
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        p0 = plain_step()
-        torch.cuda.synchronize()
-        plain_ms = 1e3 * (time.perf_counter() - t)
-        del p0
+    if not args.no_plain and args.precision == "mixed":
+        plain = {}
+        for m2l in (("spectral", "dense") if args.m2l == "spectral" else ("dense",)):
+            def plain_step():
+                p = ffmm.DevicePlan(m, xs_d, ppl=args.ppl, bbox=box, cache=cache, precision="f64", m2l=m2l)
+                p.build_expansions(tau_d)
+                p.evaluate_at_sources(ffmm.Potential, out=u_d)
+                p.evaluate(tgt_d, ffmm.PotentialGradient, out=v_d)
+                return p
+            plan = None
+            p0 = plain_step()
+            del p0
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            p0 = plain_step()
+            torch.cuda.synchronize()
+            plain_ms = 1e3 * (time.perf_counter() - t)
+            del p0
+            plain[m2l] = {"form": f"f64 {m2l}", "ms_per_step": plain_ms, "speedup_of_measured_form": plain_ms / ms}
+            log(f"[bench] plain FP64 {m2l} step: {plain_ms:.1f} ms ({plain_ms / ms:.2f}x the measured form)")
         plan = step()   # the measured form's plan again, for the checks below
         torch.cuda.synchronize()
-        plain = {"form": "f64 dense (class GEMMs, the reference device plan's form)", "ms_per_step": plain_ms,
-                 "speedup_of_measured_form": plain_ms / ms}
-        log(f"[bench] plain FP64 dense step: {plain_ms:.1f} ms ({plain_ms / ms:.2f}x the measured form)")
 
-    # accuracy on sampled points against the FP64 direct sum
+    # accuracy against the FP64 direct sum: at nerr sampled targets (potential and
+    # gradient) and at nerr sampled sources (potential)
     rng = np.random.default_rng(123)
-    ns_err = args.nerr // 2
-    rows = np.sort(rng.choice(args.n, ns_err, replace=False))
-    trows = np.sort(rng.choice(args.nt, args.nerr - ns_err, replace=False))
+    rows = np.sort(rng.choice(args.n, min(args.nerr, args.n), replace=False))
+    trows = np.sort(rng.choice(args.nt, min(args.nerr, args.nt), replace=False))
     u = u_d.cpu().numpy()
     v = v_d.cpu().numpy()
     ud = ffmm.direct_self(ffmm.Laplace3D(), pts, tau, rows=rows)
@@ -472,7 +454,7 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serving_bench(args.serve_requests, args.serve_nmin, args.serve_nmax, args.serve_rate,
-                                args.serve_slots)
+                                args.serve_slots, args.serve_plain_every)
         log("[bench] serving: " + json.dumps(serving))
 
     stokes = None

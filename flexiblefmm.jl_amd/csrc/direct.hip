@@ -93,7 +93,8 @@ void direct(Context &ctx, const double *src, const double *tau, int64_t ns, cons
 
 // The Stokes sums: velocity of stokeslets f (nd = 3) and, with nd = 9, stresslets of
 // strength g and normal nu (rows [f, g, nu]):
-//   u = 1/(8 pi) (f / r + (d . f) d / r^3) - 6/(8 pi) (d . g)(d . nu) d / r^5,  d = t - s
+//   u = 1/(8 pi) (f / r + (d . f) d / r^3) - 6/(8 pi) (d . g)(d . nu) d / r^5,  d = t - s,
+// and the pressure of both
 // (direct! for Stokeslet3D, src/kernels.jl:463-487; the stresslet as in
 // src/operators/harmonics_stokes.jl:197-217).  The sources' rows stream through LDS.
 constexpr double C8PI = 0.039788735772973836;   // 1 / (8 pi)
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(DT) k_direct_stokes(const double *__restrict__
   const double x = tgt[3 * jj], y = tgt[3 * jj + 1], z = tgt[3 * jj + 2];
   const int64_t sk = skip ? skip[jj] : -1;
   const int64_t ib = (int64_t)blockIdx.y * chunk, ie = std::min<int64_t>(ns, ib + chunk);
-  double acc[3] = {0, 0, 0};
+  double acc[4] = {0, 0, 0, 0};
   for (int64_t i0 = ib; i0 < ie; i0 += DT) {
     const int64_t i = i0 + threadIdx.x;
     if (i < ie) {
@@ -133,20 +134,30 @@ __global__ void __launch_bounds__(DT) k_direct_stokes(const double *__restrict__
         u1 += e * d1;
         u2 += e * d2;
       }
+      // the pressure: (d.f) / (4 pi r^3) (pressure_kernel, src/kernels.jl:267-271); a
+      // stresslet ((g.nu) / r^3 - 3 (d.g)(d.nu) / r^5) / (2 pi) (src/sources.jl:237-242)
+      const double ir3 = ir * ir * ir;
+      double pr = 2 * C8PI * ir3 * (d0 * v[3] + d1 * v[4] + d2 * v[5]);
+      if (ND == 9) {
+        const double gn = v[6] * v[9] + v[7] * v[10] + v[8] * v[11];
+        const double dg = d0 * v[6] + d1 * v[7] + d2 * v[8], dn = d0 * v[9] + d1 * v[10] + d2 * v[11];
+        pr += 4 * C8PI * (gn * ir3 - 3 * dg * dn * ir3 * ir * ir);
+      }
       const bool keep = i0 + q != sk;
       acc[0] += keep ? u0 : 0.0;
       acc[1] += keep ? u1 : 0.0;
       acc[2] += keep ? u2 : 0.0;
+      acc[3] += keep ? pr : 0.0;
     }
     __syncthreads();
   }
   if (j >= nt) return;
   double *p = part + ((int64_t)blockIdx.y * nt + j) * 4;
-  p[0] = 0.0; p[1] = acc[0]; p[2] = acc[1]; p[3] = acc[2];
+  p[0] = acc[3]; p[1] = acc[0]; p[2] = acc[1]; p[3] = acc[2];
 }
 
 void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
-                   int64_t nt, const int64_t *skip, double *out) {
+                   int64_t nt, const int64_t *skip, double *out, int output) {
   if (nt <= 0) return;
   const unsigned gx = nblocks(nt, DT);
   int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / gx, (ns + 4095) / 4096));
@@ -159,8 +170,9 @@ void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, 
   if (nd == 3) k_direct_stokes<3><<<grid, DT, 0, ctx.stream>>>(src, dens, ns, tgt, nt, skip, chunk, part.p);
   else k_direct_stokes<9><<<grid, DT, 0, ctx.stream>>>(src, dens, ns, tgt, nt, skip, chunk, part.p);
   FFMM_LAUNCH_CHECK();
-  // the velocity is the three gradient slots of the reduction
-  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part.p, nt, (int)nchunk, FFMM_GRADIENT, out);
+  // the velocity is the three gradient slots of the reduction, the pressure the first
+  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part.p, nt, (int)nchunk,
+                                                          output == FFMM_PRESSURE ? FFMM_POTENTIAL : FFMM_GRADIENT, out);
   FFMM_LAUNCH_CHECK();
   ctx.sync();
 }

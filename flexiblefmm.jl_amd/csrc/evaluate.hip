@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(EV_TB) k_eval(EvalArgs a) {
         accumulate<OUT>(xr - v.x, yr - v.y, zr - v.z, v.w, acc);
       }
     }
-    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363), their
+    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 37-58), their
     // sources as one list through the same tiles
     const int32_t skip = self ? j : -1;
     for (int32_t q0 = 0; q0 < nsrc; q0 += EV_CK) {
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(EV_TB) k_eval32(EvalArgs a) {
       }
       fold32<OUT>(ks, kc, g, acc);
     }
-    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 351-363), their
+    // the near field of the 27 colleagues (src/evaluate.jl:12-25, 37-58), their
     // sources as one list through the same tiles
     const int32_t skip = self ? j : -1;
     for (int32_t q0 = 0; q0 < nsrc; q0 += EV_CK) {
@@ -331,7 +331,7 @@ void evaluate_at_sources(Plan &plan, double *out, int output) {
 }
 
 // ---------------------------------------------------------------------------
-// targets: locate (src/tree/locate.jl:10-22), bucket_by_node (src/tree/locate.jl:173-191)
+// targets: locate (src/tree/locate.jl:10-22), bucket_by_node (src/tree/locate.jl:49-68)
 // ---------------------------------------------------------------------------
 __global__ void k_locate(const double *__restrict__ t, int64_t nt, double ox, double oy, double oz, double W0,
                          const double *__restrict__ centers, int64_t nn, const uint8_t *__restrict__ leaf,
@@ -349,6 +349,24 @@ __global__ void k_locate(const double *__restrict__ t, int64_t nt, double ox, do
       i = child_first[i] + slot;
     }
     k = (uint32_t)node_leafslot[i];
+  }
+  key[q] = k;
+  idx[q] = (int32_t)q;
+}
+
+// targets located by the caller (the reference's device evaluate! takes the leaves from
+// the host locate, src/device.jl:127-145): the key is the leaf slot of the given node;
+// a node that is not a real leaf of the tree gets the invalid key
+__global__ void k_node_keys(const int32_t *__restrict__ nodes, int64_t nt, int64_t nn,
+                            const int32_t *__restrict__ node_leafslot, uint32_t *__restrict__ key,
+                            int32_t *__restrict__ idx) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nt) return;
+  const int32_t i = nodes[q];
+  uint32_t k = 0xFFFFFFFFu;
+  if (i >= 0 && i < nn) {
+    const int32_t s = node_leafslot[i];
+    if (s >= 0) k = (uint32_t)s;
   }
   key[q] = k;
   idx[q] = (int32_t)q;
@@ -381,7 +399,7 @@ __global__ void k_last_key(const uint32_t *__restrict__ key, int64_t *__restrict
   if (threadIdx.x == 0) host[0] = (int64_t)key[0];
 }
 
-void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output) {
+void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output, const int32_t *nodes) {
   if (nt <= 0) return;
   if (nt >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many targets for 32-bit indices");
   Context &ctx = *plan.ctx;
@@ -393,16 +411,20 @@ void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out
   skeys.resize(nt);
   idx.resize(nt);
   plan.tgt_order.resize(nt);
-  k_locate<<<nblocks(nt, TB), TB, 0, s>>>(targets, nt, plan.origin[0], plan.origin[1], plan.origin[2], plan.W0,
-                                          plan.centers.p, plan.nnodes, plan.leaf.p, plan.child_first.p,
-                                          plan.node_leafslot.p, keys.p, idx.p);
+  if (nodes)
+    k_node_keys<<<nblocks(nt, TB), TB, 0, s>>>(nodes, nt, plan.nnodes, plan.node_leafslot.p, keys.p, idx.p);
+  else
+    k_locate<<<nblocks(nt, TB), TB, 0, s>>>(targets, nt, plan.origin[0], plan.origin[1], plan.origin[2], plan.W0,
+                                            plan.centers.p, plan.nnodes, plan.leaf.p, plan.child_first.p,
+                                            plan.node_leafslot.p, keys.p, idx.p);
   FFMM_LAUNCH_CHECK();
   sort_pairs_u32(ctx, keys.p, skeys.p, idx.p, plan.tgt_order.p, nt, 32);
   // the largest key (outside the box: all ones), read from pinned memory
   k_last_key<<<1, 64, 0, s>>>(skeys.p + nt - 1, ctx.pinned_dev);
   FFMM_LAUNCH_CHECK();
   ctx.sync();
-  if ((uint32_t)ctx.pinned[0] == 0xFFFFFFFFu) fail(FFMM_ERR_ARGUMENT, "a target lies outside the tree's box");
+  if ((uint32_t)ctx.pinned[0] == 0xFFFFFFFFu)
+    fail(FFMM_ERR_ARGUMENT, nodes ? "a target's node is not a real leaf of the tree" : "a target lies outside the tree's box");
   plan.tgt_ptr.resize(nleaves + 1);
   k_bucket_ptr<<<nblocks(nleaves + 1, TB), TB, 0, s>>>(skeys.p, nt, nleaves, plan.tgt_ptr.p);
   plan.tgt_sorted.resize(3 * nt);

@@ -174,7 +174,7 @@ using namespace ffmm;
 extern "C" {
 
 const char *ffmm_last_error(void) { return g_last_error.c_str(); }
-int ffmm_abi_version(void) { return 2; }
+int ffmm_abi_version(void) { return 3; }
 
 int ffmm_context_create(int device, void *stream, ffmm_context *out) {
   return guard([&] {
@@ -226,6 +226,19 @@ int ffmm_direct_stokes(ffmm_context ctx, const double *src, const double *dens, 
     if (!tgt || !out || (ns > 0 && (!src || !dens))) fail(FFMM_ERR_ARGUMENT, "null argument");
     set_device(ctx->c);
     direct_stokes(ctx->c, src, dens, nd, ns, tgt, nt, skip, out);
+  });
+}
+
+int ffmm_direct_stokes_pressure(ffmm_context ctx, const double *src, const double *dens, int nd, int64_t ns,
+                                const double *tgt, int64_t nt, const int64_t *skip, double *out) {
+  return guard([&] {
+    if (!ctx) fail(FFMM_ERR_ARGUMENT, "null context");
+    if (ns < 0 || nt < 0) fail(FFMM_ERR_DIMENSION, "negative sizes");
+    if (nd != 3 && nd != 9) fail(FFMM_ERR_ARGUMENT, "Stokes densities are stokeslets (3) or stokeslets + stresslets (9)");
+    if (nt == 0) return;
+    if (!tgt || !out || (ns > 0 && (!src || !dens))) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(ctx->c);
+    direct_stokes(ctx->c, src, dens, nd, ns, tgt, nt, skip, out, FFMM_PRESSURE);
   });
 }
 
@@ -680,7 +693,7 @@ struct PlanReader {
   const uint8_t *buf;
   size_t size, n = 0;
   void get(void *dst, size_t b) {
-    if (n + b > size) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
+    if (b > size - n) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
     std::memcpy(dst, buf + n, b);
     n += b;
   }
@@ -690,26 +703,18 @@ struct PlanReader {
     get(&v, sizeof v);
     return v;
   }
+  // an array of `expect` elements into a host vector
   template <class T>
   void host(std::vector<T> &v, int64_t expect) {
     const int64_t c = val<int64_t>();
     if (c != expect) fail(FFMM_ERR_ARGUMENT, "plan data: an array of the wrong length");
+    if ((uint64_t)c > (size - n) / sizeof(T)) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
     v.resize(c);
     get(v.data(), c * sizeof(T));
   }
-  template <class T>
-  void dev(Context &ctx, DBuf<T> &d, int64_t expect) {
-    const int64_t c = val<int64_t>();
-    if (c != expect) fail(FFMM_ERR_ARGUMENT, "plan data: an array of the wrong length");
-    const size_t b = (size_t)c * sizeof(T);
-    if (n + b > size) fail(FFMM_ERR_ARGUMENT, "plan data truncated");
-    d.upload(reinterpret_cast<const T *>(buf + n), c, ctx.stream);
-    n += b;
-  }
 };
 
-template <class IO, class PlanT>
-static void plan_tree_arrays(IO &io, Context &ctx, PlanT &p) {
+static void plan_tree_arrays(PlanWriter &io, Context &ctx, Plan &p) {
   const int64_t nn = p.nnodes, N = p.N;
   io.dev(ctx, p.cell, 3 * nn);
   io.dev(ctx, p.level, nn);
@@ -752,44 +757,103 @@ size_t plan_save(Plan &p, void *buf, size_t cap) {
   return w.n;
 }
 
-// a plan from plan data made by plan_save with a cache of the same operators
-void plan_restore(Plan &p, Context &ctx, OpCache &o, const void *buf, size_t size) {
+// The header and tree of plan data made by plan_save, checked in full on the host:
+// every array's length, the tree's structure (check_host_tree) and the stored node lists
+// against the tree's.  No device work; nothing of the data is trusted before this.
+struct PlanHeader {
+  int method, kernel, p, nb, Ns, Nl;
+  double w_ref, kappa;
+  int64_t N;
+  int ppl, max_levels, precision, m2l, e0;
+};
+
+static void parse_plan_data(const void *buf, size_t size, PlanHeader &h, HostTree &t) {
   if (!buf) fail(FFMM_ERR_ARGUMENT, "null plan data");
   PlanReader r{static_cast<const uint8_t *>(buf), size};
   char magic[8];
   r.get(magic, sizeof magic);
   if (std::memcmp(magic, PLAN_MAGIC, sizeof magic) != 0) fail(FFMM_ERR_ARGUMENT, "not plan data");
   if (r.val<uint32_t>() != PLAN_VERSION) fail(FFMM_ERR_ARGUMENT, "plan data of another version");
-  const int method = r.val<int32_t>(), kernel = r.val<int32_t>(), pdeg = r.val<int32_t>(), nb = r.val<int32_t>();
-  const int Ns = r.val<int32_t>(), Nl = r.val<int32_t>();
-  const double w_ref = r.val<double>(), kappa = r.val<double>();
-  if (method != o.method || kernel != o.kernel || pdeg != o.p || nb != o.nb || Ns != o.Ns || Nl != o.Nl ||
-      w_ref != o.w_ref || kappa != o.kappa)
+  h.method = r.val<int32_t>(); h.kernel = r.val<int32_t>(); h.p = r.val<int32_t>(); h.nb = r.val<int32_t>();
+  h.Ns = r.val<int32_t>(); h.Nl = r.val<int32_t>();
+  h.w_ref = r.val<double>(); h.kappa = r.val<double>();
+  h.N = r.val<int64_t>();
+  h.ppl = r.val<int32_t>(); h.max_levels = r.val<int32_t>(); h.precision = r.val<int32_t>();
+  h.m2l = r.val<int32_t>();
+  for (int d = 0; d < 3; d++) t.origin[d] = r.val<double>();
+  t.W0 = r.val<double>();
+  h.e0 = r.val<int32_t>();
+  t.L = r.val<int32_t>();
+  t.nn = r.val<int64_t>();
+  t.N = h.N;
+  const int L = t.L;
+  const int64_t nn = t.nn, N = t.N;
+  if (L < 1 || L > MAX_KEY_LEVELS + 1 || nn < 1 || nn >= INT32_MAX / NCOLL || N < 1 || N >= INT32_MAX)
+    fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  if (!(t.W0 > 0) || !std::isfinite(t.W0) || !std::isfinite(t.origin[0]) || !std::isfinite(t.origin[1]) ||
+      !std::isfinite(t.origin[2]))
+    fail(FFMM_ERR_ARGUMENT, "plan data: a bad box");
+  std::vector<int64_t> s2m_off, leaf_off;
+  r.host(t.lo, L);
+  r.host(t.nl, L);
+  r.host(s2m_off, L + 1);
+  r.host(leaf_off, L + 1);
+  if (s2m_off[L] < 0 || s2m_off[L] > nn || leaf_off[L] < 0 || leaf_off[L] > nn)
+    fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  std::vector<double> centers;
+  std::vector<int32_t> s2m, leaves;
+  r.host(t.cell, 3 * nn);
+  r.host(t.level, nn);
+  r.host(t.parent, nn);
+  r.host(t.child_first, nn);
+  r.host(t.plo, nn);
+  r.host(t.phi, nn);
+  r.host(t.col, (int64_t)NCOLL * nn);
+  r.host(t.perm, N);
+  r.host(t.point_node, N);
+  r.host(t.leaf, nn);
+  r.host(t.ghost, nn);
+  r.host(t.xlist, nn);
+  r.host(t.has_source, nn);
+  r.host(t.sx, N);
+  r.host(t.sy, N);
+  r.host(t.sz, N);
+  r.host(centers, 3 * nn);        // formed again from the cells on upload
+  r.host(s2m, s2m_off[L]);
+  r.host(leaves, leaf_off[L]);
+  if (r.n != size) fail(FFMM_ERR_ARGUMENT, "plan data: trailing bytes");
+  check_host_tree(t, "plan data", h.max_levels);
+  std::vector<int32_t> s2m_h, leaves_h;
+  std::vector<int64_t> s2m_off_h, leaf_off_h;
+  host_node_lists(t, s2m_h, s2m_off_h, leaves_h, leaf_off_h);
+  if (s2m_h != s2m || leaves_h != leaves || s2m_off_h != s2m_off || leaf_off_h != leaf_off)
+    fail(FFMM_ERR_ARGUMENT, "plan data: node lists that do not match the tree");
+}
+
+// a plan from plan data made by plan_save with a cache of the same operators
+void plan_restore(Plan &p, Context &ctx, OpCache &o, const void *buf, size_t size) {
+  PlanHeader h;
+  HostTree t;
+  parse_plan_data(buf, size, h, t);
+  if (h.method != o.method || h.kernel != o.kernel || h.p != o.p || h.nb != o.nb || h.Ns != o.Ns || h.Nl != o.Nl ||
+      h.w_ref != o.w_ref || h.kappa != o.kappa)
     fail(FFMM_ERR_ARGUMENT, "the plan data was made with another operator cache (method, order, kernel or width)");
-  const int64_t N = r.val<int64_t>();
-  const int ppl = r.val<int32_t>(), max_levels = r.val<int32_t>(), precision = r.val<int32_t>();
-  const int m2l = r.val<int32_t>();
-  double origin[3];
-  for (int d = 0; d < 3; d++) origin[d] = r.val<double>();
-  const double W0 = r.val<double>();
-  const int e0 = r.val<int32_t>(), L = r.val<int32_t>();
-  const int64_t nnodes = r.val<int64_t>();
-  plan_check(ctx, o, N, ppl, W0, e0, precision, m2l);
-  if (L < 1 || L > MAX_KEY_LEVELS + 1 || nnodes < 1) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
+  plan_check(ctx, o, h.N, h.ppl, t.W0, h.e0, h.precision, h.m2l);
   set_device(ctx);
   AllocScope scope(ctx);
-  plan_fields(p, ctx, o, N, ppl, origin, W0, e0, max_levels, precision, m2l);
-  p.L = L;
-  p.nnodes = nnodes;
-  r.host(p.lo, L);
-  r.host(p.nl, L);
-  r.host(p.s2m_off, L + 1);
-  r.host(p.leaf_off, L + 1);
-  for (int l = 0; l < L; l++)
-    if (p.lo[l] < 0 || p.nl[l] < 0 || p.lo[l] + p.nl[l] > nnodes) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
-  if (p.s2m_off[L] > nnodes || p.leaf_off[L] > nnodes) fail(FFMM_ERR_ARGUMENT, "plan data: bad tree sizes");
-  plan_tree_arrays(r, ctx, p);
-  if (r.n != size) fail(FFMM_ERR_ARGUMENT, "plan data: trailing bytes");
+  plan_fields(p, ctx, o, h.N, h.ppl, t.origin, t.W0, h.e0, h.max_levels, h.precision, h.m2l);
+  upload_host_tree(p, t);
+  plan_finish(p);
+}
+
+// a plan from a tree built on the host (ffmm_plan_from_tree)
+void plan_from_tree(Plan &p, Context &ctx, OpCache &o, HostTree &t, int ppl, int e0, int precision, int m2l) {
+  plan_check(ctx, o, t.N, ppl, t.W0, e0, precision, m2l);
+  check_host_tree(t, "the tree", 0);
+  set_device(ctx);
+  AllocScope scope(ctx);
+  plan_fields(p, ctx, o, t.N, ppl, t.origin, t.W0, e0, 30, precision, m2l);
+  upload_host_tree(p, t);
   plan_finish(p);
 }
 
@@ -870,7 +934,7 @@ void plan_eval_sources(Plan &p, double *out, int output) {
   p.ctx->sync();
 }
 
-void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output) {
+void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output, const int32_t *nodes) {
   check_output(p, output);
   if (nt < 0) fail(FFMM_ERR_DIMENSION, "negative target count");
   if (nt == 0) return;
@@ -879,7 +943,7 @@ void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, 
   AllocScope scope(*p.ctx);
   {
     StageTimer tm(p, 6);
-    evaluate_targets(p, targets, nt, out, output);
+    evaluate_targets(p, targets, nt, out, output, nodes);
   }
   p.ctx->sync();
 }
@@ -929,6 +993,70 @@ int ffmm_plan_load(ffmm_opcache cache, const void *buf, int64_t size, ffmm_plan 
     OpCache &o = cache->c;
     auto h = std::make_unique<ffmm_plan_s>();
     plan_restore(h->c, *o.ctx, o, buf, (size_t)size);
+    *out = h.release();
+  });
+}
+
+int ffmm_plan_data_check(const void *buf, int64_t size) {
+  return guard([&] {
+    if (size < 0) fail(FFMM_ERR_ARGUMENT, "negative size");
+    PlanHeader h;
+    HostTree t;
+    parse_plan_data(buf, (size_t)size, h, t);
+  });
+}
+
+int ffmm_plan_from_tree(ffmm_opcache cache, int nlevels, const int64_t *level_start, int64_t npoints,
+                        const int32_t *cell, const int32_t *parent, const int32_t *child_first, const int32_t *plo,
+                        const int32_t *phi, const uint8_t *leaf, const uint8_t *ghost, const uint8_t *xlist,
+                        const int32_t *colleagues, const int32_t *perm, const double *sorted_points, int ppl,
+                        const double origin[3], double width, int e0, int precision, int m2l, ffmm_plan *out) {
+  return guard([&] {
+    if (!cache || !out || !origin || !level_start || !cell || !parent || !child_first || !plo || !phi || !leaf ||
+        !ghost || !xlist || !colleagues || !perm || !sorted_points)
+      fail(FFMM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (nlevels < 1 || nlevels > MAX_KEY_LEVELS + 1)
+      fail(FFMM_ERR_ARGUMENT, "the tree has " + std::to_string(nlevels) + " levels (1 .. 22 supported)");
+    if (level_start[0] != 0) fail(FFMM_ERR_ARGUMENT, "the tree: level 0 does not start at node 0");
+    const int64_t nn = level_start[nlevels];
+    if (nn < 1 || nn >= INT32_MAX / NCOLL) fail(FFMM_ERR_ARGUMENT, "the tree: the node count is out of range");
+    if (npoints < 1 || npoints >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "the tree: the point count is out of range");
+    HostTree t;
+    t.L = nlevels;
+    t.nn = nn;
+    t.N = npoints;
+    for (int d = 0; d < 3; d++) t.origin[d] = origin[d];
+    t.W0 = width;
+    for (int l = 0; l < nlevels; l++) {
+      t.lo.push_back(level_start[l]);
+      t.nl.push_back(level_start[l + 1] - level_start[l]);
+    }
+    t.cell.resize(3 * nn);
+    t.col.resize((size_t)NCOLL * nn);
+    for (int64_t i = 0; i < nn; i++) {
+      for (int d = 0; d < 3; d++) t.cell[d * nn + i] = cell[3 * i + d];
+      for (int k = 0; k < NCOLL; k++) t.col[(size_t)k * nn + i] = colleagues[(size_t)NCOLL * i + k];
+    }
+    t.parent.assign(parent, parent + nn);
+    t.child_first.assign(child_first, child_first + nn);
+    t.plo.assign(plo, plo + nn);
+    t.phi.assign(phi, phi + nn);
+    t.leaf.assign(leaf, leaf + nn);
+    t.ghost.assign(ghost, ghost + nn);
+    t.xlist.assign(xlist, xlist + nn);
+    t.perm.assign(perm, perm + npoints);
+    t.sx.resize(npoints);
+    t.sy.resize(npoints);
+    t.sz.resize(npoints);
+    for (int64_t j = 0; j < npoints; j++) {
+      t.sx[j] = sorted_points[3 * j];
+      t.sy[j] = sorted_points[3 * j + 1];
+      t.sz[j] = sorted_points[3 * j + 2];
+    }
+    OpCache &o = cache->c;
+    auto h = std::make_unique<ffmm_plan_s>();
+    plan_from_tree(h->c, *o.ctx, o, t, ppl, e0, precision, m2l);
     *out = h.release();
   });
 }
@@ -1052,6 +1180,15 @@ int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out
   return guard([&] {
     if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
     plan_eval_targets(plan->c, targets, nt, out, output);
+  });
+}
+
+int ffmm_evaluate_nodes(ffmm_plan plan, const double *targets, int64_t nt, const int32_t *nodes, double *out,
+                        int output) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    if (nt > 0 && !nodes) fail(FFMM_ERR_ARGUMENT, "null nodes");
+    plan_eval_targets(plan->c, targets, nt, out, output, nodes);
   });
 }
 

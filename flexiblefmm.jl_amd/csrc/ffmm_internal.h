@@ -403,12 +403,15 @@ void m2l_spectral_level(Plan &plan, int l);
 // ---- direct sums (direct.hip) ----
 void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
             const int64_t *skip, double *out, int output);
+// output: FFMM_POTENTIAL (the velocity, 3 x nt) or FFMM_PRESSURE (nt)
 void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
-                   int64_t nt, const int64_t *skip, double *out);
+                   int64_t nt, const int64_t *skip, double *out, int output = FFMM_POTENTIAL);
 
 // ---- evaluation (evaluate.hip) ----
 void evaluate_at_sources(Plan &plan, double *out, int output);
-void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output);
+// nodes: the leaf of each target (device, 0-based), or null to locate them on the device
+void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out, int output,
+                      const int32_t *nodes = nullptr);
 
 // the arguments of the evaluation kernels: per leaf slot, its targets (or its own
 // sources), the far field of its leaf and the near field of its colleagues
@@ -479,6 +482,29 @@ __device__ __forceinline__ int32_t near_source(const NearList &nl, int32_t f) {
   return nl.lo[m] + (f - nl.pre[m]);
 }
 
+// ---- a tree on the host (host_tree.hip) ----
+// The flat tree in the plan's layout (cell: 3 x nn SoA; col: [k * nn + i]; sorted points
+// SoA), before it is checked and uploaded.  level, has_source and point_node may be left
+// empty: check_host_tree forms them (and checks them when given).
+struct HostTree {
+  int L = 0;
+  int64_t nn = 0, N = 0;
+  double origin[3] = {0, 0, 0}, W0 = 1.0;
+  std::vector<int64_t> lo, nl;
+  std::vector<int32_t> cell, level, parent, child_first, plo, phi, col, perm, point_node;
+  std::vector<uint8_t> leaf, ghost, xlist, has_source;
+  std::vector<double> sx, sy, sz;
+};
+// every invariant the kernels rely on; fails with FFMM_ERR_ARGUMENT ("<what>: ...")
+void check_host_tree(HostTree &t, const char *what, int max_levels);
+// the node lists of the passes (childless sourced nodes, real leaves) with per-level offsets
+void host_node_lists(const HostTree &t, std::vector<int32_t> &s2m, std::vector<int64_t> &s2m_off,
+                     std::vector<int32_t> &leaves, std::vector<int64_t> &leaf_off);
+// a checked tree into the plan's device arrays, with its centres and node lists
+void upload_host_tree(Plan &p, HostTree &t);
+// the node centres from the cells and levels (tree.hip)
+void compute_centers(Plan &p);
+
 // ---- the life cycles (ffmm_abi.hip), shared by the C entry points and the server ----
 void context_init(Context &c, int device, void *stream);
 void context_release(Context &c);
@@ -489,9 +515,12 @@ void plan_release(Plan &p);
 size_t plan_save(Plan &p, void *buf, size_t cap);
 // a plan from plan data of a cache with the same operators: the tree uploaded, the rest rebuilt
 void plan_restore(Plan &p, Context &ctx, OpCache &o, const void *buf, size_t size);
+// a plan from a tree built on the host, checked in full before its upload
+void plan_from_tree(Plan &p, Context &ctx, OpCache &o, HostTree &t, int ppl, int e0, int precision, int m2l);
 void plan_build(Plan &p, const double *dens, int nd, int sorted);
 void plan_eval_sources(Plan &p, double *out, int output);
-void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output);
+void plan_eval_targets(Plan &p, const double *targets, int64_t nt, double *out, int output,
+                       const int32_t *nodes = nullptr);
 // device bytes held by a plan
 size_t plan_bytes(const Plan &p);
 

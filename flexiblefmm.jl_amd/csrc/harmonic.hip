@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
         if (OUT != FFMM_POTENTIAL) yukawa_u(z2, p + 1, ugr);
       }
       if (OUT != FFMM_GRADIENT) {
-        // _harmonic_potential (src/operators/harmonics.jl:462-468)
+        // _harmonic_potential (src/operators/harmonics.jl:455-462)
         double pot = 0.0;
         for (int m = rep; m <= p; m += R)
           chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
@@ -467,8 +467,8 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
         acc[0] = pot;
       }
       if (OUT != FFMM_POTENTIAL) {
-        // _harmonic_gradient (src/operators/harmonics.jl:469-491); Yukawa adds
-        // x * (potential with the radial weights) (src/operators/harmonics_yukawa.jl:318-327)
+        // _harmonic_gradient (src/operators/harmonics.jl:463-485); Yukawa adds
+        // x * (potential with the radial weights) (src/operators/harmonics_yukawa.jl:306-313)
         double g0 = 0.0, g1 = 0.0, g2 = 0.0, pr = 0.0;
         for (int m1 = rep; m1 <= p - 1; m1 += R)
           chain(tb, xh, yh, zh, m1, p - 1, [&](int n1, double vr, double vi) {

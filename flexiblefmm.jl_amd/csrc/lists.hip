@@ -8,7 +8,7 @@
 // order, each target's consecutive, in pair order) and filled; the class-major scan and
 // the target chunks of the dense class M2L come with them.
 //
-// Follows src/device_lists.jl:261-344, 425-484 (= src/tree/lists.jl:73-124 for the
+// Follows src/device_lists.jl:57-157, 219-278 (= src/tree/lists.jl:73-124 for the
 // pairs and src/operators/symmetry.jl:207-236 for the class lists); 0-based.
 #include "ffmm_internal.h"
 
@@ -51,7 +51,7 @@ __device__ __forceinline__ int32_t pair_lane_count(const TreePtrs &t, int64_t i,
   return c;
 }
 
-// _pair_count! (src/device_lists.jl:263-285)
+// _pair_count! (src/device_lists.jl:57-78)
 __global__ void k_pair_count(TreePtrs t, int64_t lo, int64_t nl, int32_t *__restrict__ cnt) {
   const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PL;
   const int k = threadIdx.x % PL;
@@ -64,7 +64,7 @@ __global__ void k_pair_count(TreePtrs t, int64_t lo, int64_t nl, int32_t *__rest
   if (k == 0 && q < nl) cnt[q] = n;
 }
 
-// _pair_fill! (src/device_lists.jl:286-312)
+// _pair_fill! (src/device_lists.jl:80-106)
 __global__ void k_pair_fill(TreePtrs t, int64_t lo, int64_t nl, const int32_t *__restrict__ ptr,
                             int32_t *__restrict__ src, int32_t *__restrict__ far) {
   const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PL;
@@ -105,7 +105,7 @@ __device__ __forceinline__ void class_batch(int32_t c, int lane, int32_t &rank, 
   }
 }
 
-// _class_count! (src/device_lists.jl:314-324), class-major: cnt[c * nl + q]
+// _class_count! (src/device_lists.jl:108-117), class-major: cnt[c * nl + q]
 __global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__restrict__ ptr, int64_t nl, int nclass,
                               const int32_t *__restrict__ far_class, int32_t *__restrict__ cnt) {
   const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -123,7 +123,7 @@ __global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__
   if (lane < nclass) cnt[(int64_t)lane * nl + q] = total;
 }
 
-// _class_fill! (src/device_lists.jl:325-334)
+// _class_fill! (src/device_lists.jl:119-129)
 __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__restrict__ far,
                              const int32_t *__restrict__ ptr, const int32_t *__restrict__ cstart, int64_t nl,
                              int nclass, const int32_t *__restrict__ far_class, int32_t *__restrict__ csrc,

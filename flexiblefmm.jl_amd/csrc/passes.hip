@@ -208,7 +208,7 @@ __global__ void k_fill(double *__restrict__ a, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
-// the densities into the tree's order: _gather_perm! (src/device_lists.jl:369-372)
+// the densities into the tree's order: _gather_perm! (src/device_lists.jl:163-171)
 __global__ void k_gather_perm(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ perm,
                               int64_t n) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -530,10 +530,7 @@ void build_tree(Plan &plan, const double *xyz) {
                                                       poff, noff);
     FFMM_LAUNCH_CHECK();
   }
-  plan.centers.resize(3 * nn);
-  k_centers<<<nblocks(nn, TB), TB, 0, s>>>(plan.cell.p, plan.level.p, nn, plan.origin[0], plan.origin[1],
-                                           plan.origin[2], plan.W0, plan.centers.p);
-  FFMM_LAUNCH_CHECK();
+  compute_centers(plan);
   // node lists: both compacted over all nodes at once (node order is level order), the
   // level boundaries found on the device; one read back
   plan.s2m_off.assign(L + 1, 0);
@@ -554,6 +551,14 @@ void build_tree(Plan &plan, const double *xyz) {
     plan.s2m_off[l] = ctx.pinned[2 + l];
     plan.leaf_off[l] = ctx.pinned[2 + (MAX_KEY_LEVELS + 2) + l];
   }
+}
+
+void compute_centers(Plan &plan) {
+  const int64_t nn = plan.nnodes;
+  plan.centers.resize(3 * nn);
+  k_centers<<<nblocks(nn, TB), TB, 0, plan.ctx->stream>>>(plan.cell.p, plan.level.p, nn, plan.origin[0],
+                                                          plan.origin[1], plan.origin[2], plan.W0, plan.centers.p);
+  FFMM_LAUNCH_CHECK();
 }
 
 // node -> position in the leaf list (-1: not a real leaf)

@@ -8,6 +8,6 @@ from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 from .fmm import (FMM, CubeKernelIndependent, DevicePlan, FMMState, Gradient, Laplace3D, OperatorCache,  # noqa: F401
                   Potential, PotentialGradient, Pressure, SphericalHarmonics, StokesHarmonics, Stokeslet3D, Velocity, Yukawa3D,
                   build_expansions, build_expansions_, device_build_expansions_, device_matvec_, direct, direct_self,
-                  evaluate, evaluate_at_sources, square_bbox)
+                  evaluate, evaluate_at_sources, plan_data_check, square_bbox)
 from .stokes_api import (StokesFMMKernel, StokesFMMPlan, stokes_densities, stokes_fmm_eval_,  # noqa: F401
                          stokes_fmm_eval_src_trg_)

@@ -44,6 +44,7 @@ SIGNATURES = {
     "ffmm_bounds": (_i32, [_vp, _vp, _i64, _vp, _vp, ctypes.POINTER(_i32)]),
     "ffmm_direct": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i32]),
     "ffmm_direct_stokes": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _i64, _vp, _vp]),
+    "ffmm_direct_stokes_pressure": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "ffmm_opcache_create": (_i32, [_vp, _i32, _i32, _d, ctypes.POINTER(_vp)]),
     "ffmm_opcache_destroy": (_i32, [_vp]),
     "ffmm_opcache_put_cube": (_i32, [_vp, _d, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32,
@@ -60,11 +61,15 @@ SIGNATURES = {
     "ffmm_plan_perm": (_i32, [_vp, _vp]),
     "ffmm_plan_save": (_i32, [_vp, _vp, _i64, ctypes.POINTER(_i64)]),
     "ffmm_plan_load": (_i32, [_vp, _vp, _i64, ctypes.POINTER(_vp)]),
+    "ffmm_plan_data_check": (_i32, [_vp, _i64]),
+    "ffmm_plan_from_tree": (_i32, [_vp, _i32, _vp, _i64] + [_vp] * 11 + [_i32, _vp, _d, _i32, _i32, _i32,
+                                                                       ctypes.POINTER(_vp)]),
     "ffmm_plan_download_lists": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "ffmm_build_expansions": (_i32, [_vp, _vp, _i32]),
     "ffmm_build_expansions_nd": (_i32, [_vp, _vp, _i32, _i32]),
     "ffmm_evaluate_at_sources": (_i32, [_vp, _vp, _i32]),
     "ffmm_evaluate": (_i32, [_vp, _vp, _i64, _vp, _i32]),
+    "ffmm_evaluate_nodes": (_i32, [_vp, _vp, _i64, _vp, _vp, _i32]),
     "ffmm_matvec": (_i32, [_vp, _vp, _i32, _vp, _i32]),
     "ffmm_plan_download_expansions": (_i32, [_vp, _vp, _vp, _vp]),
     "ffmm_plan_stage_times": (_i32, [_vp, _vp, _i32]),

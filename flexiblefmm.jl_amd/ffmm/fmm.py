@@ -6,16 +6,16 @@ error phrases; ``!`` becomes a trailing underscore):
 =====================================================  ==========================================
 ``Laplace3D()``  (src/kernels.jl:179-200)               ``Laplace3D()``
 ``CubeKernelIndependent{3}(K; p, pc, d, pinv_rtol)``    ``CubeKernelIndependent(K, p=, pc=, d=)``
-  (src/operators/cubeki.jl:386-392)
+  (src/operators/cubeki.jl:18-48)
 ``OperatorCache(method)``, ``fit_box!``                 ``OperatorCache(method)``, ``.fit_box``
-  (src/operators/cache.jl:196-274)
+  (src/operators/cache.jl:5-83)
 ``FMM(method, sources; ppl, bbox, cache, max_levels)``  ``FMM(method, sources, ppl=, bbox=, ...)``
   (src/fmm.jl:31-49)
 ``build_expansions(f, sources, tau)`` (src/fmm.jl:306)  ``build_expansions(f, sources, tau)``
 ``build_expansions!(state, f, sources, tau)`` (:271)    ``build_expansions_(state, f, sources, tau)``
 ``evaluate_at_sources(f, st; output)``                  ``evaluate_at_sources(f, st, output=)``
-  (src/evaluate.jl:443)
-``evaluate(f, st, targets; output)`` (:412)             ``evaluate(f, st, targets, output=)``
+  (src/evaluate.jl:129-133)
+``evaluate(f, st, targets; output)`` (src/evaluate.jl:98)             ``evaluate(f, st, targets, output=)``
 ``DevicePlan(m, xs; ppl, bbox, cache, max_levels)``     ``DevicePlan(m, xs, ppl=, bbox=, ...)``
   (src/device_passes.jl:113-134)
 ``device_build_expansions!(plan, ds; sorted)`` (:511)   ``device_build_expansions_(plan, ds, sorted=)``
@@ -104,7 +104,7 @@ def _output(o):
 
 class CubeKernelIndependent:
     """``CubeKernelIndependent{3}(kernel; p=8, pc=2p, d=0.1, pinv_rtol=50 eps)``
-    (src/operators/cubeki.jl:375-392): p points per edge on the small surface, pc on the
+    (src/operators/cubeki.jl:18-48): p points per edge on the small surface, pc on the
     large one, margin d."""
 
     def __init__(self, kernel, p: int = 8, pc: int | None = None, d: float = 0.1,
@@ -245,7 +245,10 @@ def direct(K, sources, tau, targets, output=Potential, skip=None, device: int = 
     """``direct(K, sources, tau, targets; output)``: the O(N M) sum at the targets,
     excluding source ``skip[j]`` at target ``j`` when ``skip`` is given."""
     if isinstance(K, Stokeslet3D):
-        return _direct_stokes(sources, tau, targets, skip, device)
+        o = _output(output)
+        if o.code not in (_lib.POTENTIAL, _lib.PRESSURE):
+            raise ArgumentError("the Stokes direct sum gives the velocity (Potential) or the Pressure")
+        return _direct_stokes(sources, tau, targets, skip, device, pressure=o.code == _lib.PRESSURE)
     if not isinstance(K, Laplace3D):
         raise ArgumentError("the device direct sum takes Laplace3D or Stokeslet3D")
     o = _output(output)
@@ -267,8 +270,9 @@ def direct(K, sources, tau, targets, output=Potential, skip=None, device: int = 
     return out.cpu().numpy()
 
 
-def _direct_stokes(sources, dens, targets, skip, device: int) -> np.ndarray:
-    """The Stokes velocity at the targets: dens (N, 3) stokeslets or (N, 9) rows [f, g, nu]."""
+def _direct_stokes(sources, dens, targets, skip, device: int, pressure: bool = False) -> np.ndarray:
+    """The Stokes velocity (or the pressure) at the targets: dens (N, 3) stokeslets or
+    (N, 9) rows [f, g, nu]."""
     torch = _torch()
     s = _dev_f64(sources, device, cols=3)
     d = _dev_f64(dens, device)
@@ -276,13 +280,13 @@ def _direct_stokes(sources, dens, targets, skip, device: int) -> np.ndarray:
     if d.ndim != 2 or d.shape[1] not in (3, 9) or d.shape[0] != s.shape[0]:
         raise DimensionMismatch("expected (N, 3) stokeslets or (N, 9) rows [f, g, nu]")
     nt = x.shape[0]
-    out = torch.empty((nt, 3), dtype=torch.float64, device=f"cuda:{device}")
+    out = torch.empty((nt,) if pressure else (nt, 3), dtype=torch.float64, device=f"cuda:{device}")
     sk = None
     if skip is not None:
         sk = torch.as_tensor(np.asarray(skip, dtype=np.int64)).to(f"cuda:{device}")
         if sk.shape[0] != nt:
             raise DimensionMismatch("skip and targets differ in length")
-    call("ffmm_direct_stokes", context(device), _ptr(s), _ptr(d), int(d.shape[1]), s.shape[0], _ptr(x), nt,
+    call("ffmm_direct_stokes_pressure" if pressure else "ffmm_direct_stokes", context(device), _ptr(s), _ptr(d), int(d.shape[1]), s.shape[0], _ptr(x), nt,
          None if sk is None else _ptr(sk), _ptr(out))
     return out.cpu().numpy()
 
@@ -316,7 +320,7 @@ class OperatorCache:
         return not math.isnan(self.W0)
 
     def fit_box(self, lo, hi, bbox):
-        """``fit_box!(cache, lo, hi, bbox)`` (src/operators/cache.jl:254-274): a box of
+        """``fit_box!(cache, lo, hi, bbox)`` (src/operators/cache.jl:57-83): a box of
         width ``ldexp(W0, e)`` containing ``bbox``; the first call fixes ``W0``."""
         lo = np.asarray(lo, dtype=np.float64)
         hi = np.asarray(hi, dtype=np.float64)
@@ -354,7 +358,7 @@ class OperatorCache:
 
     def harmonic_operators(self, w: float):
         """``get_operators!`` of a harmonic method for boxes of width ``w``
-        (src/operators/cache.jl:224-248): Laplace and Stokes take the reference set of
+        (src/operators/cache.jl:33-46): Laplace and Stokes take the reference set of
         width W0 for every width (homogeneous), Yukawa one set per width."""
         from . import harmonics as _h
         m = self.method
@@ -438,10 +442,17 @@ class OperatorCache:
         return 0 if self._host is None else 1
 
 
+def plan_data_check(data: bytes) -> None:
+    """The host checks of plan data (``ffmm_plan_data_check``): raises ``ArgumentError``
+    with the first violation of the tree's structure; needs no device."""
+    buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    call("ffmm_plan_data_check", buf, len(data))
+
+
 def square_bbox(lo, hi):
     """The square box centred on the bounding box ``[lo, hi]`` with side its largest
     extent (1 if all points coincide), widened until it contains it
-    (src/tree/tree.jl:235-252)."""
+    (src/tree/tree.jl:97-114)."""
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
     if not (np.all(np.isfinite(lo)) and np.all(np.isfinite(hi))):
@@ -471,7 +482,7 @@ _M2L = {"dense": _lib.M2L_DENSE, "spectral": _lib.M2L_SPECTRAL}
 @dataclass
 class TreeArrays:
     """The flat tree downloaded from the device (0-based, -1 = none): the fields of the
-    reference's ``Tree`` (src/tree/tree.jl:178-209) with ``points`` as ``plo``/``phi``."""
+    reference's ``Tree`` (src/tree/tree.jl:40-72) with ``points`` as ``plo``/``phi``."""
     origin: np.ndarray
     W0: float
     level_start: np.ndarray
@@ -602,6 +613,62 @@ class DevicePlan:
         self._h = h
         self.group_sizes = [int(g) for g in head["group_sizes"]]
         self.group_offsets = np.concatenate([[0], np.cumsum(self.group_sizes)]).astype(np.int64)
+        self._gperm = None
+        self._read_sizes()
+        if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
+            cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+        return self
+
+    @classmethod
+    def from_tree(cls, method, tree, *, cache: OperatorCache = None, ppl: int = 50, precision: str = "f64",
+                  m2l: str = "dense", device: int = 0) -> "DevicePlan":
+        """``DevicePlan(f::FMM, to)`` (src/device_passes.jl:90-96): the plan from a tree
+        built on the host -- any object with the fields of ``TreeArrays`` (``origin``,
+        ``W0``, ``level_start``, ``cell``, ``parent``, ``child_first``, ``plo``, ``phi``,
+        ``leaf``, ``ghost``, ``xlist``, ``colleagues``, ``perm`` and the tree-order
+        ``points``), 0-based.  The tree is checked in full on the host before its upload
+        (``ffmm_plan_from_tree``); the lists and buffers are built on the device.  The
+        root box must be a width of the cache (``w_ref * 2^e``), which it is when the
+        cache fitted it (``fit_box``) or holds no width yet."""
+        cache = OperatorCache(method) if cache is None else cache
+        if cache.method is not method:
+            raise ArgumentError("the cache was built for a different method")
+        if precision not in _PREC:
+            raise ArgumentError(f"unknown precision {precision!r}")
+        if m2l not in _M2L:
+            raise ArgumentError(f"unknown M2L form {m2l!r}")
+        origin = np.ascontiguousarray(tree.origin, dtype=np.float64).reshape(3)
+        W = float(tree.W0)
+        if not cache.has_base_width():
+            cache.W0 = W
+        f, e0 = math.frexp(W / cache.W0)
+        if f != 0.5:
+            raise ArgumentError("the tree's box width is not a width of the cache (w_ref * 2^e)")
+        e0 -= 1
+        c = lambda a, t: np.ascontiguousarray(a, dtype=t)   # noqa: E731
+        ls = c(tree.level_start, np.int64)
+        arrs = [c(tree.cell, np.int32), c(tree.parent, np.int32), c(tree.child_first, np.int32),
+                c(tree.plo, np.int32), c(tree.phi, np.int32), c(tree.leaf, np.uint8), c(tree.ghost, np.uint8),
+                c(tree.xlist, np.uint8), c(tree.colleagues, np.int32), c(tree.perm, np.int32),
+                c(tree.points, np.float64)]
+        N = len(arrs[9])
+        nn = int(ls[-1])
+        if arrs[0].size != 3 * nn or arrs[8].size != 27 * nn or arrs[10].size != 3 * N or \
+                any(len(a) != nn for a in arrs[1:8]):
+            raise DimensionMismatch("the tree's arrays differ in length")
+        oc = cache.device(device)
+        h = ctypes.c_void_p()
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        call("ffmm_plan_from_tree", oc, len(ls) - 1, P(ls), N, *[P(a) for a in arrs], int(ppl),
+             (ctypes.c_double * 3)(*origin), W, e0, _PREC[precision], _M2L[m2l], ctypes.byref(h))
+        self = cls.__new__(cls)
+        self._h = h
+        self.method, self.cache, self.device = method, cache, device
+        self.precision, self.m2l = precision, m2l
+        self.origin, self.W0, self.e0 = origin.copy(), W, e0
+        self.N, self.ppl = N, int(ppl)
+        self.group_sizes = [N]
+        self.group_offsets = np.array([0, N], dtype=np.int64)
         self._gperm = None
         self._read_sizes()
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
@@ -762,18 +829,29 @@ class DevicePlan:
         call("ffmm_evaluate_at_sources", self._h, _ptr(out), o.code)
         return out
 
-    def evaluate(self, targets, output=Potential, out=None):
+    def evaluate(self, targets, output=Potential, out=None, nodes=None):
+        """The output at ``targets``, located on the device, or in the real leaves
+        ``nodes`` (0-based node indices, e.g. from the host ``locate``) when given:
+        ``evaluate!(out, dev, ds, targets, nodes)`` (src/device.jl:127-145)."""
         o = _output(output)
         torch = _torch()
         t = _dev_f64(targets, self.device, cols=3)
         nt = t.shape[0]
+        if nodes is not None:
+            nodes = torch.as_tensor(np.asarray(nodes) if not torch.is_tensor(nodes) else nodes).to(
+                device=f"cuda:{self.device}", dtype=torch.int32).contiguous()
+            if nodes.shape != (nt,):
+                raise DimensionMismatch("out, targets and nodes differ in length")
         nc = self._ncomp(o)
         shape = (nt,) if nc == 1 else (nt, nc)
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
         elif tuple(out.shape) != shape:
             raise DimensionMismatch("out and targets differ in length")
-        call("ffmm_evaluate", self._h, _ptr(t), nt, _ptr(out), o.code)
+        if nodes is None:
+            call("ffmm_evaluate", self._h, _ptr(t), nt, _ptr(out), o.code)
+        else:
+            call("ffmm_evaluate_nodes", self._h, _ptr(t), nt, _ptr(nodes), _ptr(out), o.code)
         return out
 
     def matvec(self, tau, output=Potential, sorted: bool = False, out=None):
@@ -896,7 +974,7 @@ def build_expansions(f: FMM, sources, tau) -> FMMState:
 
 
 def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -> np.ndarray:
-    """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:418-446): the
+    """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:104-133): the
     output at every source of group g (1-based) in the original order."""
     G = f.plan.ngroups
     if not 1 <= g <= G:
@@ -909,7 +987,7 @@ def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -
 
 
 def evaluate(f: FMM, state: FMMState, targets, output=Potential) -> np.ndarray:
-    """``evaluate(fmm, state, targets; output)`` (src/evaluate.jl:384-416)."""
+    """``evaluate(fmm, state, targets; output)`` (src/evaluate.jl:81-102)."""
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
     return f.plan.evaluate(targets, output).cpu().numpy()

@@ -8,7 +8,7 @@ receives the finished matrices (``ffmm_opcache_put_cube``).  It restates:
 * the factored truncated-SVD pseudo-inverses ``pinv_rtol`` (src/operators/ki_common.jl:38-44);
 * the folded M2M / L2L of ``_ki_dense_operators`` (src/operators/ki_common.jl:176-199)
   at the half-widths ``(1+d)w/2`` and ``(3-2d)w/2`` of ``level_operators``
-  (src/operators/cubeki.jl:404-430);
+  (src/operators/cubeki.jl:55-81);
 * the symmetry classes of the 316 far offsets under the 48 signed permutations and
   the class representatives (src/operators/symmetry.jl:14-131);
 * the spectral M2L kernel tables (src/operators/spectral_m2l.jl:119-135).
@@ -181,7 +181,7 @@ class CubeOperators:
 def level_operators(w: float, p: int = 8, pc: int | None = None, d: float = 0.1,
                     rtol: float = 50 * np.finfo(np.float64).eps, with_classes: bool = True,
                     with_tables: bool = True) -> CubeOperators:
-    """The cube KI operators of width ``w`` (src/operators/cubeki.jl:404-430 with
+    """The cube KI operators of width ``w`` (src/operators/cubeki.jl:55-81 with
     src/operators/ki_common.jl:176-199)."""
     pc = 2 * p if pc is None else pc
     if pc < p:

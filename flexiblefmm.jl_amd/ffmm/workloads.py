@@ -91,3 +91,43 @@ def stokes_densities(normals: np.ndarray, seed: int = 2) -> np.ndarray:
     rng = np.random.default_rng(seed)
     n = len(normals)
     return np.hstack([rng.standard_normal((n, 3)), rng.standard_normal((n, 3)), normals])
+
+
+def served_stream(nreq: int, nmin: int = 10_000, nmax: int = 2_000_000, rate: float = 100.0, seed: int = 5,
+                  p_laplace: float = 0.7, p_repeat: float = 0.4):
+    """The served mixed-size stream: ``nreq`` requests with Poisson arrivals at ``rate``
+    per second, point counts log-uniform in [nmin, nmax], a fraction ``p_laplace`` for
+    Laplace3D on cube surfaces (p = 8) and the rest for Yukawa3D(5) spherical harmonics
+    (p = 10), uniform or Gaussian-mixture layouts (even odds), charges N(0, 1), a fraction
+    ``p_repeat`` repeating an earlier geometry of the same kernel with new charges, and
+    the 1e-6 and 1e-3 accuracy classes (even odds).
+
+    Returns request dicts: ``t`` (arrival, s), ``lap`` (Laplace or Yukawa), ``fine`` (the
+    1e-6 class), ``gid`` (geometry id: a repeat shares its first request's), ``n``,
+    ``layout``, ``seed`` (of the points) and ``tau_seed``; ``request_points(r)`` and
+    ``request_charges(r)`` make the arrays (the same for every request of a geometry)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs, geoms = [], {True: [], False: []}
+    for k in range(nreq):
+        lap = bool(rng.random() < p_laplace)
+        fine = bool(rng.random() < 0.5)
+        if geoms[lap] and rng.random() < p_repeat:
+            g = geoms[lap][int(rng.integers(len(geoms[lap])))]
+        else:
+            n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+            layout = "gaussian_mixture" if rng.random() < 0.5 else "uniform"
+            g = dict(gid=sum(len(v) for v in geoms.values()), n=n, layout=layout, seed=int(rng.integers(1 << 30)))
+            geoms[lap].append(g)
+        reqs.append(dict(t=float(t[k]), lap=lap, fine=fine, gid=g["gid"], n=g["n"], layout=g["layout"],
+                         seed=g["seed"], tau_seed=int(rng.integers(1 << 30))))
+    return reqs
+
+
+def request_points(r) -> np.ndarray:
+    return gaussian_mixture(r["n"], seed=r["seed"]) if r["layout"] == "gaussian_mixture" else \
+        uniform(r["n"], seed=r["seed"])
+
+
+def request_charges(r) -> np.ndarray:
+    return np.random.default_rng(r["tau_seed"]).standard_normal(r["n"])

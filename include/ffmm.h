@@ -81,15 +81,22 @@ int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t 
 int ffmm_direct_stokes(ffmm_context ctx, const double *src, const double *dens, int nd, int64_t ns,
                        const double *tgt, int64_t nt, const int64_t *skip, double *out);
 
+/* The FP64 direct Stokes pressure, one value per target: direct! with Pressure() for
+ * Stokeslet3D (src/kernels.jl:463-487 with pressure_kernel, src/kernels.jl:267-271) and
+ * the stresslet's pressure (src/sources.jl:237-242); arguments as ffmm_direct_stokes,
+ * out: nt. */
+int ffmm_direct_stokes_pressure(ffmm_context ctx, const double *src, const double *dens, int nd, int64_t ns,
+                                const double *tgt, int64_t nt, const int64_t *skip, double *out);
+
 /* The device copies of the operators of one method, shared by every plan built with
- * the cache: OperatorCache.device (src/operators/cache.jl:196-213,
+ * the cache: OperatorCache.device (src/operators/cache.jl:5-22,
  * src/device_passes.jl:136-146).  The method is CubeKernelIndependent{3}(Laplace3D(); p, pc, d)
- * (src/operators/cubeki.jl:386-392). */
+ * (src/operators/cubeki.jl:18-48). */
 int ffmm_opcache_create(ffmm_context ctx, int p, int pc, double d, ffmm_opcache *out);
 int ffmm_opcache_destroy(ffmm_opcache cache);
 
 /* Upload the reference-width operators (host FP64, column major), built by
- * level_operators(method, w_ref) (src/operators/cubeki.jl:404-430) and
+ * level_operators(method, w_ref) (src/operators/cubeki.jl:55-81) and
  * class_operators (src/operators/symmetry.jl:125-131):
  *   small (3 x Ns), large (3 x Nl) surfaces relative to the box centre;
  *   P_up = V_up diag(invS_up) Uh_up (Uh_up: r_up x Nl, V_up: Ns x r_up);
@@ -146,9 +153,9 @@ int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has);
 /* The plan from device points xyz (3 x n AoS, FP64, any order) in the box (origin, width):
  * DevicePlan(m, xs; ppl, bbox, cache, max_levels) (src/device_passes.jl:113-134) with the
  * device tree device_tree (src/tree/device_builder.jl:398-474), its lists
- * (_device_level_lists, src/device_lists.jl:425-484) and the buffers of _device_plan
+ * (_device_level_lists, src/device_lists.jl:219-278) and the buffers of _device_plan
  * (src/device_passes.jl:155-338).  The box must already be snapped to a width
- * width = w_ref * 2^e0 of the cache (fit_box!, src/operators/cache.jl:254-274).
+ * width = w_ref * 2^e0 of the cache (fit_box!, src/operators/cache.jl:57-83).
  * precision: FFMM_PREC_*; m2l: FFMM_M2L_*. */
 int ffmm_plan_create(ffmm_opcache cache, const double *xyz, int64_t n, int ppl, const double origin[3],
                      double width, int e0, int max_levels, int precision, int m2l, ffmm_plan *out);
@@ -164,6 +171,33 @@ int ffmm_plan_destroy(ffmm_plan plan);
  * order, kernel, reference width), rebuilding the lists and buffers. */
 int ffmm_plan_save(ffmm_plan plan, void *buf, int64_t cap, int64_t *size);
 int ffmm_plan_load(ffmm_opcache cache, const void *buf, int64_t size, ffmm_plan *out);
+
+/* The checks ffmm_plan_load runs on plan data before any of it reaches a device, without
+ * a device or a cache: every array's length, every index in range, and the structure of
+ * the reference's trees (sibling groups of 8 on consecutive levels, child cells by child
+ * slot, point ranges partitioning the parent's, colleagues at their offsets, a permutation
+ * of the points, every point in one real leaf, the node lists of the tree).  FFMM_OK or
+ * FFMM_ERR_ARGUMENT with the first violation.  No counterpart in the reference (its plans
+ * are built in process). */
+int ffmm_plan_data_check(const void *buf, int64_t size);
+
+/* The plan from a tree built on the host, e.g. by the reference's host builder:
+ * DevicePlan(f::FMM, to; tol, fast, eltype) (src/device_passes.jl:90-96) with the tree
+ * moved by DeviceTree(f.tree, to) (src/tree/device_builder.jl:54-66).  Host arrays in
+ * the layout of ffmm_plan_download_tree, 0-based (-1 = none): level_start (nlevels + 1:
+ * the first node of each level, then the node count), cell (3 x nnodes), parent,
+ * child_first, plo, phi (the points of node i are plo[i] .. phi[i] of the tree order),
+ * leaf, ghost, xlist (0/1 bytes), colleagues (27 per node, slot k at the offset
+ * (k mod 3 - 1, (k / 3) mod 3 - 1, k / 9 - 1)), perm (npoints: input index of each
+ * tree-order point) and sorted_points (3 x npoints, the points in tree order).  The box
+ * (origin, width = w_ref * 2^e0) is the tree's root.  The tree is checked in full as
+ * ffmm_plan_data_check checks plan data (FFMM_ERR_ARGUMENT on a violation), uploaded,
+ * and the lists and buffers are built as for ffmm_plan_create.  At most 22 levels. */
+int ffmm_plan_from_tree(ffmm_opcache cache, int nlevels, const int64_t *level_start, int64_t npoints,
+                        const int32_t *cell, const int32_t *parent, const int32_t *child_first, const int32_t *plo,
+                        const int32_t *phi, const uint8_t *leaf, const uint8_t *ghost, const uint8_t *xlist,
+                        const int32_t *colleagues, const int32_t *perm, const double *sorted_points, int ppl,
+                        const double origin[3], double width, int e0, int precision, int m2l, ffmm_plan *out);
 
 /* Tree and list sizes: nlevels, nnodes, npoints, and per level (arrays of length
  * nlevels, any may be NULL) the first node, the node count, the far pairs, the childless
@@ -186,7 +220,7 @@ int ffmm_plan_perm(ffmm_plan plan, int32_t *perm_dev);
 
 /* Download one level's far pairs, targets in node order: ptr (count + 1, relative to the
  * level's first node), src (global node), far (0-based far offset index), as
- * _pair_fill! writes them (src/device_lists.jl:286-312; src/tree/lists.jl:96-124). */
+ * _pair_fill! writes them (src/device_lists.jl:80-106; src/tree/lists.jl:96-124). */
 int ffmm_plan_download_lists(ffmm_plan plan, int level, int32_t *ptr, int32_t *src, int32_t *far);
 
 /* The upward and downward passes from densities tau (N, FP64, device): in the user's
@@ -212,6 +246,13 @@ int ffmm_evaluate_at_sources(ffmm_plan plan, double *out, int output);
  * (src/device.jl:134-145) with the host locate (src/tree/locate.jl:10-66) replaced by a
  * device walk.  Targets must lie in the root box (FFMM_ERR_ARGUMENT otherwise). */
 int ffmm_evaluate(ffmm_plan plan, const double *targets, int64_t nt, double *out, int output);
+
+/* Output at nt device targets located by the caller: evaluate!(out, dev, ds, targets,
+ * nodes; output) (src/device.jl:127-145), nodes (device, nt int32, 0-based) the real
+ * leaf of each target, as the host locate gives them (src/tree/locate.jl:10-47).  A node
+ * that is not a real leaf of the plan's tree is refused (FFMM_ERR_ARGUMENT). */
+int ffmm_evaluate_nodes(ffmm_plan plan, const double *targets, int64_t nt, const int32_t *nodes, double *out,
+                        int output);
 
 /* build_expansions then evaluate_at_sources: device_matvec! (src/device_passes.jl:619-640). */
 int ffmm_matvec(ffmm_plan plan, const double *tau, int sorted, double *out, int output);

@@ -18,7 +18,7 @@ kernel-independent surfaces:
   src/fmm.jl:100-127, the form the reference device plan uses) or spectral
   (``_spectral_m2l_level!``, src/operators/spectral_m2l.jl:328-451, the form the
   reference CPU driver uses for the cube method);
-* ``evaluate_at_sources!`` / ``evaluate!`` (src/evaluate.jl:395-441).
+* ``evaluate_at_sources!`` / ``evaluate!`` (src/evaluate.jl:81-127).
 
 The operators (surfaces, pseudo-inverse factors, folded M2M/L2L, class
 representatives, spectral tables) are supplied by the caller from the host
@@ -92,7 +92,7 @@ class OracleError(ValueError):
 @dataclass
 class Tree:
     """The flat tree (0-based; -1 = none), the fields of the reference's ``Tree``
-    (src/tree/tree.jl:178-209) with ``points`` as ``plo``/``phi``."""
+    (src/tree/tree.jl:40-72) with ``points`` as ``plo``/``phi``."""
     origin: np.ndarray
     W0: float
     level_start: np.ndarray
@@ -204,7 +204,7 @@ def direct_self(sources, tau, grad: bool = False, rows=None) -> np.ndarray:
 class FMM:
     """``FMM(method, sources; ppl, bbox)`` with the operators ``ops`` (reference width
     ``ops.width``) shared by every level through the homogeneous rescale
-    (src/fmm.jl:31-49, src/operators/cache.jl:224-237)."""
+    (src/fmm.jl:31-49, src/operators/cache.jl:33-46)."""
 
     def __init__(self, ops, points, ppl: int, origin, W0: float, max_levels: int = 30, perm=None):
         self.ops = ops
@@ -332,7 +332,7 @@ class FMM:
         ts = np.unique(tgt)
         pL[ts - lo] += alpha * u[slot[ts]]
 
-    # evaluate_at_sources! (src/evaluate.jl:425-441)
+    # evaluate_at_sources! (src/evaluate.jl:111-127)
     def evaluate_at_sources(self, Lc, tau_sorted, grad=False):
         t = self.tree
         N = len(t.perm)
@@ -343,7 +343,7 @@ class FMM:
                     int(grad), out)
         return out
 
-    # evaluate! (src/evaluate.jl:395-410)
+    # evaluate! (src/evaluate.jl:81-96)
     def evaluate(self, Lc, tau_sorted, targets, grad=False):
         t = self.tree
         x = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)

@@ -16,7 +16,7 @@
  *   flatten (+ group_parent)                  src/tree/builder.jl:392-436, 447-473
  *   _count_pairs! / _fill_pairs!              src/tree/lists.jl:73-124
  *   locate                                    src/tree/locate.jl:10-22
- *   child_slot / _center                      src/util.jl:39-45, src/tree/tree.jl:219-221
+ *   child_slot / _center                      src/util.jl:39-45, src/tree/tree.jl:81-89
  *
  * Box centres are formed exactly as the reference forms them
  * (origin + (cell + 0.5) * w, no contraction: build with -ffp-contract=off), so a
@@ -85,7 +85,7 @@ static void level_free(Level *L) {
     free(L->depth); free(L->leaf); free(L->ghost); free(L->xlist); free(L->col);
 }
 
-/* src/tree/tree.jl:219-221, src/util.jl:105 */
+/* src/tree/tree.jl:81-89, src/util.jl:105 */
 static void center_of(const double origin[3], const int32_t *cell, double w, double c[3]) {
     for (int d = 0; d < 3; d++) c[d] = origin[d] + ((double)cell[d] + 0.5) * w;
 }

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), f"{n} is declared in include/ffmm.h but not exported"
     assert set(names) == set(_lib.SIGNATURES), "the ctypes binding and the header disagree"
-    assert _lib.lib().ffmm_abi_version() == 2
+    assert _lib.lib().ffmm_abi_version() == 3
 
 
 def test_status_codes_and_messages_without_a_device():
@@ -63,7 +63,7 @@ def test_host_api_validation():
     o, w, e = c.fit_box([0.1, 0.1, 0.1], [0.9, 0.9, 0.9], (np.zeros(3), 1.0))
     assert w == 1.0 and e == 0 and c.W0 == 1.0
     o, w, e = c.fit_box([0.1, 0.1, 0.1], [0.3, 0.3, 0.3], (np.full(3, 0.1), 0.2))
-    assert w == 0.25 and e == -2                       # snapped to ldexp(W0, e) >= W (src/operators/cache.jl:254-274)
+    assert w == 0.25 and e == -2                       # snapped to ldexp(W0, e) >= W (src/operators/cache.jl:57-83)
     o, w, e = c.fit_box([0, 0, 0], [3, 3, 3], (np.zeros(3), 3.0))
     assert w == 4.0 and e == 2
     origin, W = ffmm.square_bbox([0, 0, 0], [1, 2, 0.5])

@@ -1,6 +1,6 @@
 """The device passes and evaluation against the oracle (the restated host FMM on the
 same tree and operators) and against the FP64 direct sum.  Tolerances follow the
-reference's device-vs-CPU checks (test/test_device_passes.jl:141-166: matvec 1e-11,
+reference's device-vs-CPU checks (test/test_device_passes.jl:6-31: matvec 1e-11,
 locals 100x that; vs direct 1e-6 at p = 8, test/test_3d.jl:155-163)."""
 import numpy as np
 import pytest
@@ -98,7 +98,7 @@ def test_expansions_vs_oracle(setup):
     what they represent is pinned: the multipoles through their potentials on the
     upward check surface (K_up M), the locals through theirs on the downward check
     surface (K_down L), the partial locals (check data) directly.  The reference's own
-    comparison, at p = 6, is test/test_device_passes.jl:158-161."""
+    comparison, at p = 6, is test/test_device_passes.jl:91-109 (and 6-31)."""
     from ffmm import operators as fo
     s = setup
     plan = s["plan"]
@@ -193,7 +193,7 @@ def test_determinism_sorted_inputs_linearity_nan_recovery(setup):
 
 def test_host_api_mirror(setup):
     """FMM / build_expansions / evaluate_at_sources / evaluate, as the reference's
-    host driver is called (src/fmm.jl:31-49, 300-314; src/evaluate.jl:412-446)."""
+    host driver is called (src/fmm.jl:31-49, 300-314; src/evaluate.jl:98-133)."""
     s = setup
     ffmm = s["ffmm"]
     f = ffmm.FMM(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"])

@@ -1,6 +1,6 @@
 """The device tree and its lists against the oracle's restatement of the host builder,
 field by field (src/tree/builder.jl, src/tree/lists.jl; the reference's own check is
-test/test_device_tree.jl:77-102 and test/test_device_passes.jl:246-291): every
+test/test_device_tree.jl:12-38 and test/test_device_passes.jl:111-147): every
 structural array bit-exact, the points of every leaf the same set (their order inside
 a leaf follows the full key on the device, the input order on the host)."""
 import numpy as np
