@@ -111,6 +111,9 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, pla
             geo[r["gid"]] = W.request_points(r)
         data.append((geo[r["gid"]], W.request_charges(r)))
     total = float(sum(r["n"] for r in reqs))
+    # the server's budget: 85% of the device memory free when the leg starts
+    torch.cuda.empty_cache()
+    budget = int(0.85 * torch.cuda.mem_get_info()[0])
     log(f"[bench] serving: {nreq} requests, {total:.3e} points, {len(geo)} geometries, {nslots} slots "
         f"(generated in {time.perf_counter() - t:.1f} s)")
 
@@ -118,7 +121,7 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, pla
         """The stream through a fresh server, arrivals at the stream's times x time_scale
         (0: all at once) replayed by a submitting thread; per request the latency and
         the submission lag, and the first served result of each (kernel, class)."""
-        srv = Server(nslots=nslots, hbm_budget=200 << 30)
+        srv = Server(nslots=nslots, hbm_budget=budget)
         for k, c in caches.items():      # warm the operator caches (host precompute, device copies)
             c.fit_box(np.zeros(3), np.ones(3), box)
             srv._prepare(c, np.zeros(3), np.ones(3), box)
@@ -158,7 +161,9 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, pla
                 "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
                 "submit_lag_ms": {"mean": 1e3 * float(lag.mean()), "max": 1e3 * float(lag.max())},
                 "plans_built": st["plans_built"], "plans_reused": st["plans_reused"],
-                "evictions": st["evictions"], "peak_device_bytes": st["peak_bytes"]}, first
+                "evictions": st["evictions"], "peak_device_bytes": st["peak_bytes"], "budget_bytes": budget,
+                "requeued_out_of_memory": st["requeued_out_of_memory"],
+                "worst_plan_over_estimate": st["worst_plan_over_estimate_permille"] / 1000}, first
 
     sustained, first = serve(1.0)
     log(f"[bench] serving at {rate}/s: " + json.dumps(sustained))
@@ -451,8 +456,12 @@ def main():
                 "eval_sources": {"flops": flops_eval, "ms": stages["eval_sources"],
                                  "achieved_tflops": flops_eval / (stages["eval_sources"] * 1e-3) / 1e12}}
 
+    tree_stats = {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
+                  "leaves": int(plan.level_leaves.sum())}
     serving = None
     if args.serve_requests > 0:
+        del plan                 # the measured plan's device memory goes back before the stream
+        torch.cuda.synchronize()
         serving = serving_bench(args.serve_requests, args.serve_nmin, args.serve_nmax, args.serve_rate,
                                 args.serve_slots, args.serve_plain_every)
         log("[bench] serving: " + json.dumps(serving))
@@ -477,8 +486,7 @@ def main():
                                    f"{args.precision}, M2L {args.m2l}",
                        "nsources": args.n, "ntargets": args.nt, "ppl": args.ppl, "p": 8},
             "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
-            "tree": {"levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
-                     "leaves": int(plan.level_leaves.sum())},
+            "tree": tree_stats,
             "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "plain_f64": plain,
             "stokes_surface": stokes}
     print(json.dumps(line), flush=True)

@@ -16,11 +16,45 @@ thread_local Context *g_alloc_ctx = nullptr;
 
 void fail(int code, const std::string &msg) { throw Error(code, msg); }
 
-static void flush_blocks(Context &c) {
+// every live context, so that an allocation that fails can take back the idle blocks
+// of all of them (a server's slots and the host's default context each keep their own)
+static std::mutex g_ctx_mu;
+static std::vector<Context *> g_contexts;
+
+// return idle blocks of a context to HIP, the largest first, until at most `keep` bytes
+// stay cached; the bytes returned
+static size_t trim_locked(BlockCache &bc, size_t keep) {
+  size_t freed = 0;
+  while (bc.bytes > keep && !bc.blocks.empty()) {
+    auto last = std::prev(bc.blocks.end());
+    (void)hipFree(last->second);
+    bc.bytes -= last->first;
+    freed += last->first;
+    bc.blocks.erase(last);
+  }
+  return freed;
+}
+
+size_t context_trim(Context &c, size_t keep) {
   std::lock_guard<std::mutex> lk(c.blocks->mu);
-  for (auto &kv : c.blocks->blocks) (void)hipFree(kv.second);
-  c.blocks->blocks.clear();
-  c.blocks->bytes = 0;
+  return trim_locked(*c.blocks, keep);
+}
+
+size_t context_idle_bytes(Context &c) {
+  std::lock_guard<std::mutex> lk(c.blocks->mu);
+  return c.blocks->bytes;
+}
+
+static void flush_blocks(Context &c) { context_trim(c, 0); }
+
+// the idle blocks of every context on `device` (hipFree waits for the device, so a block
+// that another context's stream freed in stream order is idle when it is returned)
+static size_t flush_device(int device) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  size_t freed = 0;
+  for (Context *c : g_contexts)
+    if (c->device == device) freed += context_trim(*c, 0);
+  return freed;
 }
 
 Context *device_alloc(void **p, size_t bytes, size_t *blk) {
@@ -39,10 +73,13 @@ Context *device_alloc(void **p, size_t bytes, size_t *blk) {
     }
   }
   hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess && c) {
-    // the cached blocks may be what is missing: give them back and try once more
+  if (e != hipSuccess) {
+    // the cached blocks may be what is missing: those of this context first, then
+    // those of every context on the device; then try once more
     (void)hipGetLastError();
-    flush_blocks(*c);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (!(c && context_trim(*c, 0) >= bytes)) flush_device(dev);
     e = hipMalloc(p, bytes);
   }
   if (e != hipSuccess) {
@@ -62,12 +99,7 @@ void device_free(void *p, size_t blk, Context *owner) {
     bc.blocks.emplace(blk, p);
     bc.bytes += blk;
     // keep at most the limit: return the largest idle blocks first
-    while (bc.bytes > bc.limit && !bc.blocks.empty()) {
-      auto last = std::prev(bc.blocks.end());
-      (void)hipFree(last->second);
-      bc.bytes -= last->first;
-      bc.blocks.erase(last);
-    }
+    trim_locked(bc, bc.limit);
   } else {
     (void)hipFree(p);
   }
@@ -106,9 +138,15 @@ void context_init(Context &c, int device, void *stream) {
   size_t fr = 0, tot = 0;
   FFMM_HIP(hipMemGetInfo(&fr, &tot));
   c.blocks->limit = tot / 2;
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  g_contexts.push_back(&c);
 }
 
 void context_release(Context &c) {
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), &c), g_contexts.end());
+  }
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   c.tmp.release();
@@ -871,6 +909,19 @@ void plan_release(Plan &p) {
   p.lev_ev[1].clear();
 }
 
+void plan_shrink(Plan &p) {
+  if (p.ctx) FFMM_HIP(hipStreamSynchronize(p.ctx->stream));
+  for (DBuf<double> *b : {&p.Mp, &p.Pl, &p.Lc, &p.tau, &p.pM, &p.tup, &p.Mup, &p.tM, &p.G, &p.tdn, &p.Gc, &p.Yc,
+                          &p.spec_S, &p.spec_U, &p.tgt_sorted})
+    b->release();
+  for (DBuf<float> *b : {&p.Lc32, &p.tau32, &p.Gc32, &p.Yc32, &p.pM32, &p.B32, &p.tM32, &p.spec_S32, &p.spec_U32})
+    b->release();
+  for (DBuf<int32_t> *b : {&p.tgt_node, &p.tgt_leafslot, &p.tgt_order, &p.tgt_ptr}) b->release();
+  p.tgt_keys.release();
+  p.expansions_ready = false;
+  p.shrunk = true;
+}
+
 void plan_build(Plan &p, const double *tau, int nd, int sorted) {
   OpCache &o = *p.ops;
   const bool stokes = o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET;
@@ -897,6 +948,11 @@ void plan_build(Plan &p, const double *tau, int nd, int sorted) {
   }
   p.nd = nd;
   p.expansions_ready = false;
+  if (p.shrunk) {
+    allocate_pass_buffers(p);
+    if (p.m2l == FFMM_M2L_SPECTRAL) spectral_workspace(p);
+    p.shrunk = false;
+  }
   {
     StageTimer tm(p, 7);
     gather_inputs(p, tau, sorted);

@@ -347,6 +347,7 @@ struct Plan {
   std::vector<char> lev_recorded[2];     // per level: both events recorded by the last pass
   bool stage_recorded[NSTAGES] = {};
   bool expansions_ready = false;
+  bool shrunk = false;       // the work buffers are released (plan_shrink)
 };
 
 // ---- primitives (prims.hip) ----
@@ -398,6 +399,8 @@ void upward_pass(Plan &plan);
 void downward_pass(Plan &plan);
 void m2l_dense_level(Plan &plan, int l);
 void spectral_prepare(Plan &plan);
+// the spectra of one level's sources and targets (the largest level's size)
+void spectral_workspace(Plan &plan);
 void m2l_spectral_level(Plan &plan, int l);
 
 // ---- direct sums (direct.hip) ----
@@ -508,9 +511,17 @@ void compute_centers(Plan &p);
 // ---- the life cycles (ffmm_abi.hip), shared by the C entry points and the server ----
 void context_init(Context &c, int device, void *stream);
 void context_release(Context &c);
+// return a context's idle cached blocks to HIP, the largest first, until at most `keep`
+// bytes stay cached (the bytes returned); the bytes it holds idle
+size_t context_trim(Context &c, size_t keep);
+size_t context_idle_bytes(Context &c);
 void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
                double width, int e0, int max_levels, int precision, int m2l);
 void plan_release(Plan &p);
+// a plan kept for later solves on the same points: the tree, lists and sorted points
+// stay; the expansions and work buffers are released (the next plan_build allocates
+// them again, from the block cache of the context it runs on)
+void plan_shrink(Plan &p);
 // the plan's tree as plan data (into buf when not null and cap suffices); its size in bytes
 size_t plan_save(Plan &p, void *buf, size_t cap);
 // a plan from plan data of a cache with the same operators: the tree uploaded, the rest rebuilt

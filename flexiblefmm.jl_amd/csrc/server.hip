@@ -87,6 +87,9 @@ struct Request {
   int64_t est = 0;
   double t_submit = 0, t_done = 0;
   bool done = false;
+  // out of device memory while other solves ran: queued again, admitted only when it
+  // would run alone (then its own footprint is all the device holds)
+  bool alone = false;
   int status = FFMM_OK;
   std::string error;
 };
@@ -123,6 +126,8 @@ struct ffmm_server_s {
   int64_t next_ticket = 1;
   uint64_t clock = 0;
   int64_t completed = 0, built = 0, reused = 0, evictions = 0;
+  int64_t worst_ratio = 0;   // largest (plan bytes after the solve) / (admission estimate), in thousandths
+  int64_t retried = 0;       // requests queued again after running out of device memory
   int running = 0;
   bool stop = false;
   // plan data in and out of the resident set: a context of its own (created at the first
@@ -158,18 +163,40 @@ static bool evict_one(ffmm_server_s &S, std::vector<std::unique_ptr<Resident>> &
   return true;
 }
 
-// under the lock: the next admissible request (or null), with its resident plan if any
+// the idle blocks the slots' contexts keep for reuse (device memory the server holds)
+static int64_t idle_bytes(ffmm_server_s &S) {
+  size_t b = 0;
+  for (auto &w : S.workers) b += context_idle_bytes(w->ctx);
+  return (int64_t)b;
+}
+
+// under the lock: the next admissible request (or null), with its resident plan if any.
+// The budget covers the estimates of the running solves and the resident plans (their
+// trees and lists; the work buffers go back to the slots); to make room the resident
+// plans are evicted, least recently used first.  The slots' idle cached blocks are not
+// counted: an allocation that fails returns every context's idle blocks and tries again.
 static Request *pick(ffmm_server_s &S, Resident *&res, std::vector<std::unique_ptr<Resident>> &graveyard) {
   const double t = now_ms();
   for (auto it = S.queue.begin(); it != S.queue.end(); ++it) {
     Request *q = *it;
+    if (q->alone && S.running > 0) return nullptr;   // it goes next, alone
     Resident *r = find_resident(S, q->cache, q->r.geometry);
     if (r) {
       if (r->busy) continue;             // another slot holds this geometry's plan
-      r->busy = true;
-      res = r;
-      S.queue.erase(it);
-      return q;
+      // its work buffers are allocated again: the request's estimate is reserved
+      r->busy = true;                    // (not evicted to make room for itself)
+      while (S.in_use + q->est > S.budget && evict_one(S, graveyard)) {
+      }
+      if (S.in_use + q->est <= S.budget || S.running == 0) {
+        S.in_use += q->est;
+        S.peak = std::max(S.peak, S.in_use);
+        res = r;
+        S.queue.erase(it);
+        return q;
+      }
+      r->busy = false;
+      if (t - q->t_submit > S.aging_ms) return nullptr;
+      continue;
     }
     if (q->r.geometry >= 0 &&
         std::find(S.building.begin(), S.building.end(), std::make_pair(q->cache, q->r.geometry)) != S.building.end())
@@ -268,19 +295,55 @@ static void worker_loop(ffmm_server_s *S, Worker *w) {
       err = e.what();
     }
     size_t real = own ? plan_bytes(*own) : 0;
+    const int64_t ratio = own && q->est > 0 ? (int64_t)(1000.0 * (double)real / (double)q->est) : 0;
     bool keep = own && status == FFMM_OK && q->r.geometry >= 0;
+    // a kept plan holds its tree and lists only; its work buffers go back to this slot
+    try {
+      if (keep) {
+        plan_shrink(*own);
+        real = plan_bytes(*own);
+      } else if (res) {
+        plan_shrink(*res->plan);
+      }
+    } catch (const std::exception &) {
+      keep = false;
+    }
     if (own && !keep) {
       plan_release(*own);
       own.reset();
     }
     lk.lock();
     S->running--;
+    S->worst_ratio = std::max(S->worst_ratio, ratio);
+    // out of memory next to other solves: give the reservation back and queue the
+    // request again at the front, to run alone
+    const bool retry = status == FFMM_ERR_MEMORY && !q->alone && S->running > 0;
+    if (retry) {
+      S->in_use -= q->est;
+      if (res) {
+        res->busy = false;
+      } else {
+        auto bi = std::find(S->building.begin(), S->building.end(), std::make_pair(q->cache, q->r.geometry));
+        if (bi != S->building.end()) S->building.erase(bi);
+      }
+      q->alone = true;
+      S->retried++;
+      S->queue.push_front(q);
+      S->cv_work.notify_all();
+      if (own) {
+        lk.unlock();
+        plan_release(*own);
+        own.reset();
+        lk.lock();
+      }
+      continue;
+    }
+    S->in_use -= q->est;
     if (res) {
       res->busy = false;
       res->last_use = ++S->clock;
       S->reused++;
     } else {
-      S->in_use -= q->est;
       if (status == FFMM_OK) S->built++;
       auto bi = std::find(S->building.begin(), S->building.end(), std::make_pair(q->cache, q->r.geometry));
       if (bi != S->building.end()) S->building.erase(bi);
@@ -391,9 +454,10 @@ int ffmm_server_stats(ffmm_server S, int64_t *stats, int n) {
   return guard([&] {
     if (!S || !stats) fail(FFMM_ERR_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(S->mu);
-    const int64_t v[8] = {S->completed, S->built,  S->reused, (int64_t)S->resident.size(),
-                          S->in_use,    S->peak,   S->evictions, (int64_t)S->queue.size()};
-    for (int i = 0; i < n && i < 8; i++) stats[i] = v[i];
+    const int64_t v[11] = {S->completed, S->built,     S->reused,     (int64_t)S->resident.size(),
+                           S->in_use,    S->peak,      S->evictions,  (int64_t)S->queue.size(),
+                           idle_bytes(*S), S->worst_ratio, S->retried};
+    for (int i = 0; i < n && i < 11; i++) stats[i] = v[i];
   });
 }
 
@@ -461,6 +525,12 @@ int ffmm_server_import(ffmm_server S, ffmm_opcache cache, int64_t geometry, cons
     Context &io = server_io(*S);
     try {
       plan_restore(*rr->plan, io, *c, buf, (size_t)size);
+    } catch (...) {
+      plan_release(*rr->plan);
+      throw;
+    }
+    try {
+      plan_shrink(*rr->plan);
     } catch (...) {
       plan_release(*rr->plan);
       throw;

@@ -517,9 +517,7 @@ __global__ void __launch_bounds__(IV_TB) k_spec_inverse(const typename cplx_t<T>
 }
 
 // host side -----------------------------------------------------------------
-void spectral_prepare(Plan &plan) {
-  OpCache &o = *plan.ops;
-  if (o.p != SP_P) fail(FFMM_ERR_ARGUMENT, "the spectral M2L is built for p = 8");
+void spectral_workspace(Plan &plan) {
   int64_t nmax = 1;
   for (int l = 2; l < plan.L; l++) nmax = std::max(nmax, plan.nl[l]);
   if (plan.precision == FFMM_PREC_MIXED) {
@@ -529,6 +527,12 @@ void spectral_prepare(Plan &plan) {
     plan.spec_S.resize((size_t)2 * SP_NF * nmax);
     plan.spec_U.resize((size_t)2 * SP_NF * nmax);
   }
+}
+
+void spectral_prepare(Plan &plan) {
+  OpCache &o = *plan.ops;
+  if (o.p != SP_P) fail(FFMM_ERR_ARGUMENT, "the spectral M2L is built for p = 8");
+  spectral_workspace(plan);
   Context &ctx = *plan.ctx;
   hipStream_t s = ctx.stream;
   if (plan.precision == FFMM_PREC_MIXED) {

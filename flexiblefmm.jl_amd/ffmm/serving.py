@@ -124,10 +124,10 @@ class Server:
         call("ffmm_server_import", self._h, oc, int(geometry), buf, len(data))
 
     def stats(self) -> dict:
-        v = (ctypes.c_int64 * 8)()
-        call("ffmm_server_stats", self._h, v, 8)
+        v = (ctypes.c_int64 * 11)()
+        call("ffmm_server_stats", self._h, v, 11)
         keys = ["completed", "plans_built", "plans_reused", "resident", "bytes_in_use", "peak_bytes", "evictions",
-                "queued"]
+                "queued", "idle_cached_bytes", "worst_plan_over_estimate_permille", "requeued_out_of_memory"]
         return dict(zip(keys, v[:]))
 
     def close(self):

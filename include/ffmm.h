@@ -313,7 +313,12 @@ int ffmm_server_submit(ffmm_server server, const ffmm_request *req, int64_t *tic
  * and its latency from submission to the last output copy. */
 int ffmm_server_wait(ffmm_server server, int64_t ticket, double *latency_ms);
 /* [0] completed, [1] plans built, [2] resident plans reused, [3] resident plans,
- * [4] device bytes held or reserved, [5] their peak, [6] evictions, [7] queued. */
+ * [4] device bytes held or reserved, [5] their peak, [6] evictions, [7] queued,
+ * [8] idle device bytes the slots keep for reuse (outside the budget: an allocation
+ *     that fails returns them),
+ * [9] the largest ratio of a built plan's bytes to its admission estimate, x 1000,
+ * [10] requests queued again after running out of device memory next to other solves
+ *      (they then run alone). */
 int ffmm_server_stats(ffmm_server server, int64_t *stats, int n);
 /* Plan data of the resident plan of (cache, geometry), as ffmm_plan_save writes it (buf
  * may be NULL to ask for the size).  No counterpart in the reference: with
