@@ -72,6 +72,7 @@ def lib():
         L.om2l_gather.argtypes = [f64p, ctypes.c_int, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p]
         L.om2l_scatter.argtypes = [f64p, ctypes.c_int, i64p, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p,
                                    ctypes.c_double]
+        L.ohadamard.argtypes = [i32p, i32p, i32p, i64p, ctypes.c_int64, f64p, f64p, ctypes.c_int64, f64p]
         tabs = [f64p] * 6
         L.oh_s2m.argtypes = [ctypes.c_int, ctypes.c_int, *tabs, i32p, ctypes.c_int64, f64p, i32p, f64p, f64p, i32p,
                              i32p, f64p, f64p, ctypes.c_int, f64p]
@@ -223,11 +224,13 @@ class FMM:
         L = t.nlevels
         self.lam = np.array([t.width(l) / ops.width for l in range(L)])
         self.centers = np.ascontiguousarray(t.centers())
-        childless = t.child_first < 0
-        self.s2m_nodes = [np.ascontiguousarray([i for i in t.level_range(l) if childless[i] and t.has_source[i]],
-                                               dtype=np.int32) for l in range(L)]
-        self.leaf_nodes = [np.ascontiguousarray([i for i in t.level_range(l) if t.leaf[i]], dtype=np.int32)
-                           for l in range(L)]
+        s2m = (t.child_first < 0) & (t.has_source != 0)
+        leaf = t.leaf != 0
+        self.s2m_nodes, self.leaf_nodes = [], []
+        for l in range(L):
+            r = t.level_range(l)
+            self.s2m_nodes.append(np.ascontiguousarray(r.start + np.nonzero(s2m[r.start:r.stop])[0], dtype=np.int32))
+            self.leaf_nodes.append(np.ascontiguousarray(r.start + np.nonzero(leaf[r.start:r.stop])[0], dtype=np.int32))
         self.surf = np.ascontiguousarray(ops.large)
 
     # upward_pass! (src/fmm.jl:170-205); expansions are stored node major (row per node)
@@ -311,26 +314,43 @@ class FMM:
             lib().om2l_scatter(pL, Ns, tc.copy(), far[a:b].copy(), runs, len(runs) - 1, idx, sgn, Y, alpha)
 
     def _m2l_spectral(self, pL, Mp, l, alpha):
-        """The level-wise spectral translation: transform every box, Hadamard-accumulate
-        over the pairs, transform back and extract."""
+        """The level-wise spectral translation (_spectral_m2l_level!,
+        src/operators/spectral_m2l.jl:328-451): transform every source box of the level's
+        pairs (threaded FFTs), accumulate the Hadamard products per target in C, transform
+        the targets back and extract their small-surface entries."""
+        import scipy.fft as sfft
         ops = self.ops
         n = 2 * ops.p
-        lo = self.tree.level_start[l]
-        tgt, src, far = self._pairs(l)
-        if len(tgt) == 0:
+        ptr, src, far = self.tree.lists[l]
+        if len(src) == 0:
             return
-        tgt = tgt + lo
-        nodes = np.unique(np.concatenate([src, tgt]))
-        slot = np.full(self.tree.nnodes, -1, np.int64)
-        slot[nodes] = np.arange(len(nodes))
-        lat = np.zeros((len(nodes), n ** 3))
-        lat[:, ops.embed] = Mp[nodes]
-        S = np.fft.rfftn(lat.reshape(-1, n, n, n), axes=(1, 2, 3))
-        U = np.zeros_like(S)
-        np.add.at(U, slot[tgt], ops.tables[far] * S[slot[src]])
-        u = np.fft.irfftn(U, s=(n, n, n), axes=(1, 2, 3)).reshape(len(nodes), -1)[:, ops.embed]
-        ts = np.unique(tgt)
-        pL[ts - lo] += alpha * u[slot[ts]]
+        workers = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        srcs = np.unique(src)
+        slot = np.full(self.tree.nnodes, -1, np.int32)
+        slot[srcs] = np.arange(len(srcs), dtype=np.int32)
+        # the multipoles sit in the [0, p)^3 corner of the (2p)^3 lattice: the transform
+        # pads the p^3 block with zeros
+        p = ops.p
+        e = np.asarray(ops.embed)
+        corner = (e // (n * n)) * p * p + ((e // n) % n) * p + e % n
+        lat = np.zeros((len(srcs), p ** 3))
+        lat[:, corner] = Mp[srcs]
+        S = sfft.rfftn(lat.reshape(-1, p, p, p), s=(n, n, n), axes=(1, 2, 3), workers=workers)
+        del lat
+        nf = S[0].size
+        S = np.ascontiguousarray(S.reshape(len(srcs), nf)).view(np.float64)
+        tables = getattr(ops, "_tables_flat", None)
+        if tables is None:
+            tables = np.ascontiguousarray(ops.tables.reshape(len(ops.tables), nf)).view(np.float64)
+            ops._tables_flat = tables
+        ts = np.nonzero(np.diff(ptr))[0].astype(np.int64)
+        U = np.empty((len(ts), 2 * nf))
+        lib().ohadamard(np.ascontiguousarray(ptr, np.int32), np.ascontiguousarray(slot[src]),
+                        np.ascontiguousarray(far, np.int32), ts, len(ts), tables, S, nf, U)
+        del S
+        U = U.view(np.complex128).reshape(len(ts), n, n, n // 2 + 1)
+        u = sfft.irfftn(U, s=(n, n, n), axes=(1, 2, 3), workers=workers).reshape(len(ts), -1)[:, ops.embed]
+        pL[ts] += alpha * u
 
     # evaluate_at_sources! (src/evaluate.jl:111-127)
     def evaluate_at_sources(self, Lc, tau_sorted, grad=False):
