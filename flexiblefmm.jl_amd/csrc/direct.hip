@@ -7,6 +7,9 @@
 #include "ffmm_internal.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
 
 namespace ffmm {
 
@@ -70,25 +73,95 @@ __global__ void k_direct_reduce(const double *__restrict__ part, int64_t nt, int
   }
 }
 
+// the source chunks of a direct sum over ns sources at nt targets: about 4096 workgroups
+// in all, chunks of at least 4096 sources; the number of chunks, *chunk their length
+static int64_t direct_chunks(int64_t ns, int64_t nt, int64_t *chunk) {
+  const unsigned gx = nblocks(nt, DT);
+  int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / gx, (ns + 4095) / 4096));
+  nchunk = std::min<int64_t>(nchunk, 65535);
+  *chunk = std::max<int64_t>(1, (ns + nchunk - 1) / nchunk);
+  return std::max<int64_t>(1, (ns + *chunk - 1) / *chunk);
+}
+
+// the two passes over a partial-sum buffer `part` of nchunk x nt x 4 doubles
+static void direct_run(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt,
+                       int64_t nt, const int64_t *skip, double *out, int output, int64_t nchunk, int64_t chunk,
+                       double *part) {
+  dim3 grid(nblocks(nt, DT), (unsigned)nchunk);
+  if (output == FFMM_POTENTIAL) k_direct<FFMM_POTENTIAL><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part);
+  else if (output == FFMM_GRADIENT) k_direct<FFMM_GRADIENT><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part);
+  else k_direct<2><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part);
+  FFMM_LAUNCH_CHECK();
+  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part, nt, (int)nchunk, output, out);
+  FFMM_LAUNCH_CHECK();
+}
+
 void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
             const int64_t *skip, double *out, int output) {
   if (nt <= 0) return;
-  const unsigned gx = nblocks(nt, DT);
-  // about 4096 workgroups in all, chunks of at least 4096 sources
-  int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / gx, (ns + 4095) / 4096));
-  nchunk = std::min<int64_t>(nchunk, 65535);
-  const int64_t chunk = std::max<int64_t>(1, (ns + nchunk - 1) / nchunk);
-  nchunk = std::max<int64_t>(1, (ns + chunk - 1) / chunk);
+  int64_t chunk = 0;
+  const int64_t nchunk = direct_chunks(ns, nt, &chunk);
   DBuf<double> part;
   part.resize((size_t)nchunk * nt * 4);
-  dim3 grid(gx, (unsigned)nchunk);
-  if (output == FFMM_POTENTIAL) k_direct<FFMM_POTENTIAL><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
-  else if (output == FFMM_GRADIENT) k_direct<FFMM_GRADIENT><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
-  else k_direct<2><<<grid, DT, 0, ctx.stream>>>(src, tau, ns, tgt, nt, skip, chunk, part.p);
-  FFMM_LAUNCH_CHECK();
-  k_direct_reduce<<<nblocks(nt, DT), DT, 0, ctx.stream>>>(part.p, nt, (int)nchunk, output, out);
-  FFMM_LAUNCH_CHECK();
+  direct_run(ctx, src, tau, ns, tgt, nt, skip, out, output, nchunk, chunk, part.p);
   ctx.sync();   // the partial sums are freed on return
+}
+
+// The partial sums of the allocation-free direct sum (direct_noalloc): one array per
+// device, 2^22 doubles (32 MiB), held until the process ends.  Host code never passes
+// the array's name to a kernel: each device's address of it is looked up once, when a
+// context is set up on that device (direct_scratch_setup), and kept.
+constexpr int64_t DIRECT_SCRATCH_DOUBLES = (int64_t)1 << 22;
+static __device__ __align__(256) double scratch[DIRECT_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+struct ScratchAddr {
+  int device;
+  double *ptr;
+  std::string error;   // why the lookup failed (ptr null)
+};
+std::vector<ScratchAddr> g_scratch;
+}  // namespace
+
+void direct_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  for (const auto &a : g_scratch)
+    if (a.device == device) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    g_scratch.push_back({device, nullptr, std::string("hipGetSymbolAddress of the direct-sum scratch array: ") +
+                                              hipGetErrorString(e)});
+    return;
+  }
+  g_scratch.push_back({device, (double *)p, std::string()});
+}
+
+static double *direct_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  for (const auto &a : g_scratch)
+    if (a.device == device) {
+      if (!a.ptr) fail(FFMM_ERR_DEVICE, a.error);
+      return a.ptr;
+    }
+  fail(FFMM_ERR_STATE, "no context was set up on device " + std::to_string(device));
+  return nullptr;
+}
+
+void direct_noalloc(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
+                    const int64_t *skip, double *out, int output) {
+  if (nt <= 0) return;
+  int64_t chunk = 0;
+  const int64_t nchunk = direct_chunks(ns, nt, &chunk);
+  if (nchunk * nt * 4 > DIRECT_SCRATCH_DOUBLES)
+    fail(FFMM_ERR_ARGUMENT, "ffmm_direct_noalloc takes at most " + std::to_string(DIRECT_SCRATCH_DOUBLES / 4) +
+                                " targets (its partial sums live in a fixed 32 MiB device array); got " +
+                                std::to_string(nt));
+  // every entry of the used part of the array is written by the first pass before the
+  // second reads it: nothing to clear
+  direct_run(ctx, src, tau, ns, tgt, nt, skip, out, output, nchunk, chunk, direct_scratch(ctx.device));
 }
 
 // The Stokes sums: velocity of stokeslets f (nd = 3) and, with nd = 9, stresslets of

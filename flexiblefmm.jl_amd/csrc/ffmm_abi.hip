@@ -138,6 +138,8 @@ void context_init(Context &c, int device, void *stream) {
   size_t fr = 0, tot = 0;
   FFMM_HIP(hipMemGetInfo(&fr, &tot));
   c.blocks->limit = tot / 2;
+  // the device address of the allocation-free direct sum's array on this device
+  direct_scratch_setup(device);
   std::lock_guard<std::mutex> lk(g_ctx_mu);
   g_contexts.push_back(&c);
 }
@@ -250,6 +252,20 @@ int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t 
     if (!tgt || !out || (ns > 0 && (!src || !tau))) fail(FFMM_ERR_ARGUMENT, "null argument");
     set_device(ctx->c);
     direct(ctx->c, src, tau, ns, tgt, nt, skip, out, output);
+    ctx->c.sync();
+  });
+}
+
+int ffmm_direct_noalloc(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt,
+                        int64_t nt, const int64_t *skip, double *out, int output) {
+  return guard([&] {
+    if (!ctx) fail(FFMM_ERR_ARGUMENT, "null context");
+    if (ns < 0 || nt < 0) fail(FFMM_ERR_DIMENSION, "negative sizes");
+    if (output < 0 || output > 2) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    if (nt == 0) return;
+    if (!tgt || !out || (ns > 0 && (!src || !tau))) fail(FFMM_ERR_ARGUMENT, "null argument");
+    set_device(ctx->c);
+    direct_noalloc(ctx->c, src, tau, ns, tgt, nt, skip, out, output);
     ctx->c.sync();
   });
 }

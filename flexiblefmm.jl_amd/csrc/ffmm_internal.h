@@ -406,6 +406,12 @@ void m2l_spectral_level(Plan &plan, int l);
 // ---- direct sums (direct.hip) ----
 void direct(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
             const int64_t *skip, double *out, int output);
+// direct() with its partial sums in the library's fixed device array (no allocation;
+// at most 2^20 targets); calls that overlap on one device share the array and race
+void direct_noalloc(Context &ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
+                    const int64_t *skip, double *out, int output);
+// the device address of that array on `device`, looked up once (at context set-up)
+void direct_scratch_setup(int device);
 // output: FFMM_POTENTIAL (the velocity, 3 x nt) or FFMM_PRESSURE (nt)
 void direct_stokes(Context &ctx, const double *src, const double *dens, int nd, int64_t ns, const double *tgt,
                    int64_t nt, const int64_t *skip, double *out, int output = FFMM_POTENTIAL);

@@ -43,6 +43,7 @@ SIGNATURES = {
     "ffmm_context_destroy": (_i32, [_vp]),
     "ffmm_bounds": (_i32, [_vp, _vp, _i64, _vp, _vp, ctypes.POINTER(_i32)]),
     "ffmm_direct": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i32]),
+    "ffmm_direct_noalloc": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i32]),
     "ffmm_direct_stokes": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "ffmm_direct_stokes_pressure": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "ffmm_opcache_create": (_i32, [_vp, _i32, _i32, _d, ctypes.POINTER(_vp)]),

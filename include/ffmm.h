@@ -74,6 +74,16 @@ int ffmm_bounds(ffmm_context ctx, const double *xyz, int64_t n, double lo[3], do
 int ffmm_direct(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt, int64_t nt,
                 const int64_t *skip, double *out, int output);
 
+/* ffmm_direct without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  The chunks' partial sums live in a 32 MiB device array
+ * of the library (one per device, held until the process ends), so nt is at most
+ * 1,048,576 (larger nt: FFMM_ERR_ARGUMENT, nothing launched); results equal
+ * ffmm_direct's bitwise.  Calls that overlap on one device (several host threads or
+ * streams) share the array and race: serialise them. */
+int ffmm_direct_noalloc(ffmm_context ctx, const double *src, const double *tau, int64_t ns, const double *tgt,
+                        int64_t nt, const int64_t *skip, double *out, int output);
+
 /* FP64 direct Stokes velocity: direct! for Stokeslet3D (src/kernels.jl:463-487) with the
  * stresslet of src/operators/harmonics_stokes.jl:197-217.  dens: nd x ns, point major,
  * nd = 3 (stokeslets f) or 9 (rows [f, g, nu]: stokeslet f, stresslet strength g and
