@@ -61,23 +61,73 @@ def traffic_from_profile(kname: str, args):
     return prof["traffic_bytes_per_launch"] if kname.startswith(prof["kernel"]) else None
 
 
-def cpu_baseline(n_sample: int, ppl: int):
-    """The oracle's restated reference CPU FMM on a Gaussian-mixture sample of the same
-    generator: tree + passes + potential at the sources, points per second."""
+PUBLISHED_CPU = {"value": 0.69e6, "unit": "points/s",
+                 "case": "Laplace CubeKI p=8, 1e5 uniform points, ppl=200, up 0.025 + down 0.086 + self 0.033 s, "
+                         "FP64, 12 threads on an Apple M3 Max", "source": "docs/harmonics.md:79-85"}
+
+
+def _cpu_solve(O, ops, pts, tau, ppl):
+    """One solve of the restated reference CPU FMM (spectral M2L, FP64): tree, expansions
+    (upward and downward passes), potential at the sources; the three times and the
+    output in the original order."""
+    t0 = time.perf_counter()
+    f = O.FMM(ops, pts, ppl, np.zeros(3), 1.0)
+    t1 = time.perf_counter()
+    ts = np.ascontiguousarray(tau[f.tree.perm])
+    Mp = f.upward(ts)
+    _, Lc = f.downward(Mp, "spectral")
+    t2 = time.perf_counter()
+    u = f.evaluate_at_sources(Lc, ts)
+    t3 = time.perf_counter()
+    return (t1 - t0, t2 - t1, t3 - t2), u
+
+
+def _best_of(O, ops, pts, tau, ppl, runs):
+    """One warm-up solve, then the best (by total) of `runs` solves."""
+    _cpu_solve(O, ops, pts, tau, ppl)
+    best, u = None, None
+    for _ in range(runs):
+        t, u = _cpu_solve(O, ops, pts, tau, ppl)
+        if best is None or sum(t) < sum(best):
+            best = t
+    return best, u
+
+
+def cpu_baseline(n_sample: int, ppl: int, runs: int, gpu_solve):
+    """The oracle's restatement of the reference CPU FMM (C/OpenMP per-point loops and
+    spectral Hadamard, threaded FFTs and BLAS; FP64, spectral M2L as
+    src/operators/spectral_m2l.jl:328-451) on the host cores: the best of `runs` warm
+    solves on `n_sample` Gaussian-mixture sources of the same generator, split as
+    tree / expansions / evaluation (bench/bench.jl:44-56), and on the reference's
+    published case (1e5 uniform points, ppl = 200) beside the published rate.  The
+    device result on the same sample (the measured form, `gpu_solve`) is compared with
+    the restated FMM's output: the error against the reference FMM's own output."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from ffmm import operators, workloads as W
+    ops = operators.level_operators(1.0, p=8, with_tables=True)
     pts = W.gaussian_mixture(n_sample, seed=0)
     tau = W.charges(n_sample, seed=1)
-    ops = operators.level_operators(1.0, p=8, with_tables=False)
-    t0 = time.perf_counter()
-    f = O.FMM(ops, pts, ppl, np.zeros(3), 1.0)
-    f.matvec(tau)
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n_sample / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"{n_sample} Gaussian-mixture sources (same generator), ppl={ppl}, p=8, tree + passes + "
-                      f"potential at the sources, FP64, {dt:.1f} s"}
+    (tt, te, tv), u_cpu = _best_of(O, ops, pts, tau, ppl, runs)
+    total = tt + te + tv
+    # the reference's published configuration, same restatement
+    pu = W.uniform(100_000, seed=3)
+    tu = W.charges(100_000, seed=4)
+    (ut, ue, uv), _ = _best_of(O, ops, pu, tu, 200, runs)
+    u_gpu = gpu_solve(pts, tau)
+    rel = float(np.linalg.norm(u_gpu - u_cpu) / np.linalg.norm(u_cpu))
+    rows = np.sort(np.random.default_rng(11).choice(n_sample, 2000, replace=False))
+    ud = O.direct_self(pts, tau, rows=rows)
+    cpu_err = float(np.linalg.norm(u_cpu[rows] - ud) / np.linalg.norm(ud))
+    return {"value": n_sample / total, "unit": "points/s", "cores": O.lib().othreads(), "kind": "port",
+            "sample": f"{n_sample} Gaussian-mixture sources (the same generator), ppl={ppl}, p=8, FP64 spectral "
+                      f"M2L, tree + expansions + potential at the sources, best of {runs} warm runs",
+            "split_s": {"tree": tt, "expansions": te, "evaluation": tv, "total": total},
+            "rel_l2_vs_direct": cpu_err,
+            "published_case": {"points_per_s": 1e5 / (ut + ue + uv), "split_s": {"tree": ut, "expansions": ue,
+                                                                                 "evaluation": uv},
+                               "published": PUBLISHED_CPU},
+            "gpu_rel_l2_vs_restated_cpu_fmm": rel}
 
 
 def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, plain_every: int):
@@ -290,14 +340,16 @@ def main():
     ap.add_argument("--ppl", type=int, default=64)
     ap.add_argument("--precision", default="mixed", choices=["f64", "mixed"])
     ap.add_argument("--m2l", default="spectral", choices=["dense", "spectral"])
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 steps")
     ap.add_argument("--nerr", type=int, default=10_000)
     ap.add_argument("--serve-requests", type=int, default=2000, help="served-stream requests (0: skip the leg)")
     ap.add_argument("--serve-nmin", type=int, default=10_000)
     ap.add_argument("--serve-nmax", type=int, default=2_000_000)
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=50.0,
+                    help="Poisson arrivals per second of the latency pass (about half the saturated rate)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-plain-every", type=int, default=8,
                     help="the plain-form sequential leg runs every k-th request")
@@ -472,9 +524,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle on {args.cpu_sample} points")
-        cpu = cpu_baseline(args.cpu_sample, args.ppl)
-        log(f"[bench] cpu baseline: {cpu['value']:.3e} points/s")
+        log(f"[bench] cpu baseline: restated CPU FMM on {args.cpu_sample} points, best of {args.cpu_runs}")
+
+        def gpu_solve(p_, t_):
+            pl = ffmm.DevicePlan(m, p_, ppl=args.ppl, bbox=box, cache=cache, precision=args.precision, m2l=args.m2l)
+            return pl.matvec(t_).cpu().numpy()
+        cpu = cpu_baseline(args.cpu_sample, args.ppl, args.cpu_runs, gpu_solve)
+        log("[bench] cpu baseline: " + json.dumps(cpu))
+        err["rel_l2_vs_restated_cpu_fmm"] = cpu["gpu_rel_l2_vs_restated_cpu_fmm"]
+        err["restated_cpu_fmm_sample"] = args.cpu_sample
 
     line = {"metric": "Laplace3D 4M-pt FMM points/s at 1e-6 rel L2; served p50/p99 solve latency",
             "value": args.n / (ms * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,

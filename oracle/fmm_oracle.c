@@ -221,3 +221,7 @@ void ohadamard(const int32_t *ptr, const int32_t *src_slot, const int32_t *far, 
         }
     }
 }
+
+/* the threads the OpenMP loops above run on */
+#include <omp.h>
+int othreads(void) { return omp_get_max_threads(); }

@@ -73,6 +73,7 @@ def lib():
         L.om2l_scatter.argtypes = [f64p, ctypes.c_int, i64p, i64p, i64p, ctypes.c_int64, i32p, i8p, f64p,
                                    ctypes.c_double]
         L.ohadamard.argtypes = [i32p, i32p, i32p, i64p, ctypes.c_int64, f64p, f64p, ctypes.c_int64, f64p]
+        L.othreads.restype = ctypes.c_int
         tabs = [f64p] * 6
         L.oh_s2m.argtypes = [ctypes.c_int, ctypes.c_int, *tabs, i32p, ctypes.c_int64, f64p, i32p, f64p, f64p, i32p,
                              i32p, f64p, f64p, ctypes.c_int, f64p]
