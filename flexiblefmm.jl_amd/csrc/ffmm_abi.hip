@@ -176,9 +176,11 @@ static hipEvent_t &stage_event(Plan &p, int stage, int level, int which) {
   return v[2 * level + which];
 }
 StageTimer::StageTimer(Plan &p, int s, int l) : plan(p), stage(s), level(l) {
+  if (plan.capturing) return;
   FFMM_HIP(hipEventRecord(stage_event(plan, stage, level, 0), plan.ctx->stream));
 }
 StageTimer::~StageTimer() {
+  if (plan.capturing) return;
   if (hipEventRecord(stage_event(plan, stage, level, 1), plan.ctx->stream) != hipSuccess) {
     (void)hipGetLastError();   // a failed record only loses the time: clear the error state
     return;
@@ -617,6 +619,40 @@ int ffmm_opcache_put_harmonic_level(ffmm_opcache cache, double width, const doub
   });
 }
 
+int ffmm_opcache_put_class_factors(ffmm_opcache cache, double tol, const int32_t *ranks, const double *A,
+                                   const double *Bt) {
+  return guard([&] {
+    if (!cache || !ranks || !A || !Bt) fail(FFMM_ERR_ARGUMENT, "null argument");
+    OpCache &o = cache->c;
+    if (!o.ready || o.method != METHOD_CUBE)
+      fail(FFMM_ERR_ARGUMENT, "low-rank class operators belong to the cube method's operators");
+    if (!(tol > 0 && tol < 1)) fail(FFMM_ERR_ARGUMENT, "tol must lie in (0, 1)");
+    auto f = std::make_unique<ClassFactors>();
+    f->tol = tol;
+    int64_t tot = 0;
+    for (int c = 0; c < o.nclass; c++) {
+      if (ranks[c] < 1 || ranks[c] > o.Ns) fail(FFMM_ERR_ARGUMENT, "a class rank lies outside 1:Ns");
+      f->rank.push_back(ranks[c]);
+      f->off.push_back(tot);
+      f->kmax = std::max(f->kmax, ranks[c]);
+      tot += (int64_t)ranks[c] * o.Ns;
+    }
+    Context &ctx = *o.ctx;
+    set_device(ctx);
+    hipStream_t s = ctx.stream;
+    f->A.upload(A, tot, s);
+    f->Bt.upload(Bt, tot, s);
+    std::vector<float> a32(A, A + tot), b32(Bt, Bt + tot);
+    f->A32.upload(a32.data(), tot, s);
+    f->Bt32.upload(b32.data(), tot, s);
+    ctx.sync();
+    std::lock_guard<std::mutex> lk(o.factors_mu);
+    for (auto &g : o.factors)
+      if (g->tol == tol) return;   // the factors of a tolerance are fixed once put
+    o.factors.push_back(std::move(f));
+  });
+}
+
 int ffmm_opcache_has_width(ffmm_opcache cache, double width, int *has) {
   return guard([&] {
     if (!cache || !has) fail(FFMM_ERR_ARGUMENT, "null argument");
@@ -915,6 +951,7 @@ void plan_release(Plan &p) {
   if (!p.ctx) return;
   (void)hipSetDevice(p.ctx->device);
   (void)hipStreamSynchronize(p.ctx->stream);
+  plan_drop_graph(p);
   for (auto &e : p.ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &v : p.lev_ev)
@@ -925,8 +962,88 @@ void plan_release(Plan &p) {
   p.lev_ev[1].clear();
 }
 
+void plan_drop_graph(Plan &p) {
+  if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  p.gexec = nullptr;
+  p.graph = nullptr;
+  p.gin.release();
+  p.gout.release();
+  p.g_nd = 0;
+  p.g_output = -1;
+  p.g_ncomp = 0;
+}
+
+int plan_ncomp(const Plan &p, int output) {
+  const OpCache &o = *p.ops;
+  if (o.method == METHOD_HARMONIC && o.kernel == FFMM_KERNEL_STOKESLET) return output == FFMM_PRESSURE ? 1 : 3;
+  return output == FFMM_POTENTIAL ? 1 : output == FFMM_GRADIENT ? 3 : 4;
+}
+
+void plan_capture(Plan &p, int nd, int output) {
+  set_device(*p.ctx);
+  if (p.ctx->sync(), p.gexec) plan_drop_graph(p);
+  AllocScope scope(*p.ctx);
+  const int nc = plan_ncomp(p, output);
+  p.gin.resize((size_t)nd * p.N);
+  p.gout.resize((size_t)nc * p.N);
+  fill_f64(*p.ctx, p.gin.p, (int64_t)nd * p.N, 0.0);
+  // one eager pass: its checks, the buffers it allocates and the libraries' workspaces
+  plan_build(p, p.gin.p, nd, 0);
+  plan_eval_sources(p, p.gout.p, output);
+  hipStream_t s = p.ctx->stream;
+  hipGraph_t g = nullptr;
+  p.capturing = true;
+  FFMM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    gather_inputs(p, p.gin.p, 0);
+    upward_pass(p);
+    downward_pass(p);
+    evaluate_at_sources(p, p.gout.p, output);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    p.capturing = false;
+    throw;
+  }
+  p.capturing = false;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    fail(FFMM_ERR_DEVICE, std::string("the pass sequence could not be captured: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipGraphDestroy(g);
+    fail(FFMM_ERR_DEVICE, std::string("the captured pass sequence could not be instantiated: ") +
+                              hipGetErrorString(ei));
+  }
+  p.graph = g;
+  p.gexec = x;
+  p.g_nd = nd;
+  p.g_output = output;
+  p.g_ncomp = nc;
+}
+
+void plan_replay(Plan &p, const double *dens, double *out) {
+  if (!p.gexec) fail(FFMM_ERR_STATE, "the plan holds no captured pass sequence");
+  set_device(*p.ctx);
+  hipStream_t s = p.ctx->stream;
+  FFMM_HIP(hipMemcpyAsync(p.gin.p, dens, sizeof(double) * p.g_nd * p.N, hipMemcpyDefault, s));
+  FFMM_HIP(hipGraphLaunch(p.gexec, s));
+  FFMM_HIP(hipMemcpyAsync(out, p.gout.p, sizeof(double) * p.g_ncomp * p.N, hipMemcpyDefault, s));
+  p.ctx->sync();
+  p.nd = p.g_nd;
+  p.expansions_ready = true;
+}
+
 void plan_shrink(Plan &p) {
   if (p.ctx) FFMM_HIP(hipStreamSynchronize(p.ctx->stream));
+  plan_drop_graph(p);
   for (DBuf<double> *b : {&p.Mp, &p.Pl, &p.Lc, &p.tau, &p.pM, &p.tup, &p.Mup, &p.tM, &p.G, &p.tdn, &p.Gc, &p.Yc,
                           &p.spec_S, &p.spec_U, &p.tgt_sorted})
     b->release();
@@ -934,6 +1051,8 @@ void plan_shrink(Plan &p) {
     b->release();
   for (DBuf<int32_t> *b : {&p.tgt_node, &p.tgt_leafslot, &p.tgt_order, &p.tgt_ptr}) b->release();
   p.tgt_keys.release();
+  p.Zc.release();
+  p.Zc32.release();
   p.expansions_ready = false;
   p.shrunk = true;
 }
@@ -1280,6 +1399,48 @@ int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *pl
     if (plocals) p.Pl.download(plocals, (size_t)p.ops->Ns * p.nnodes, s);
     if (leaf_locals) p.Lc.download(leaf_locals, (size_t)p.ops->Nl * p.leaf_off[p.L], s);
     p.ctx->sync();
+  });
+}
+
+int ffmm_plan_capture(ffmm_plan plan, int nd, int output) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    if (output < 0 || output > 3) fail(FFMM_ERR_ARGUMENT, "unknown output");
+    plan_capture(plan->c, nd, output);
+  });
+}
+
+int ffmm_plan_replay(ffmm_plan plan, const double *dens, double *out) {
+  return guard([&] {
+    if (!plan || !dens || !out) fail(FFMM_ERR_ARGUMENT, "null argument");
+    plan_replay(plan->c, dens, out);
+  });
+}
+
+int ffmm_plan_set_tol(ffmm_plan plan, double tol) {
+  return guard([&] {
+    if (!plan) fail(FFMM_ERR_ARGUMENT, "null plan");
+    Plan &p = plan->c;
+    OpCache &o = *p.ops;
+    if (!(tol >= 0 && tol < 1)) fail(FFMM_ERR_ARGUMENT, "tol must lie in [0, 1)");
+    const ClassFactors *lr = nullptr;
+    if (tol > 0) {
+      if (o.method != METHOD_CUBE || p.m2l != FFMM_M2L_DENSE)
+        fail(FFMM_ERR_ARGUMENT, "tol > 0 takes the cube method with the dense class M2L");
+      std::lock_guard<std::mutex> lk(o.factors_mu);
+      for (auto &g : o.factors)
+        if (g->tol == tol) lr = g.get();
+      if (!lr) fail(FFMM_ERR_STATE, "the cache holds no low-rank class operators for this tol");
+    }
+    set_device(*p.ctx);
+    p.ctx->sync();
+    plan_drop_graph(p);   // the buffers may move
+    AllocScope scope(*p.ctx);
+    p.lowrank = lr;
+    p.Zc.release();
+    p.Zc32.release();
+    p.expansions_ready = false;
+    if (!p.shrunk) allocate_pass_buffers(p);
   });
 }
 

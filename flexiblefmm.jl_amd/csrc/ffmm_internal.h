@@ -222,6 +222,8 @@ struct HTab {
   const double *a, *b, *cz, *cm, *cp, *e;
 };
 
+struct ClassFactors;
+
 struct OpCache {
   Context *ctx = nullptr;
   int method = METHOD_CUBE;
@@ -274,6 +276,22 @@ struct OpCache {
                                              // row 216: all near (read past the last candidate)
   std::vector<int32_t> embed_h;
   std::vector<int32_t> far_class_h;
+  // low-rank factors of the class representatives, one set per tolerance (added, never
+  // replaced or removed while the cache lives: plans hold pointers to them)
+  std::vector<std::unique_ptr<ClassFactors>> factors;
+  std::mutex factors_mu;
+};
+
+// the class representatives of the cube method as low-rank products A_c Bt_c (A_c: Ns x k_c,
+// Bt_c: k_c x Ns, column major, concatenated over the classes): class_factors,
+// src/operators/symmetry.jl:169-178 (singular values above tol times the largest)
+struct ClassFactors {
+  double tol = 0.0;
+  std::vector<int> rank;        // k_c per class
+  std::vector<int64_t> off;     // first entry of A_c (and of Bt_c) in A, Bt
+  int kmax = 0;
+  DBuf<double> A, Bt;
+  DBuf<float> A32, Bt32;
 };
 
 // per-level lists of far pairs (targets in node order) and their class-major form
@@ -348,6 +366,19 @@ struct Plan {
   bool stage_recorded[NSTAGES] = {};
   bool expansions_ready = false;
   bool shrunk = false;       // the work buffers are released (plan_shrink)
+  // the dense class M2L with low-rank class operators (null: the dense representatives),
+  // and the products Bt_c G of one class of a chunk
+  const ClassFactors *lowrank = nullptr;
+  DBuf<double> Zc;
+  DBuf<float> Zc32;
+  // a captured pass sequence (plan_capture): the densities' gather from gin (input
+  // order), the upward and downward passes and the evaluation at the sources into gout,
+  // as one HIP graph; every buffer it touches stays where it is while it lives
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  DBuf<double> gin, gout;
+  int g_nd = 0, g_output = -1, g_ncomp = 0;
+  bool capturing = false;    // stage timers record no events into a capture
 };
 
 // ---- primitives (prims.hip) ----
@@ -394,6 +425,8 @@ inline int64_t m2l_chunk_pairs(const Plan &p) {
 
 // ---- passes (passes.hip) ----
 void allocate_pass_buffers(Plan &plan);
+// a[0 .. n) = v on the context's stream (a kernel: no memset node in a captured graph)
+void fill_f64(Context &ctx, double *a, int64_t n, double v);
 void gather_inputs(Plan &plan, const double *tau, int sorted);
 void upward_pass(Plan &plan);
 void downward_pass(Plan &plan);
@@ -528,6 +561,14 @@ void plan_release(Plan &p);
 // stay; the expansions and work buffers are released (the next plan_build allocates
 // them again, from the block cache of the context it runs on)
 void plan_shrink(Plan &p);
+// capture the pass sequence of nd densities per point and `output` at the sources (one
+// eager pass with zero densities first); replay it: dens (device, nd x N, input order)
+// to out (device, ncomp x N); drop it (the buffers may move again)
+void plan_capture(Plan &p, int nd, int output);
+void plan_replay(Plan &p, const double *dens, double *out);
+void plan_drop_graph(Plan &p);
+// the output components per point of a plan's method (ffmm_evaluate_at_sources)
+int plan_ncomp(const Plan &p, int output);
 // the plan's tree as plan data (into buf when not null and cap suffices); its size in bytes
 size_t plan_save(Plan &p, void *buf, size_t cap);
 // a plan from plan data of a cache with the same operators: the tree uploaded, the rest rebuilt

@@ -208,6 +208,12 @@ __global__ void k_fill(double *__restrict__ a, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
+
+void fill_f64(Context &ctx, double *a, int64_t n, double v) {
+  if (n <= 0) return;
+  k_fill<<<nblocks(n, TB), TB, 0, ctx.stream>>>(a, n, v);
+  FFMM_LAUNCH_CHECK();
+}
 // the densities into the tree's order: _gather_perm! (src/device_lists.jl:163-171)
 __global__ void k_gather_perm(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ perm,
                               int64_t n) {
@@ -629,6 +635,12 @@ void allocate_pass_buffers(Plan &plan) {
     plan.Gc.resize((size_t)Ns * pmax);
     plan.Yc.resize((size_t)Ns * pmax);
   }
+  if (plan.lowrank && plan.m2l == FFMM_M2L_DENSE) {   // Bt_c G of the pairs of one class of a chunk
+    if (plan.precision == FFMM_PREC_MIXED)
+      plan.Zc32.resize((size_t)plan.lowrank->kmax * pmax);
+    else
+      plan.Zc.resize((size_t)plan.lowrank->kmax * pmax);
+  }
 }
 
 void gather_inputs(Plan &plan, const double *tau, int sorted) {
@@ -765,7 +777,13 @@ void m2l_dense_level(Plan &plan, int l) {
         const int32_t n = ca.co[c + 1] - ca.co[c];
         if (n == 0) continue;
         const size_t off = (size_t)ca.co[c] * nbN;
-        gemm32(ctx, N, nb * n, N, reps32 + (size_t)c * N * N, N, plan.Gc32.p + off, N, plan.Yc32.p + off, N);
+        if (const ClassFactors *lr = plan.lowrank) {   // A_c (Bt_c G): src/device_passes.jl:570-576
+          const int kc = lr->rank[c];
+          gemm32(ctx, kc, nb * n, N, lr->Bt32.p + lr->off[c], kc, plan.Gc32.p + off, N, plan.Zc32.p, kc);
+          gemm32(ctx, N, nb * n, kc, lr->A32.p + lr->off[c], N, plan.Zc32.p, kc, plan.Yc32.p + off, N);
+        } else {
+          gemm32(ctx, N, nb * n, N, reps32 + (size_t)c * N * N, N, plan.Gc32.p + off, N, plan.Yc32.p + off, N);
+        }
       }
       k_chunk_scatter<float><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
                                                          li.cfar.p, o.ssel.p, alpha, ca);
@@ -777,7 +795,13 @@ void m2l_dense_level(Plan &plan, int l) {
         const int32_t n = ca.co[c + 1] - ca.co[c];
         if (n == 0) continue;
         const size_t off = (size_t)ca.co[c] * nbN;
-        gemm(ctx, N, nb * n, N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p + off, N, 0.0, plan.Yc.p + off, N);
+        if (const ClassFactors *lr = plan.lowrank) {   // A_c (Bt_c G): src/device_passes.jl:570-576
+          const int kc = lr->rank[c];
+          gemm(ctx, kc, nb * n, N, 1.0, lr->Bt.p + lr->off[c], kc, plan.Gc.p + off, N, 0.0, plan.Zc.p, kc);
+          gemm(ctx, N, nb * n, kc, 1.0, lr->A.p + lr->off[c], N, plan.Zc.p, kc, 0.0, plan.Yc.p + off, N);
+        } else {
+          gemm(ctx, N, nb * n, N, 1.0, reps + (size_t)c * N * N, N, plan.Gc.p + off, N, 0.0, plan.Yc.p + off, N);
+        }
       }
       k_chunk_scatter<double><<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc.p, Ns, N, nb, li.cstart.p, plan.nl[l], t0,
                                                           li.cfar.p, o.ssel.p, alpha, ca);

@@ -54,6 +54,10 @@ SIGNATURES = {
     "ffmm_opcache_put_harmonic_classes": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "ffmm_opcache_put_harmonic_level": (_i32, [_vp, _d, _vp, _vp, _vp]),
     "ffmm_opcache_has_width": (_i32, [_vp, _d, ctypes.POINTER(_i32)]),
+    "ffmm_opcache_put_class_factors": (_i32, [_vp, _d, _vp, _vp, _vp]),
+    "ffmm_plan_set_tol": (_i32, [_vp, _d]),
+    "ffmm_plan_capture": (_i32, [_vp, _i32, _i32]),
+    "ffmm_plan_replay": (_i32, [_vp, _vp, _vp]),
     "ffmm_plan_create": (_i32, [_vp, _vp, _i64, _i32, _vp, _d, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)]),
     "ffmm_plan_destroy": (_i32, [_vp]),
     "ffmm_plan_sizes": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i64), _vp, _vp, _vp,

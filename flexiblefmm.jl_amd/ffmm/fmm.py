@@ -315,6 +315,7 @@ class OperatorCache:
         self._device = {}
         self._widths = {}          # harmonic methods: width -> HarmonicOperators (Yukawa: one per width)
         self._put = {}             # device -> widths uploaded
+        self._factors = {}         # (device, tol) -> ranks of the uploaded low-rank class factors
 
     def has_base_width(self) -> bool:
         return not math.isnan(self.W0)
@@ -438,6 +439,25 @@ class OperatorCache:
             self._device[device] = h
         return h
 
+    def class_factors(self, device: int, tol: float):
+        """The low-rank class factors of tolerance ``tol`` on the device (host SVDs of the
+        class representatives, ``class_factors``, src/operators/symmetry.jl:169-178),
+        uploaded once per tolerance; their ranks."""
+        if _is_harmonic(self.method):
+            raise ArgumentError("tol > 0 takes the cube method (low-rank class operators)")
+        got = self._factors.get((device, tol))
+        if got is not None:
+            return got
+        oc = self.device(device)
+        ranks, A, Bt = _ops.class_factors(self.host_operators(), tol)
+        r32 = np.ascontiguousarray(ranks, dtype=np.int32)
+        a = np.ascontiguousarray(np.concatenate([x.ravel(order="F") for x in A]))
+        b = np.ascontiguousarray(np.concatenate([x.ravel(order="F") for x in Bt]))
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        call("ffmm_opcache_put_class_factors", oc, float(tol), P(r32), P(a), P(b))
+        self._factors[(device, tol)] = ranks
+        return ranks
+
     def __len__(self):
         return 0 if self._host is None else 1
 
@@ -506,6 +526,24 @@ class TreeArrays:
         return len(self.level_start) - 1
 
 
+def _check_plan_kwargs(method, m2l: str, tol: float, fast: bool, eltype):
+    """The reference's DevicePlan keywords (src/device_passes.jl:90, 113-115): ``tol`` >
+    0 (low-rank class operators) takes the cube method with the dense class M2L; the
+    operators are FP64, and ``eltype`` must hold them ("eltype must hold the operators'
+    element type Float64", src/device_passes.jl:168-169); ``fast`` (relaxed arithmetic
+    in the evaluation, src/device.jl:12-20) is not offered: ``precision="mixed"`` is the
+    reduced-precision form."""
+    if not 0 <= tol < 1:
+        raise ArgumentError("tol must lie in [0, 1)")
+    if tol > 0 and (_is_harmonic(method) or m2l != "dense"):
+        raise ArgumentError("tol > 0 takes the cube method with the dense class M2L")
+    if eltype is not None and np.dtype(eltype) not in (np.dtype(np.float64), np.dtype(np.complex128)):
+        raise ArgumentError("eltype must hold the operators' element type Float64")
+    if fast:
+        raise ArgumentError("fast=true (relaxed evaluation arithmetic) is not offered; precision=\"mixed\" is the "
+                            "reduced-precision form")
+
+
 class DevicePlan:
     """``DevicePlan(method, xs; ppl=50, bbox, cache, max_levels=30)``: the device tree,
     its lists, the operators' device copies and the work buffers
@@ -517,7 +555,9 @@ class DevicePlan:
     group (``None``: targets only) and outputs come back per group."""
 
     def __init__(self, method, xs, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
-                 max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
+                 max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0,
+                 tol: float = 0.0, fast: bool = False, eltype=None):
+        _check_plan_kwargs(method, m2l, tol, fast, eltype)
         if isinstance(xs, tuple):
             if len(xs) == 0:
                 raise ArgumentError("no groups")
@@ -565,6 +605,20 @@ class DevicePlan:
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
             # the per-width Yukawa operators of the levels with translations (2 .. L-1)
             cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+        self.tol = 0.0
+        if tol > 0:
+            self.set_tol(tol)
+
+    def set_tol(self, tol: float):
+        """The class M2L through the low-rank class factors of tolerance ``tol``
+        (``DevicePlan(...; tol)``, src/device_passes.jl:14-27, 570-576; 0: the dense
+        representatives); the per-class ranks are ``self.class_ranks``.  A captured pass
+        sequence is dropped."""
+        self._captured = None
+        tol = float(tol)
+        self.class_ranks = self.cache.class_factors(self.device, tol) if tol > 0 else None
+        call("ffmm_plan_set_tol", self._h, tol)
+        self.tol = tol
 
     # -- plan data: warm restarts without the tree build --
     _FILE_MAGIC = b"FFMMPLAN\n"
@@ -617,11 +671,13 @@ class DevicePlan:
         self._read_sizes()
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
             cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+        self.tol = 0.0
         return self
 
     @classmethod
     def from_tree(cls, method, tree, *, cache: OperatorCache = None, ppl: int = 50, precision: str = "f64",
-                  m2l: str = "dense", device: int = 0) -> "DevicePlan":
+                  m2l: str = "dense", device: int = 0, tol: float = 0.0, fast: bool = False,
+                  eltype=None) -> "DevicePlan":
         """``DevicePlan(f::FMM, to)`` (src/device_passes.jl:90-96): the plan from a tree
         built on the host -- any object with the fields of ``TreeArrays`` (``origin``,
         ``W0``, ``level_start``, ``cell``, ``parent``, ``child_first``, ``plo``, ``phi``,
@@ -630,6 +686,7 @@ class DevicePlan:
         (``ffmm_plan_from_tree``); the lists and buffers are built on the device.  The
         root box must be a width of the cache (``w_ref * 2^e``), which it is when the
         cache fitted it (``fit_box``) or holds no width yet."""
+        _check_plan_kwargs(method, m2l, tol, fast, eltype)
         cache = OperatorCache(method) if cache is None else cache
         if cache.method is not method:
             raise ArgumentError("the cache was built for a different method")
@@ -673,6 +730,9 @@ class DevicePlan:
         self._read_sizes()
         if _is_harmonic(method) and isinstance(method.kernel, Yukawa3D):
             cache.ensure_widths(device, [math.ldexp(self.W0, -l) for l in range(2, self.nlevels)])
+        self.tol = 0.0
+        if tol > 0:
+            self.set_tol(tol)
         return self
 
     def __del__(self):
@@ -852,6 +912,37 @@ class DevicePlan:
             call("ffmm_evaluate", self._h, _ptr(t), nt, _ptr(out), o.code)
         else:
             call("ffmm_evaluate_nodes", self._h, _ptr(t), nt, _ptr(nodes), _ptr(out), o.code)
+        return out
+
+    def capture(self, output=Potential, nd: int = 1):
+        """Capture the pass sequence of ``nd`` densities per point and ``output`` at the
+        sources as one HIP graph (``ffmm_plan_capture``), for solves repeated on these
+        points; ``replay`` then runs one."""
+        if self.ngroups != 1:
+            raise ArgumentError("a plan of several groups is not captured")
+        o = _output(output)
+        call("ffmm_plan_capture", self._h, int(nd), o.code)
+        self._captured = (int(nd), o)
+        return self
+
+    def replay(self, tau, out=None):
+        """One captured solve (``ffmm_plan_replay``): the densities in input order, the
+        output at the sources as ``matvec`` returns it."""
+        cap = getattr(self, "_captured", None)
+        if cap is None:
+            raise ArgumentError("capture the plan first")
+        nd, o = cap
+        t = _dev_f64(tau, self.device)
+        if t.shape[0] != self.N or (t.ndim == 1 and nd != 1) or (t.ndim == 2 and t.shape[1] != nd):
+            raise DimensionMismatch(f"expected {self.N} x {nd} densities")
+        torch = _torch()
+        nc = self._ncomp(o)
+        shape = (self.N,) if nc == 1 else (self.N, nc)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        elif tuple(out.shape) != shape:
+            raise DimensionMismatch(f"expected an output of shape {shape}")
+        call("ffmm_plan_replay", self._h, _ptr(t), _ptr(out))
         return out
 
     def matvec(self, tau, output=Potential, sorted: bool = False, out=None):

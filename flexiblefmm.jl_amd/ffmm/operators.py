@@ -211,6 +211,20 @@ def level_operators(w: float, p: int = 8, pc: int | None = None, d: float = 0.1,
     return ops
 
 
+def class_factors(ops: CubeOperators, tol: float):
+    """``class_factors`` (src/operators/symmetry.jl:169-178): each class representative
+    as ``A·Bt`` from its SVD, keeping the singular values above ``tol`` times the
+    largest (at least one); returns (ranks, [A_c], [Bt_c])."""
+    ranks, A, Bt = [], [], []
+    for M in ops.class_ops:
+        U, S, Vt = np.linalg.svd(M)
+        k = max(1, int(np.count_nonzero(S > tol * S[0])))
+        ranks.append(k)
+        A.append(U[:, :k] * S[:k])
+        Bt.append(Vt[:k])
+    return np.array(ranks), A, Bt
+
+
 def m2l_dense(ops: CubeOperators, f: int) -> np.ndarray:
     """The dense M2L matrix of far offset ``f`` (multipole -> check data) at the
     operators' width (src/operators/symmetry.jl:321-325)."""

@@ -123,6 +123,14 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
                           const double *M2M, const double *L2L, int nclass, const double *class_ops,
                           const int32_t *far_class, const int32_t *idx, const int8_t *sgn, const double *tables);
 
+/* Low-rank factors of the nclass representatives of a cube cache, for one tolerance:
+ * class_factors (src/operators/symmetry.jl:169-178), rep_c ~ A_c Bt_c with the singular
+ * values above tol times the largest kept (k_c = ranks[c]); A: the A_c (Ns x k_c, column
+ * major) one after another, Bt: the Bt_c (k_c x Ns, column major) one after another.
+ * A tolerance's factors are fixed once put. */
+int ffmm_opcache_put_class_factors(ffmm_opcache cache, double tol, const int32_t *ranks, const double *A,
+                                   const double *Bt);
+
 /* kernels of the harmonic methods */
 #define FFMM_KERNEL_LAPLACE 0    /* Laplace3D(): 1/(4 pi r) (src/kernels.jl:179-200) */
 #define FFMM_KERNEL_YUKAWA 1     /* Yukawa3D(kappa): e^{-kappa r}/(4 pi r) (src/kernels.jl:220-240) */
@@ -271,6 +279,26 @@ int ffmm_matvec(ffmm_plan plan, const double *tau, int sorted, double *out, int 
  * partial locals (Ns x nnodes) and the locals of the real leaves (Nl x nleaves, in the
  * order of the leaves level by level). */
 int ffmm_plan_download_expansions(ffmm_plan plan, double *multipoles, double *plocals, double *leaf_locals);
+
+/* Capture the plan's pass sequence as one HIP graph, for solves repeated on the same
+ * points (iterative solvers, time steps, served requests of a resident geometry): the
+ * gather of nd densities per point from a plan-owned input buffer, the upward and
+ * downward passes, and the evaluation of `output` at the sources into a plan-owned
+ * output buffer.  One eager pass with zero densities runs first (its checks, buffers
+ * and library workspaces); the plan's buffers then stay in place while the graph lives
+ * (ffmm_plan_set_tol and a new capture drop it).  No reference counterpart: the
+ * reference launches its passes one by one (src/device_passes.jl:619-640). */
+int ffmm_plan_capture(ffmm_plan plan, int nd, int output);
+/* One captured solve: dens (device or host, nd x N, input order) in, out (device or
+ * host, ncomp x N, input order) written; the results of ffmm_build_expansions_nd +
+ * ffmm_evaluate_at_sources, bitwise.  The expansions are left as that pass left them. */
+int ffmm_plan_replay(ffmm_plan plan, const double *dens, double *out);
+
+/* The class M2L of a cube plan with the dense class M2L through the low-rank factors of
+ * tolerance tol (put with ffmm_opcache_put_class_factors), two GEMMs per class and chunk:
+ * DevicePlan(...; tol) (src/device_passes.jl:14-27, 570-576); tol = 0: the dense
+ * representatives again.  The expansions are to be built again after the change. */
+int ffmm_plan_set_tol(ffmm_plan plan, double tol);
 
 /* Per-stage device times of the last calls, in milliseconds (hipEvent pairs on the
  * plan's stream): [0] tree, [1] lists, [2] upward, [3] downward, [4] M2L (within

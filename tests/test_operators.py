@@ -165,3 +165,21 @@ def test_homogeneous_rescale():
     assert mrd(b.P_up.apply(x), lam * a.P_up.apply(x)) < 1e-8
     assert mrd(op.m2l_dense(b, 7), op.m2l_dense(a, 7) / lam) < 1e-14
     assert b.P_up.rank == a.P_up.rank and b.P_down.rank == a.P_down.rank
+
+
+def test_class_factors(ops8):
+    """Low-rank class factors (class_factors, src/operators/symmetry.jl:169-178): each
+    representative is reproduced to tol in the spectral norm, the ranks lie well below
+    n = 296 (test/test_device_passes.jl:44: below n/2 at 1e-8), and the smallest ranks
+    are those the reference's docstring gives for Laplace at p = 8 ("13 and 26" at 1e-6
+    and 1e-8; here 13 and 24)."""
+    from ffmm import operators as op
+    for tol, smallest in ((1e-6, 13), (1e-8, 24)):
+        ranks, A, Bt = op.class_factors(ops8, tol)
+        assert len(ranks) == len(ops8.class_ops) == 16
+        assert ranks.min() == smallest and ranks.max() < ops8.nsmall // 2
+        for a, b, M in zip(A, Bt, ops8.class_ops):
+            assert np.linalg.norm(a @ b - M, 2) <= tol * np.linalg.norm(M, 2)
+    r6, _, _ = op.class_factors(ops8, 1e-6)
+    r8, _, _ = op.class_factors(ops8, 1e-8)
+    assert np.all(r6 <= r8)
