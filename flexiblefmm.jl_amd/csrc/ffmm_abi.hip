@@ -980,14 +980,17 @@ int plan_ncomp(const Plan &p, int output) {
   return output == FFMM_POTENTIAL ? 1 : output == FFMM_GRADIENT ? 3 : 4;
 }
 
-void plan_capture(Plan &p, int nd, int output) {
+void plan_capture(Plan &p, int nd, int output, const double *dens) {
   set_device(*p.ctx);
   if (p.ctx->sync(), p.gexec) plan_drop_graph(p);
   AllocScope scope(*p.ctx);
   const int nc = plan_ncomp(p, output);
   p.gin.resize((size_t)nd * p.N);
   p.gout.resize((size_t)nc * p.N);
-  fill_f64(*p.ctx, p.gin.p, (int64_t)nd * p.N, 0.0);
+  if (dens)
+    FFMM_HIP(hipMemcpyAsync(p.gin.p, dens, sizeof(double) * nd * p.N, hipMemcpyDefault, p.ctx->stream));
+  else
+    fill_f64(*p.ctx, p.gin.p, (int64_t)nd * p.N, 0.0);
   // one eager pass: its checks, the buffers it allocates and the libraries' workspaces
   plan_build(p, p.gin.p, nd, 0);
   plan_eval_sources(p, p.gout.p, output);

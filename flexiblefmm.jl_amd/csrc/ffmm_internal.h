@@ -562,9 +562,10 @@ void plan_release(Plan &p);
 // them again, from the block cache of the context it runs on)
 void plan_shrink(Plan &p);
 // capture the pass sequence of nd densities per point and `output` at the sources (one
-// eager pass with zero densities first); replay it: dens (device, nd x N, input order)
-// to out (device, ncomp x N); drop it (the buffers may move again)
-void plan_capture(Plan &p, int nd, int output);
+// eager pass first, with `dens` (device or host, nd x N) when given, else with zeros:
+// its output is then in gout); replay it: dens (nd x N, input order) to out (ncomp x N);
+// drop it (the buffers may move again)
+void plan_capture(Plan &p, int nd, int output, const double *dens = nullptr);
 void plan_replay(Plan &p, const double *dens, double *out);
 void plan_drop_graph(Plan &p);
 // the output components per point of a plan's method (ffmm_evaluate_at_sources)
