@@ -9,5 +9,6 @@ from .fmm import (FMM, CubeKernelIndependent, DevicePlan, FMMState, Gradient, La
                   Potential, PotentialGradient, Pressure, SphericalHarmonics, StokesHarmonics, Stokeslet3D, Velocity, Yukawa3D,
                   build_expansions, build_expansions_, device_build_expansions_, device_matvec_, direct, direct_self,
                   evaluate, evaluate_at_sources, plan_data_check, square_bbox)
+from .circle2d import CircleOperatorCache, CirclePlan, KernelIndependent, Laplace2D  # noqa: F401
 from .stokes_api import (StokesFMMKernel, StokesFMMPlan, stokes_densities, stokes_fmm_eval_,  # noqa: F401
                          stokes_fmm_eval_src_trg_)

@@ -57,6 +57,15 @@ SIGNATURES = {
     "ffmm_opcache_put_class_factors": (_i32, [_vp, _d, _vp, _vp, _vp]),
     "ffmm_plan_set_tol": (_i32, [_vp, _d]),
     "ffmm_plan_capture": (_i32, [_vp, _i32, _i32]),
+    "ffmm_circle_create": (_i32, [_vp, _i32, _vp, _i64, _i32, _vp, _d, _i32, ctypes.POINTER(_vp)]),
+    "ffmm_circle_sizes": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), _vp, _vp]),
+    "ffmm_circle_put_level": (_i32, [_vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "ffmm_circle_build_expansions": (_i32, [_vp, _vp]),
+    "ffmm_circle_evaluate_at_sources": (_i32, [_vp, _vp]),
+    "ffmm_circle_evaluate": (_i32, [_vp, _vp, _i64, _vp]),
+    "ffmm_circle_download_tree": (_i32, [_vp] * 11),
+    "ffmm_circle_download_lists": (_i32, [_vp, _i32, _vp, _vp, _vp]),
+    "ffmm_circle_destroy": (_i32, [_vp]),
     "ffmm_plan_replay": (_i32, [_vp, _vp, _vp]),
     "ffmm_plan_create": (_i32, [_vp, _vp, _i64, _i32, _vp, _d, _i32, _i32, _i32, _i32, ctypes.POINTER(_vp)]),
     "ffmm_plan_destroy": (_i32, [_vp]),

@@ -999,13 +999,21 @@ class FMM:
     (src/fmm.jl:24-49): here the tree, its lists and the operators live on the GPU (a
     ``DevicePlan``)."""
 
-    def __init__(self, method, sources, *, ppl: int = 50, bbox=None, cache: OperatorCache = None,
+    def __init__(self, method, sources, *, ppl: int = 50, bbox=None, cache=None,
                  max_levels: int = 30, precision: str = "f64", m2l: str = "dense", device: int = 0):
         if isinstance(sources, tuple) and len(sources) == 1:
             sources = sources[0]
         self.method = method
-        self.plan = DevicePlan(method, sources, ppl=ppl, bbox=bbox, cache=cache, max_levels=max_levels,
-                               precision=precision, m2l=m2l, device=device)
+        from . import circle2d
+        if isinstance(method, circle2d.KernelIndependent):
+            # the 2D circle method (src/operators/kifmm.jl): FP64, its own plan
+            if precision != "f64" or m2l != "dense":
+                raise ArgumentError("the circle method runs in FP64 with its dense M2L")
+            self.plan = circle2d.CirclePlan(method, sources, ppl=ppl, bbox=bbox, cache=cache, max_levels=max_levels,
+                                            device=device)
+        else:
+            self.plan = DevicePlan(method, sources, ppl=ppl, bbox=bbox, cache=cache, max_levels=max_levels,
+                                   precision=precision, m2l=m2l, device=device)
         self.cache = self.plan.cache
 
     @property
@@ -1013,8 +1021,9 @@ class FMM:
         return self.plan.nlevels
 
     def __repr__(self):
-        return (f"FMM({type(self.method).__name__}, Tree{{3,Float64}}: {self.plan.N} points, {self.plan.nnodes} nodes on "
-                f"{self.plan.nlevels} levels)")
+        d = 2 if hasattr(self.plan, "download_tree") and type(self.plan).__name__ == "CirclePlan" else 3
+        return (f"FMM({type(self.method).__name__}, Tree{{{d},Float64}}: {self.plan.N} points, {self.plan.nnodes} nodes "
+                f"on {self.plan.nlevels} levels)")
 
 
 class FMMState:
@@ -1029,7 +1038,7 @@ def build_expansions_(state: FMMState, f: FMM, sources, tau) -> FMMState:
     """``build_expansions!(state, fmm, sources, tau)`` (src/fmm.jl:264-298)."""
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
-    G = f.plan.ngroups
+    G = getattr(f.plan, "ngroups", 1)
     if G > 1:
         # one tuple entry per group (src/fmm.jl:270-281)
         if not isinstance(tau, (tuple, list)) or len(tau) != G or (
@@ -1067,11 +1076,15 @@ def build_expansions(f: FMM, sources, tau) -> FMMState:
 def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -> np.ndarray:
     """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:104-133): the
     output at every source of group g (1-based) in the original order."""
-    G = f.plan.ngroups
+    G = getattr(f.plan, "ngroups", 1)
     if not 1 <= g <= G:
         raise ArgumentError(f"group {g} of {G}")
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
+    if not hasattr(f.plan, "ngroups"):          # the 2D circle method: the potential
+        if not isinstance(_output(output), Potential):
+            raise ArgumentError("the circle method evaluates the potential")
+        return f.plan.evaluate_at_sources().cpu().numpy()
     out = f.plan.evaluate_at_sources(output)
     a, b = f.plan.group_offsets[g - 1], f.plan.group_offsets[g]
     return out[a:b].cpu().numpy()
@@ -1081,4 +1094,8 @@ def evaluate(f: FMM, state: FMMState, targets, output=Potential) -> np.ndarray:
     """``evaluate(fmm, state, targets; output)`` (src/evaluate.jl:81-102)."""
     if state.fmm is not f:
         raise ArgumentError("the state belongs to another FMM")
+    if not hasattr(f.plan, "ngroups"):          # the 2D circle method: the potential
+        if not isinstance(_output(output), Potential):
+            raise ArgumentError("the circle method evaluates the potential")
+        return f.plan.evaluate(targets).cpu().numpy()
     return f.plan.evaluate(targets, output).cpu().numpy()

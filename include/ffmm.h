@@ -370,6 +370,48 @@ int ffmm_server_import(ffmm_server server, ffmm_opcache cache, int64_t geometry,
 /* Finish every queued request, then release the slots and the resident plans. */
 int ffmm_server_destroy(ffmm_server server);
 
+/* ---- the 2D kernel-independent method on circles ----
+ * KernelIndependent(Laplace2D(); nequiv) (src/operators/kifmm.jl:1-87) for charges and
+ * the potential: the reference's FMM(m, sources; ppl, bbox) with build_expansions and
+ * evaluate_at_sources / evaluate (src/fmm.jl:31-49, 264-314), its quadtree from the host
+ * builder for D = 2 (src/tree/builder.jl, lists src/tree/lists.jl:71-124), the passes and
+ * the evaluation on the device in FP64. */
+typedef struct ffmm_circle_s *ffmm_circle;
+
+/* The plan of n host points (xy, 2 x n AoS) in the box (origin, width), ppl points per
+ * leaf at most, nequiv (4:64) points per circle: the quadtree and its interaction lists
+ * built on the host and uploaded.  Errors as FMM(m, sources): "the points must be
+ * finite", "points lie outside the requested box", "tree reached max_levels". */
+int ffmm_circle_create(ffmm_context ctx, int nequiv, const double *xy, int64_t n, int ppl, const double origin[2],
+                       double width, int max_levels, ffmm_circle *out);
+/* Levels, nodes, the first node of each level (nlevels + 1) and the far pairs of each
+ * level (nlevels); the arrays may be NULL. */
+int ffmm_circle_sizes(ffmm_circle plan, int *nlevels, int64_t *nnodes, int64_t *level_start, int64_t *level_pairs);
+/* The operators of level `level` (2 .. nlevels-1; boxes of width ldexp(width, -level)):
+ * level_operators(m, w) (src/operators/kifmm.jl:72-86, src/operators/ki_common.jl:
+ * 176-199), column major: the large circle (2 x nequiv SoA, relative to the centre),
+ * the pseudo-inverse factors Uh_up (r_up x n), Vs_up = V_up diag(invS_up) (n x r_up),
+ * Uh_dn, Vs_dn, M2M (n x 4n), L2L (4n x n) and the 40 M2L matrices (n x n each, far
+ * offsets in table order, src/util.jl:59-97). */
+int ffmm_circle_put_level(ffmm_circle plan, int level, const double *large, int r_up, const double *Uh_up,
+                          const double *Vs_up, int r_dn, const double *Uh_dn, const double *Vs_dn, const double *M2M,
+                          const double *L2L, const double *M2L);
+/* build_expansions (src/fmm.jl:264-298): tau, n device charges in input order. */
+int ffmm_circle_build_expansions(ffmm_circle plan, const double *tau);
+/* evaluate_at_sources (src/evaluate.jl:111-127): out, n device values in input order. */
+int ffmm_circle_evaluate_at_sources(ffmm_circle plan, double *out);
+/* evaluate (src/evaluate.jl:81-96) at nt device targets (2 x nt AoS) in the box,
+ * located on the device (src/tree/locate.jl:10-22). */
+int ffmm_circle_evaluate(ffmm_circle plan, const double *targets, int64_t nt, double *out);
+/* The tree, 0-based (host buffers, NULL to skip): cell (2 x nnodes), parent,
+ * child_first, plo, phi, leaf, ghost, xlist, colleagues (9 x nnodes), perm (n). */
+int ffmm_circle_download_tree(ffmm_circle plan, int32_t *cell, int32_t *parent, int32_t *child_first, int32_t *plo,
+                              int32_t *phi, uint8_t *leaf, uint8_t *ghost, uint8_t *xlist, int32_t *colleagues,
+                              int32_t *perm);
+/* The far pairs of a level by target: ptr (level's nodes + 1), src (global), far index. */
+int ffmm_circle_download_lists(ffmm_circle plan, int level, int32_t *ptr, int32_t *src, int32_t *far);
+int ffmm_circle_destroy(ffmm_circle plan);
+
 #ifdef __cplusplus
 }
 #endif
