@@ -330,6 +330,86 @@ def stokes_bench(n: int, ppl: int, nerr: int):
     return res
 
 
+PUBLISHED_2D = {"matvec_s": 0.31, "points": 1_000_000, "error": 8e-14,
+                "case": "2D Laplace, 10^6 uniform points, KernelIndependent nequiv=60 (circles 60), matvec with cached "
+                        "operators (tree build excluded), FP64, 12 Apple cores", "source": "docs/kifmm.md:169"}
+
+
+def circle2d_bench(n: int, nt: int, n_pub: int, nerr: int):
+    """The 2D circle method (the Laplace2D configuration: KernelIndependent nequiv=60,
+    FP64, ppl=50, box ((0,0),1)): n uniform sources with N(0,1) charges and nt uniform
+    targets, potential at the sources and at the targets; the plan (host quadtree and
+    lists, operators per level, uploads) and the solve (expansions + evaluations) timed
+    apart, warm; the error against an FP64 direct sum (numpy) at nerr sampled sources and
+    targets.  Then the reference's published 2D case (n_pub uniform points, matvec =
+    expansions + evaluation at the sources) beside its published time."""
+    import ffmm
+    torch = __import__("torch")
+    rng = np.random.default_rng(21)
+    m = ffmm.KernelIndependent(ffmm.Laplace2D(), nequiv=60)
+    cache = ffmm.CircleOperatorCache(m)
+    box = ((0.0, 0.0), 1.0)
+    C = -1.0 / (4.0 * np.pi)
+
+    def direct(src, tau, x, skip=None):
+        out = np.empty(len(x))
+        for a in range(0, len(x), 256):
+            d = x[a:a + 256, None, :] - src[None, :, :]
+            r2 = d[..., 0] ** 2 + d[..., 1] ** 2
+            if skip is not None:
+                r2[np.arange(len(r2)), skip[a:a + 256]] = 1.0
+            out[a:a + 256] = (C * np.log(r2)) @ tau
+        return out
+
+    def run(pts, tau, tgt):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan = ffmm.CirclePlan(m, pts, ppl=50, bbox=box, cache=cache)
+        t1 = time.perf_counter()
+        tau_d = torch.from_numpy(tau).cuda()
+        tgt_d = None if tgt is None else torch.from_numpy(tgt).cuda()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        u = plan.matvec(tau_d)
+        v = None if tgt is None else plan.evaluate(tgt_d)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return plan, u, v, (t1 - t0), (t3 - t2)
+
+    pts = rng.random((n, 2))
+    tau = rng.standard_normal(n)
+    tgt = rng.random((nt, 2))
+    run(pts, tau, tgt)                       # warm: operators of every width, library workspaces
+    plan, u, v, tplan, tsolve = run(pts, tau, tgt)
+    rows = np.sort(rng.choice(n, min(nerr, n), replace=False))
+    trows = np.sort(rng.choice(nt, min(nerr, nt), replace=False))
+    u = u.cpu().numpy()
+    v = v.cpu().numpy()
+    ud = direct(pts, tau, pts[rows], rows)
+    vd = direct(pts, tau, tgt[trows])
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))   # noqa: E731
+    res = {"workload": f"Laplace2D KernelIndependent nequiv=60, {n} uniform sources + {nt} uniform targets, ppl=50, "
+                       f"FP64, potential at sources and targets",
+           "levels": plan.nlevels, "nodes": plan.nnodes, "far_pairs": int(plan.level_pairs.sum()),
+           "plan_ms": 1e3 * tplan, "solve_ms": 1e3 * tsolve, "points_per_s": n / tsolve,
+           "rel_l2_sources": rel(u[rows], ud), "rel_l2_targets": rel(v[trows], vd)}
+    log(f"[bench] circle2d: plan {1e3 * tplan:.1f} ms, solve {1e3 * tsolve:.1f} ms, errors "
+        f"{res['rel_l2_sources']:.1e} / {res['rel_l2_targets']:.1e}")
+    if n_pub > 0:
+        pp = rng.random((n_pub, 2))
+        tp = rng.standard_normal(n_pub)
+        run(pp, tp, None)
+        plan, up, _, tplan, tsolve = run(pp, tp, None)
+        rows = np.sort(rng.choice(n_pub, 1000, replace=False))
+        err = rel(up.cpu().numpy()[rows], direct(pp, tp, pp[rows], rows))
+        res["published_case"] = {"points": n_pub, "plan_ms": 1e3 * tplan, "matvec_ms": 1e3 * tsolve,
+                                 "rel_l2": err, "published": PUBLISHED_2D,
+                                 "speedup_vs_published": PUBLISHED_2D["matvec_s"] / tsolve}
+        log(f"[bench] circle2d 1e6: matvec {1e3 * tsolve:.1f} ms (published 310 ms), plan {1e3 * tplan:.1f} ms, "
+            f"error {err:.1e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +435,10 @@ def main():
                     help="the plain-form sequential leg runs every k-th request")
     ap.add_argument("--stokes-n", type=int, default=2_000_000,
                     help="points of the Stokes surface leg (0: skip the leg)")
+    ap.add_argument("--circle-n", type=int, default=100_000,
+                    help="sources of the 2D circle leg (0: skip the leg); targets are half as many")
+    ap.add_argument("--circle-pub-n", type=int, default=1_000_000,
+                    help="points of the 2D leg's published comparison (0: skip)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="before the steps, copy a buffer of this size once on the device (a kernel of known "
                          "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile)")
@@ -522,6 +606,10 @@ def main():
     if args.stokes_n > 0:
         stokes = stokes_bench(args.stokes_n, args.ppl, min(args.nerr, 1000))
 
+    circle = None
+    if args.circle_n > 0:
+        circle = circle2d_bench(args.circle_n, args.circle_n // 2, args.circle_pub_n, min(args.nerr, 2000))
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: restated CPU FMM on {args.cpu_sample} points, best of {args.cpu_runs}")
@@ -546,7 +634,7 @@ def main():
             "errors": err, "stages_ms": stages, "matvec_ms": matvec_ms,
             "tree": tree_stats,
             "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "plain_f64": plain,
-            "stokes_surface": stokes}
+            "stokes_surface": stokes, "laplace2d_circle": circle}
     print(json.dumps(line), flush=True)
 
 

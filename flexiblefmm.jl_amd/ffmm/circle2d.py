@@ -232,7 +232,12 @@ class CirclePlan:
     def evaluate(self, targets):
         """``evaluate!`` at targets inside the root box (located on the device)."""
         torch = _torch()
-        x = torch.as_tensor(_f64(targets, 2)).to(f"cuda:{self.device}")
+        if isinstance(targets, torch.Tensor):
+            x = targets.to(device=f"cuda:{self.device}", dtype=torch.float64).contiguous()
+            if x.ndim != 2 or x.shape[1] != 2:
+                raise DimensionMismatch(f"expected an N × 2 array, got {tuple(x.shape)}")
+        else:
+            x = torch.as_tensor(_f64(targets, 2)).to(f"cuda:{self.device}")
         out = torch.empty(x.shape[0], dtype=torch.float64, device=f"cuda:{self.device}")
         torch.cuda.synchronize(self.device)
         call("ffmm_circle_evaluate", self._h, _ptr(x), x.shape[0], _ptr(out))

@@ -5,7 +5,7 @@ invariants and FP64 direct sums.
 Reference checks of the same kind: the circle operators' accuracy statement
 (src/operators/kifmm.jl:7-11: worst-case translation error ~0.547^N, floor ~1e-14 near
 N = 64), the tree invariants of test/test_tree.jl:7-131 and the matvec against direct
-sums (README.md:38-51; docs/kifmm.md:165-168: circles 60 at 8e-14)."""
+sums (README.md:38-51; docs/kifmm.md:169: circles 60 at 8e-14)."""
 import math
 
 import numpy as np

@@ -1,7 +1,7 @@
 """The 2D circle method (KernelIndependent(Laplace2D(); nequiv)) through the library on
 the GPU: the quadtree and lists it builds equal the oracle's field by field; the
 device passes match the oracle's FP64 FMM on the same tree (1e-12) and the FP64 direct
-sums (rel L2 1e-12 at nequiv = 60, the method's floor: docs/kifmm.md:165-168, circles
+sums (rel L2 1e-12 at nequiv = 60, the method's floor: docs/kifmm.md:169, circles
 60 at 8e-14); the host API (FMM / build_expansions / evaluate_at_sources / evaluate,
 README.md:38-51) returns the same; errors carry the reference's phrases."""
 import numpy as np
