@@ -130,7 +130,8 @@ def cpu_baseline(n_sample: int, ppl: int, runs: int, gpu_solve):
             "gpu_rel_l2_vs_restated_cpu_fmm": rel}
 
 
-def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, plain_every: int):
+def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, plain_every: int,
+                  timeline: str | None = None):
     """The served stream (ffmm.workloads.served_stream): Poisson arrivals, point counts
     log-uniform in [nmin, nmax], 70% Laplace3D cube p = 8 (mixed, spectral M2L) / 30%
     Yukawa3D(5) harmonics p = 10 (mixed, dense class M2L), uniform or Gaussian-mixture
@@ -161,9 +162,10 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, pla
             geo[r["gid"]] = W.request_points(r)
         data.append((geo[r["gid"]], W.request_charges(r)))
     total = float(sum(r["n"] for r in reqs))
-    # the server's budget: 85% of the device memory free when the leg starts
+    # the server's budget: 60% of the device memory free when the leg starts; the slots'
+    # idle blocks (kept for the next solve) use the rest
     torch.cuda.empty_cache()
-    budget = int(0.85 * torch.cuda.mem_get_info()[0])
+    budget = int(0.6 * torch.cuda.mem_get_info()[0])
     log(f"[bench] serving: {nreq} requests, {total:.3e} points, {len(geo)} geometries, {nslots} slots "
         f"(generated in {time.perf_counter() - t:.1f} s)")
 
@@ -207,12 +209,25 @@ def serving_bench(nreq: int, nmin: int, nmax: int, rate: float, nslots: int, pla
         st = srv.stats()
         srv.close()
         lat = np.array(lat)
+        sizes = np.array([r["n"] for r in reqs])
+        if timeline:
+            with open(f"{timeline}.{'rate' if time_scale else 'burst'}.json", "w") as fh:
+                json.dump([{"i": i, "t_submit": t_sub[i] - t0, "latency_ms": float(lat[i]), "n": int(reqs[i]["n"]),
+                            "lap": bool(reqs[i]["lap"]), "gid": int(reqs[i]["gid"])} for i in range(nreq)], fh)
+        by_size = {}
+        for lo_, hi_ in ((0, 1e5), (1e5, 5e5), (5e5, 1e7)):
+            sel = (sizes >= lo_) & (sizes < hi_)
+            if sel.any():
+                by_size[f"{int(lo_)}-{int(hi_)}"] = {"requests": int(sel.sum()),
+                                                      "p50": float(np.percentile(lat[sel], 50)),
+                                                      "p99": float(np.percentile(lat[sel], 99))}
         return {"wall_s": wall, "served_points_per_s": total / wall, "p50_latency_ms": float(np.percentile(lat, 50)),
                 "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
                 "submit_lag_ms": {"mean": 1e3 * float(lag.mean()), "max": 1e3 * float(lag.max())},
                 "plans_built": st["plans_built"], "plans_reused": st["plans_reused"],
                 "evictions": st["evictions"], "peak_device_bytes": st["peak_bytes"], "budget_bytes": budget,
-                "requeued_out_of_memory": st["requeued_out_of_memory"],
+                "requeued_out_of_memory": st["requeued_out_of_memory"], "allocation_flushes": st["allocation_flushes"],
+                "latency_by_size_ms": by_size,
                 "worst_plan_over_estimate": st["worst_plan_over_estimate_permille"] / 1000}, first
 
     sustained, first = serve(1.0)
@@ -430,7 +445,9 @@ def main():
     ap.add_argument("--serve-nmax", type=int, default=2_000_000)
     ap.add_argument("--serve-rate", type=float, default=50.0,
                     help="Poisson arrivals per second of the latency pass (about half the saturated rate)")
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-timeline", default=None,
+                    help="write each served request's submission time, latency and size to PATH.{rate,burst}.json")
     ap.add_argument("--serve-plain-every", type=int, default=8,
                     help="the plain-form sequential leg runs every k-th request")
     ap.add_argument("--stokes-n", type=int, default=2_000_000,
@@ -599,7 +616,7 @@ def main():
         del plan                 # the measured plan's device memory goes back before the stream
         torch.cuda.synchronize()
         serving = serving_bench(args.serve_requests, args.serve_nmin, args.serve_nmax, args.serve_rate,
-                                args.serve_slots, args.serve_plain_every)
+                                args.serve_slots, args.serve_plain_every, args.serve_timeline)
         log("[bench] serving: " + json.dumps(serving))
 
     stokes = None

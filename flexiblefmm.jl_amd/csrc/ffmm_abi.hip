@@ -4,6 +4,7 @@
 #include "ffmm_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -20,6 +21,9 @@ void fail(int code, const std::string &msg) { throw Error(code, msg); }
 // of all of them (a server's slots and the host's default context each keep their own)
 static std::mutex g_ctx_mu;
 static std::vector<Context *> g_contexts;
+static std::atomic<int64_t> g_flushes{0};   // allocations that failed and returned idle blocks
+
+int64_t allocation_flushes() { return g_flushes.load(); }
 
 // return idle blocks of a context to HIP, the largest first, until at most `keep` bytes
 // stay cached; the bytes returned
@@ -74,11 +78,13 @@ Context *device_alloc(void **p, size_t bytes, size_t *blk) {
   }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
-    // the cached blocks may be what is missing: those of this context first, then
-    // those of every context on the device; then try once more
+    // the cached blocks may be what is missing: those of this context first, then those
+    // of every context on the device; then try once more (returning only as many as the
+    // allocation needs measured slower: far more failures, each waiting on the device)
     (void)hipGetLastError();
     int dev = 0;
     (void)hipGetDevice(&dev);
+    g_flushes++;
     if (!(c && context_trim(*c, 0) >= bytes)) flush_device(dev);
     e = hipMalloc(p, bytes);
   }

@@ -300,6 +300,7 @@ struct LevelLists {
   bool filled = false;                  // src/far written (the spectral M2L needs only ptr)
   DBuf<int32_t> ptr, src, far;          // by target: ptr (nl + 1, 0-based pair index)
   DBuf<int32_t> csrc, cfar;             // class major, by target within a class
+  DBuf<int32_t> ctgt;                   // the level-local target of each class-major pair (fused mixed M2L)
   DBuf<int32_t> cstart;                 // nclass nl + 1: first class-major pair of (class c, target t) at c nl + t
   std::vector<int64_t> class_pair;      // nclass + 1 offsets into csrc
   // target chunks of the dense class M2L: chunk k covers the level-local targets
@@ -554,6 +555,8 @@ void context_release(Context &c);
 // bytes stay cached (the bytes returned); the bytes it holds idle
 size_t context_trim(Context &c, size_t keep);
 size_t context_idle_bytes(Context &c);
+// allocations so far that failed and returned the contexts' idle blocks
+int64_t allocation_flushes();
 void plan_init(Plan &p, Context &ctx, OpCache &o, const double *xyz, int64_t n, int ppl, const double origin[3],
                double width, int e0, int max_levels, int precision, int m2l);
 void plan_release(Plan &p);

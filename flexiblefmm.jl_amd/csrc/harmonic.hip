@@ -371,7 +371,9 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
 template <int KIND, int OUT>
 __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   extern __shared__ double sm[];
-  constexpr int TW = KIND == 2 ? 12 : 7;   // doubles per tile entry: position and density components
+  // doubles per tile entry: position and density components (Stokes 9, Laplace a charge
+  // and a dipole, Yukawa a charge)
+  constexpr int TW = KIND == 2 ? 12 : KIND == 0 ? 7 : 4;
   __shared__ double tile[HT_CK * TW];
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];

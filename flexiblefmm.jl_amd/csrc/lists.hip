@@ -127,7 +127,7 @@ __global__ void k_class_count(const int32_t *__restrict__ far, const int32_t *__
 __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__restrict__ far,
                              const int32_t *__restrict__ ptr, const int32_t *__restrict__ cstart, int64_t nl,
                              int nclass, const int32_t *__restrict__ far_class, int32_t *__restrict__ csrc,
-                             int32_t *__restrict__ cfar) {
+                             int32_t *__restrict__ cfar, int32_t *__restrict__ ctgt) {
   const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x % 64;
   if (q >= nl) return;   // whole waves: q is wave-uniform
@@ -143,6 +143,7 @@ __global__ void k_class_fill(const int32_t *__restrict__ src, const int32_t *__r
     if (p < p1) {
       csrc[at + rank] = src[p];
       cfar[at + rank] = f;
+      if (ctgt) ctgt[at + rank] = (int32_t)q;
     }
     next += mine;
   }
@@ -262,9 +263,14 @@ void build_lists(Plan &plan) {
     exclusive_scan_i32_async(ctx, ccnt.p, li.cstart.p, ne, 0);
     li.csrc.resize(np);
     li.cfar.resize(np);
+    // the target of each class-major pair: the harmonic methods' fused mixed class M2L
+    // sums a target's consecutive products in a tile
+    const bool want_tgt = ops.method == METHOD_HARMONIC && plan.precision == FFMM_PREC_MIXED;
+    if (want_tgt) li.ctgt.resize(np);
     if (np > 0) {
       k_class_fill<<<nblocks(64 * nl, TB), TB, 0, s>>>(li.src.p, li.far.p, li.ptr.p, li.cstart.p, nl, nclass,
-                                                  ops.far_class.p, li.csrc.p, li.cfar.p);
+                                                  ops.far_class.p, li.csrc.p, li.cfar.p,
+                                                  want_tgt ? li.ctgt.p : nullptr);
       FFMM_LAUNCH_CHECK();
     }
     const int64_t nch = (np + cap - 1) / cap;

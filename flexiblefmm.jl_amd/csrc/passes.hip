@@ -330,10 +330,10 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
                                                           const int32_t *__restrict__ gsel,
                                                           const int32_t *__restrict__ fsel,
                                                           const double *__restrict__ offs, ChunkArgs ca,
-                                                          int32_t ntiles) {
+                                                          int32_t ntiles, const int32_t *__restrict__ ctgt) {
   constexpr int PPT = FM_COLS / NB;   // pairs per tile
   __shared__ float Bs[FM_COLS * FM_BS];
-  __shared__ int32_t pf[PPT], ps[PPT];
+  __shared__ int32_t pf[PPT], ps[PPT], pt[PPT];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nkb = (N + 15) >> 4;      // K blocks of 16
@@ -356,6 +356,7 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
       const int32_t p = ca.cs[c] + lp0 + tid;
       pf[tid] = cfar[p];
       ps[tid] = csrc[p];
+      pt[tid] = ctgt[p];
     }
     __syncthreads();
     // gather: a wave per pair, a lane per coefficient, all blocks
@@ -422,39 +423,59 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
       }
     }
     __syncthreads();
-    // coalesced copy of the tile's columns: pair column q = co[c] + lp0 + pl, NB N entries
+    // coalesced copy of the tile's columns (pair column q = co[c] + lp0 + pl, NB N
+    // entries), the products of consecutive pairs of one target summed first: the first
+    // pair of each such run carries the run's sum, the others are not written
     float *__restrict__ out = Yc + ((int64_t)ca.co[c] + lp0) * NB * N;
-    for (int col = w; col < npt * NB; col += FM_TB / 64)
-      for (int j = lane; j < N; j += 64) out[(int64_t)col * N + j] = Bs[col * FM_BS + j];
+    for (int col = w; col < npt * NB; col += FM_TB / 64) {
+      const int pl = col / NB, b = col - pl * NB;
+      if (pl > 0 && pt[pl] == pt[pl - 1]) continue;
+      int pe = pl + 1;
+      while (pe < npt && pt[pe] == pt[pl]) pe++;
+      for (int j = lane; j < N; j += 64) {
+        float acc = Bs[col * FM_BS + j];
+        for (int r = pl + 1; r < pe; r++) acc += Bs[(r * NB + b) * FM_BS + j];
+        out[(int64_t)col * N + j] = acc;
+      }
+    }
     __syncthreads();   // before the next tile's pair list and gather (the K padding is never written)
   }
 }
 
 // The scatter of products already in the target frame: pL[e, t] += alpha sum over the
-// target's pairs (class by class, in pair order) of Y'[e, q]
+// target's columns (class by class) of Y'[e, q]; the fused kernel has summed each run of
+// the target's pairs within a tile of ppt pairs into the run's first column
 __global__ void k_chunk_scatter_framed(double *__restrict__ pL, const float *__restrict__ Yc, int Ns, int nbN,
                                        const int32_t *__restrict__ cstart, int64_t nl, int32_t t0, double alpha,
-                                       ChunkArgs ca) {
+                                       ChunkArgs ca, int ppt) {
   __shared__ int32_t pq[SCATTER_MAXP];
   __shared__ int32_t npair;
   const int32_t t = t0 + (int32_t)blockIdx.x;
   if (threadIdx.x < 64) {
+    // lane c: the columns of class c that carry the target's sums: its first pair's,
+    // and each tile start (class-local index a multiple of ppt) among its later pairs
     const int c = threadIdx.x;
-    int32_t p0 = 0, n = 0;
+    int32_t q0 = 0, f = 0, nh = 0;
     if (c < ca.nclass) {
       const int64_t ec = (int64_t)c * nl + t;
-      p0 = cstart[ec];
-      n = cstart[ec + 1] - p0;
+      const int32_t p0 = cstart[ec], n = cstart[ec + 1] - p0;
+      if (n > 0) {
+        q0 = p0 + ca.co[c] - ca.cs[c];
+        f = q0 - ca.co[c];
+        nh = 1 + (f + n - 1) / ppt - f / ppt;
+      }
     }
-    int32_t v = n;
+    int32_t v = nh;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int32_t u = __shfl_up(v, off, 64);
       if (c >= off) v += u;
     }
-    const int32_t start = v - n;
-    const int32_t qoff = c < ca.nclass ? ca.co[c] - ca.cs[c] : 0;
-    for (int32_t k = 0; k < n; k++) pq[start + k] = p0 + k + qoff;
+    const int32_t start = v - nh;
+    if (nh > 0) {
+      pq[start] = q0;
+      for (int32_t k = 1; k < nh; k++) pq[start + k] = ca.co[c] + (f / ppt + k) * ppt;
+    }
     if (c == 63) npair = v;
   }
   __syncthreads();
@@ -471,27 +492,35 @@ __global__ void k_chunk_scatter_framed(double *__restrict__ pL, const float *__r
 // four consecutive entries per lane, 16-byte loads (same summation order per entry)
 __global__ void k_chunk_scatter_framed4(double *__restrict__ pL, const float4 *__restrict__ Yc, int Ns, int nbN4,
                                         const int32_t *__restrict__ cstart, int64_t nl, int32_t t0, double alpha,
-                                        ChunkArgs ca) {
+                                        ChunkArgs ca, int ppt) {
   __shared__ int32_t pq[SCATTER_MAXP];
   __shared__ int32_t npair;
   const int32_t t = t0 + (int32_t)blockIdx.x;
   if (threadIdx.x < 64) {
+    // lane c: the columns of class c that carry the target's sums: its first pair's,
+    // and each tile start (class-local index a multiple of ppt) among its later pairs
     const int c = threadIdx.x;
-    int32_t p0 = 0, n = 0;
+    int32_t q0 = 0, f = 0, nh = 0;
     if (c < ca.nclass) {
       const int64_t ec = (int64_t)c * nl + t;
-      p0 = cstart[ec];
-      n = cstart[ec + 1] - p0;
+      const int32_t p0 = cstart[ec], n = cstart[ec + 1] - p0;
+      if (n > 0) {
+        q0 = p0 + ca.co[c] - ca.cs[c];
+        f = q0 - ca.co[c];
+        nh = 1 + (f + n - 1) / ppt - f / ppt;
+      }
     }
-    int32_t v = n;
+    int32_t v = nh;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int32_t u = __shfl_up(v, off, 64);
       if (c >= off) v += u;
     }
-    const int32_t start = v - n;
-    const int32_t qoff = c < ca.nclass ? ca.co[c] - ca.cs[c] : 0;
-    for (int32_t k = 0; k < n; k++) pq[start + k] = p0 + k + qoff;
+    const int32_t start = v - nh;
+    if (nh > 0) {
+      pq[start] = q0;
+      for (int32_t k = 1; k < nh; k++) pq[start + k] = ca.co[c] + (f / ppt + k) * ppt;
+    }
     if (c == 63) npair = v;
   }
   __syncthreads();
@@ -752,18 +781,18 @@ void m2l_dense_level(Plan &plan, int l) {
       const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 4 * 256);
       if (nb == 4)
         k_class_m2l32<4><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32r, li.csrc.p, li.cfar.p, o.gsel.p,
-                                                o.fsel.p, o.offs.p, ca, ntiles);
+                                                o.fsel.p, o.offs.p, ca, ntiles, li.ctgt.p);
       else
         k_class_m2l32<1><<<grid, FM_TB, 0, s>>>(plan.Yc32.p, plan.Mp.p, Ns, N, reps32r, li.csrc.p, li.cfar.p, o.gsel.p,
-                                                o.fsel.p, o.offs.p, ca, ntiles);
+                                                o.fsel.p, o.offs.p, ca, ntiles, li.ctgt.p);
       FFMM_LAUNCH_CHECK();
       if (nbN % 4 == 0) {
         const unsigned bt4 = (unsigned)std::min(1024, (nbN / 4 + 63) / 64 * 64);
         k_chunk_scatter_framed4<<<(unsigned)nt, bt4, 0, s>>>(pL, reinterpret_cast<const float4 *>(plan.Yc32.p), Ns,
-                                                              nbN / 4, li.cstart.p, plan.nl[l], t0, alpha, ca);
+                                                              nbN / 4, li.cstart.p, plan.nl[l], t0, alpha, ca, ppt);
       } else {
         k_chunk_scatter_framed<<<(unsigned)nt, bt, 0, s>>>(pL, plan.Yc32.p, Ns, nbN, li.cstart.p, plan.nl[l], t0,
-                                                           alpha, ca);
+                                                           alpha, ca, ppt);
       }
       FFMM_LAUNCH_CHECK();
       continue;

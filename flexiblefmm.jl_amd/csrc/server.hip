@@ -5,6 +5,10 @@
 //
 // * Slots: each worker thread owns a context (its own HIP stream and rocBLAS handle),
 //   so the solves admitted at one time run concurrently on the device.
+// * Memory: each slot keeps the blocks its solves free for the next one (no hipMalloc /
+//   hipFree between requests of similar sizes), up to its share of the device memory the
+//   budget leaves free; an allocation that fails returns the idle blocks of every
+//   context and tries again.
 // * Admission: a free slot takes the first queued request whose device-memory
 //   estimate fits the budget next to what running solves and resident plans hold
 //   (evicting resident plans, least recently used first, to make room); a request
@@ -50,6 +54,7 @@ size_t plan_bytes(const Plan &p) {
   s += cap_bytes(p.s2m_nodes) + cap_bytes(p.leaf_nodes);
   for (const auto &li : p.lists)
     s += cap_bytes(li.ptr) + cap_bytes(li.src) + cap_bytes(li.far) + cap_bytes(li.csrc) + cap_bytes(li.cfar) +
+         cap_bytes(li.ctgt) +
          cap_bytes(li.cstart);
   s += cap_bytes(p.Mp) + cap_bytes(p.Pl) + cap_bytes(p.Lc) + cap_bytes(p.Lc32) + cap_bytes(p.tau) + cap_bytes(p.tau32);
   s += cap_bytes(p.pM) + cap_bytes(p.tup) + cap_bytes(p.Mup) + cap_bytes(p.tM) + cap_bytes(p.G) + cap_bytes(p.tdn) +
@@ -398,6 +403,14 @@ int ffmm_server_create(int device, int nslots, int64_t hbm_budget, ffmm_server *
       context_init(w->ctx, device, nullptr);
       S->workers.push_back(std::move(w));
     }
+    // the slots' idle blocks live in the device memory the budget leaves free: each slot
+    // keeps at most its share of it (at least 256 MiB), so that the budget plus the idle
+    // blocks fit the device
+    size_t fr = 0, tot = 0;
+    FFMM_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t outside = fr > (size_t)hbm_budget ? fr - (size_t)hbm_budget : 0;
+    const size_t share = std::max<size_t>((size_t)256 << 20, outside / (size_t)nslots);
+    for (auto &w : S->workers) w->ctx.blocks->limit = share;
     for (auto &w : S->workers) w->th = std::thread(worker_loop, S.get(), w.get());
     *out = S.release();
   });
@@ -455,10 +468,10 @@ int ffmm_server_stats(ffmm_server S, int64_t *stats, int n) {
   return guard([&] {
     if (!S || !stats) fail(FFMM_ERR_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(S->mu);
-    const int64_t v[11] = {S->completed, S->built,     S->reused,     (int64_t)S->resident.size(),
+    const int64_t v[12] = {S->completed, S->built,     S->reused,     (int64_t)S->resident.size(),
                            S->in_use,    S->peak,      S->evictions,  (int64_t)S->queue.size(),
-                           idle_bytes(*S), S->worst_ratio, S->retried};
-    for (int i = 0; i < n && i < 11; i++) stats[i] = v[i];
+                           idle_bytes(*S), S->worst_ratio, S->retried, allocation_flushes()};
+    for (int i = 0; i < n && i < 12; i++) stats[i] = v[i];
   });
 }
 

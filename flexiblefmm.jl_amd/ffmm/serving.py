@@ -124,10 +124,11 @@ class Server:
         call("ffmm_server_import", self._h, oc, int(geometry), buf, len(data))
 
     def stats(self) -> dict:
-        v = (ctypes.c_int64 * 11)()
-        call("ffmm_server_stats", self._h, v, 11)
+        v = (ctypes.c_int64 * 12)()
+        call("ffmm_server_stats", self._h, v, 12)
         keys = ["completed", "plans_built", "plans_reused", "resident", "bytes_in_use", "peak_bytes", "evictions",
-                "queued", "idle_cached_bytes", "worst_plan_over_estimate_permille", "requeued_out_of_memory"]
+                "queued", "idle_cached_bytes", "worst_plan_over_estimate_permille", "requeued_out_of_memory",
+                "allocation_flushes"]
         return dict(zip(keys, v[:]))
 
     def close(self):

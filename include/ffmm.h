@@ -344,6 +344,10 @@ typedef struct {
   double *out_targets;    /* host, ncomp x nt */
 } ffmm_request;
 
+/* nslots worker slots (1:16), each with its own stream; hbm_budget: the device bytes the
+ * running solves' estimates and the resident plans may hold.  Each slot keeps the idle
+ * blocks of its solves for the next one, up to its share of the device memory free at
+ * creation beyond the budget. */
 int ffmm_server_create(int device, int nslots, int64_t hbm_budget, ffmm_server *out);
 /* Queue a request; *ticket identifies it. */
 int ffmm_server_submit(ffmm_server server, const ffmm_request *req, int64_t *ticket);
@@ -356,7 +360,9 @@ int ffmm_server_wait(ffmm_server server, int64_t ticket, double *latency_ms);
  *     that fails returns them),
  * [9] the largest ratio of a built plan's bytes to its admission estimate, x 1000,
  * [10] requests queued again after running out of device memory next to other solves
- *      (they then run alone). */
+ *      (they then run alone),
+ * [11] allocations in the process so far that failed and returned the contexts' idle
+ *      blocks before trying again. */
 int ffmm_server_stats(ffmm_server server, int64_t *stats, int n);
 /* Plan data of the resident plan of (cache, geometry), as ffmm_plan_save writes it (buf
  * may be NULL to ask for the size).  No counterpart in the reference: with
