@@ -355,6 +355,11 @@ struct Plan {
   std::vector<DBuf<int32_t>> spec_gptr, spec_cand;   // per level: sibling groups' far-source candidates (FP64)
   std::vector<DBuf<int32_t>> spec_gcol, spec_tiles;  // per level: groups' colleague children, active tiles (FP32)
   std::vector<int64_t> spec_ntiles;
+  // per level and active tile: its sourced colleagues staged together (the boxes by
+  // staging row, each (group, colleague)'s row); the tiles that fit and those that do not
+  std::vector<DBuf<int32_t>> spec_ulist, spec_fit, spec_over;
+  std::vector<DBuf<uint8_t>> spec_uslot;
+  std::vector<int64_t> spec_nfit, spec_nover;
   // targets
   DBuf<double> tgt_sorted;
   DBuf<int32_t> tgt_node, tgt_leafslot, tgt_order, tgt_ptr;

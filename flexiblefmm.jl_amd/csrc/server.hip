@@ -65,6 +65,9 @@ size_t plan_bytes(const Plan &p) {
   for (const auto &b : p.spec_cand) s += cap_bytes(b);
   for (const auto &b : p.spec_gcol) s += cap_bytes(b);
   for (const auto &b : p.spec_tiles) s += cap_bytes(b);
+  for (const auto *v : {&p.spec_ulist, &p.spec_fit, &p.spec_over})
+    for (const auto &b : *v) s += cap_bytes(b);
+  for (const auto &b : p.spec_uslot) s += cap_bytes(b);
   s += cap_bytes(p.tgt_sorted) + cap_bytes(p.tgt_node) + cap_bytes(p.tgt_leafslot) + cap_bytes(p.tgt_order) +
        cap_bytes(p.tgt_ptr) + cap_bytes(p.tgt_keys);
   return s;

@@ -321,6 +321,11 @@ __global__ void k_fill_u8(uint8_t *__restrict__ a, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
 }
+__global__ void k_take_i32(const int32_t *__restrict__ in, const int32_t *__restrict__ idx, int64_t n,
+                           int32_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
 
 // The staging buffer of a quad: row r = 16 q + j (child q of the quad in group j), 8
 // chunks of 16 bytes (2 frequencies); chunk h of row r sits at position h ^ ((r >> 1) & 7).
@@ -442,6 +447,338 @@ __global__ void __launch_bounds__(HM_TB, 1) k_spec_hadamard_mfma(const float2 *_
 }
 
 // ---------------------------------------------------------------------------
+// 2''. The FP32 Hadamard on the matrix cores, each tile's source boxes staged once.
+//
+// The groups of a tile are consecutive in the level's order, so their parents lie close
+// together and share colleagues: a tile's (group, colleague) entries name about a third
+// as many distinct boxes (a compact 2 x 2 x 4 block of parents: 96 boxes for 416
+// entries).  Once per plan, each tile's distinct sourced colleagues (its union, at most
+// HU_UMAX) and every group's slot in it are found.  A workgroup takes a block of 8
+// frequencies and HU_R consecutive tiles.  Per tile it stages the union's child spectra
+// in LDS (zeros for children without sources; one zero box that missing colleagues
+// point at), the next tile's spectra being loaded into registers meanwhile.  Wave f runs
+// the MFMAs of frequency f (the MFMA form of k_spec_hadamard_mfma) over all 52 quads,
+// unrolled: the table operands of its quads are read once per workgroup into registers
+// (104 per lane), the groups' slots once per tile, so each pair of MFMAs reads one
+// spectrum value from LDS.  Tiles of the busy levels have a sourced child in nearly
+// every quad, so quads without one cost little: they multiply zeros.  The tile's 128
+// target rows go out through LDS, 64 bytes each.  So a spectrum slice is read from L2
+// once per tile, not once per group that has its box as a colleague.  Tiles whose union
+// is larger go to k_spec_hadamard_mfma.
+// ---------------------------------------------------------------------------
+constexpr int HU_FB = 4;                    // frequencies per workgroup
+constexpr int HU_TB = 64 * HU_FB * 2;       // two waves per frequency, 13 colleagues each
+constexpr int HU_UMAX = 200;                // distinct colleagues staged per tile
+constexpr int HU_NC = 32;                   // colours of the staged boxes
+constexpr int HU_NR = 224;                  // rows: colour + 32 x (index within the colour)
+constexpr int HU_ZROW = HU_NR;              // a zero box
+constexpr int HU_NRS = 256;                 // staged rows per child: thread t stages row t / 2
+constexpr int HU_US = HU_NRS;               // rows per child in LDS
+constexpr int HU_NS = HM_TILE * NCOLL;      // (group, colleague) slots of a tile
+constexpr int HU_KH = (NCOLL - 1) / 2;      // colleagues per wave: k < 13, then k > 13
+constexpr int HU_SW = (HU_KH + 1) / 2;      // 4-byte words of a group's slot offsets per half in LDS
+constexpr int HU_R = 16;                    // tiles per workgroup
+constexpr int HU_IS = HU_FB / 2;            // 16-byte staging items per slice
+constexpr int HU_NI = NCHILD * HU_NRS * HU_IS;                   // per tile
+constexpr int HU_NV = HU_NI / HU_TB;                             // per thread: one per child
+static_assert(HU_NV == NCHILD && HU_NRS * HU_IS == HU_TB && HU_ZROW < HU_NRS, "item n of thread t: row t / 2, child n");
+constexpr int HU_NQ = 2 * HU_KH;            // quads of a wave: its colleagues, 2 halves each
+static_assert(HU_NC * ((HU_UMAX + HU_NC - 1) / HU_NC) <= HU_NR && HU_ZROW <= 255, "rows are bytes");
+
+// Per active tile ti (tile id tiles[ti]): the distinct sourced colleagues of its groups
+// other than the parents themselves (its union), each placed in a row of the staging
+// buffer; ulist[ti][row] = the box's (level-local first child << 8 | sourced-children
+// mask), 0 for an unused row; uslot[ti][j][k] = the row of group j's colleague k
+// (HU_ZROW when it has none); fit[ti] = 1 when the union has at most HU_UMAX boxes (else
+// over[ti] = 1).  The 16 groups' colleague k are read by one MFMA, 16 lanes (one LDS
+// phase of 8-byte reads) per child; a box's row is its colour + 32 x (its place among
+// its colour), the colours chosen greedily (in order of first appearance over (group,
+// colleague)) so that the boxes of one colleague index have distinct colours: then
+// those 16 reads fall in distinct LDS banks.  With 32 colours for the 16 boxes of a
+// colleague index the greedy choice nearly always succeeds; where it does not, a
+// colour with room is taken (a bank conflict, the result unchanged).  One wave per tile.
+__global__ void __launch_bounds__(64) k_tile_union(const int32_t *__restrict__ gcol, const int32_t *__restrict__ tiles,
+                                                   int64_t ngroups, int32_t *__restrict__ ulist,
+                                                   uint8_t *__restrict__ uslot, uint8_t *__restrict__ fit,
+                                                   uint8_t *__restrict__ over) {
+  constexpr int HT = 1024;                 // hash table of the tile's boxes (at most 416 distinct)
+  __shared__ int32_t e[HU_NS];
+  __shared__ int16_t first[HU_NS];
+  __shared__ int16_t slot[HU_NS];
+  __shared__ int32_t hkey[HT], hmin[HT];
+  __shared__ uint32_t colset[HU_UMAX];     // per box: the colleague indices it appears at
+  __shared__ uint8_t row[HU_UMAX];
+  const int64_t ti = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t g0 = (int64_t)tiles[ti] * HM_TILE;
+  for (int i = lane; i < HT; i += 64) {
+    hkey[i] = -1;
+    hmin[i] = HU_NS;
+  }
+  for (int i = lane; i < HU_NS; i += 64) {
+    const int64_t g = g0 + i / NCOLL;
+    const int k = i % NCOLL;
+    e[i] = (g < ngroups && k != SELF_SLOT) ? gcol[g * NCOLL + k] : -1;
+  }
+  __syncthreads();
+  // each box's first entry (the smallest index naming it), by open addressing
+  int hs[(HU_NS + 63) / 64];
+  for (int n = 0, i = lane; i < HU_NS; i += 64, n++) {
+    const int32_t v = e[i];
+    hs[n] = -1;
+    if (v < 0) continue;
+    int h = (int)(((uint32_t)v * 2654435761u) >> 22) & (HT - 1);
+    for (;;) {
+      const int32_t old = atomicCAS(&hkey[h], -1, v);
+      if (old == -1 || old == v) break;
+      h = (h + 1) & (HT - 1);
+    }
+    atomicMin(&hmin[h], i);
+    hs[n] = h;
+  }
+  __syncthreads();
+  for (int n = 0, i = lane; i < HU_NS; i += 64, n++) first[i] = (int16_t)(hs[n] >= 0 ? hmin[hs[n]] : i);
+  __syncthreads();
+  int32_t count = 0;
+  for (int base = 0; base < HU_NS; base += 64) {
+    const int i = base + lane;
+    const bool isfirst = i < HU_NS && e[i] >= 0 && first[i] == i;
+    const uint64_t b = __ballot(isfirst);
+    if (isfirst) slot[i] = (int16_t)(count + __popcll(b & ((1ull << lane) - 1)));
+    count += __popcll(b);
+  }
+  const bool ok = count <= HU_UMAX;
+  if (lane == 0) {
+    fit[ti] = ok;
+    over[ti] = !ok;
+  }
+  if (!ok) return;
+  for (int u = lane; u < count; u += 64) colset[u] = 0;
+  __syncthreads();
+  for (int i = lane; i < HU_NS; i += 64)
+    if (e[i] >= 0) atomicOr(&colset[slot[first[i]]], 1u << (i % NCOLL));
+  __syncthreads();
+  {   // greedy colouring, one box at a time, the state spread over the wave's lanes:
+      // lane k < 27 holds the colours taken at colleague index k, lane c < 32 the boxes
+      // of colour c so far (the loop is wave-uniform)
+    constexpr int CAP = HU_NR / HU_NC;
+    static_assert(HU_NC == 32, "a colour set is a 32-bit mask");
+    uint32_t cm = 0, cnt = 0;
+    uint32_t avail = 0xFFFFFFFFu;          // colours with room
+    uint32_t csn = colset[0];
+    for (int u = 0; u < count; u++) {
+      const uint32_t cs = __builtin_amdgcn_readfirstlane(csn);
+      if (u + 1 < count) csn = colset[u + 1];
+      uint32_t forb = 0;
+      for (uint32_t m = cs; m; m &= m - 1) forb |= __builtin_amdgcn_readlane(cm, __builtin_ctz(m));
+      const uint32_t free = avail & ~forb;
+      const int pick = __builtin_ctz(free ? free : avail);   // none free: a bank conflict, still correct
+      const uint32_t c_cnt = __builtin_amdgcn_readlane(cnt, pick);
+      if (lane == 0) row[u] = (uint8_t)(pick + HU_NC * c_cnt);
+      if (lane == pick) cnt++;
+      if (c_cnt + 1 == CAP) avail &= ~(1u << pick);
+      if ((cs >> lane) & 1) cm |= 1u << pick;
+    }
+  }
+  __syncthreads();
+  for (int r = lane; r < HU_NRS; r += 64) ulist[ti * HU_NRS + r] = 0;
+  __syncthreads();
+  for (int i = lane; i < HU_NS; i += 64) {
+    const int32_t v = e[i];
+    if (v >= 0 && first[i] == i) ulist[ti * HU_NRS + row[slot[i]]] = v;
+    uslot[ti * HU_NS + i] = (uint8_t)(v >= 0 ? row[slot[first[i]]] : HU_ZROW);
+  }
+}
+
+// LDS position of frequency f of child c of the box in row `row`: the 4 frequencies of
+// a slice (32 bytes, rows 8 to the 256 bytes of the 64 banks) are XOR-swizzled by row
+// bits 3-4, so that the 16 lanes of one read phase (one child), at rows of distinct
+// colours (distinct mod 32), touch distinct banks.
+__device__ __forceinline__ int hu_swz(int row) { return (row >> 3) & 3; }
+__device__ __forceinline__ int hu_spos(int c, int row, int f) { return (c * HU_US + row) * HU_FB + (f ^ hu_swz(row)); }
+__device__ __forceinline__ int hu_opos(int ta, int f) { return ta * HU_FB + (f ^ ((ta >> 3) & 3)); }
+
+// Staging items: 16 bytes (frequencies 2 h, 2 h + 1) of child c of the box in row r.
+// Item n of thread t is (h, r, c) = (t % 2, t / 2, n): 2 consecutive items are one
+// 32-byte slice, 16 consecutive threads 8 consecutive rows.
+// The staged spectra of the tile whose rows' entries are in Ent (LDS); bit n of live:
+// item n has a box (an unused row is not written).
+__device__ __forceinline__ void hu_load(const float2 *__restrict__ S, const int32_t *Ent, int f0, int t,
+                                        float4 v[HU_NV], uint32_t &live) {
+  const int h = t % HU_IS, r = t / HU_IS;
+  const int32_t ent = Ent[r];
+  live = ent != 0 ? (1u << HU_NV) - 1 : 0;
+  // 32-bit byte offsets into the level's spectra (the host keeps them under 4 GiB)
+  const uint32_t base = ((uint32_t)(ent >> 8) * SP_NF + (uint32_t)(f0 + 2 * h)) * (uint32_t)sizeof(float2);
+#pragma unroll
+  for (int n = 0; n < HU_NV; n++) {
+    v[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((ent >> n) & 1)
+      v[n] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(S) + base +
+                                               (uint32_t)n * SP_NF * (uint32_t)sizeof(float2));
+  }
+}
+__device__ __forceinline__ void hu_store(float2 *Us, int t, const float4 v[HU_NV], uint32_t live) {
+  if (!live) return;
+  const int h = t % HU_IS, r = t / HU_IS;
+  // frequencies 2 h and 2 h + 1 sit side by side at chunk h ^ (swz >> 1), swapped when
+  // the swizzle is odd
+  const int sw = hu_swz(r);
+  const int ch = 2 * (h ^ (sw >> 1));
+#pragma unroll
+  for (int n = 0; n < HU_NV; n++) {
+    const float4 w = (sw & 1) ? make_float4(v[n].z, v[n].w, v[n].x, v[n].y) : v[n];
+    *reinterpret_cast<float4 *>(Us + (n * HU_US + r) * HU_FB + ch) = w;
+  }
+}
+
+// A workgroup barrier that orders LDS accesses only: no wait for the global loads and
+// stores in flight (a prefetch consumed by the same thread, results written out).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// two workgroups per CU (8 waves each): 4 waves per SIMD, at most 128 VGPRs
+__global__ void __launch_bounds__(HU_TB) __attribute__((amdgpu_waves_per_eu(4, 4))) k_spec_hadamard_union(
+    const float2 *__restrict__ S, float2 *__restrict__ U, const float2 *__restrict__ tables,
+    const int16_t *__restrict__ pairf, const int32_t *__restrict__ tiles, const int32_t *__restrict__ fitl,
+    const int32_t *__restrict__ ulist, const uint8_t *__restrict__ uslot, int64_t nfit, int64_t ngroups,
+    int64_t nranges, int64_t nreal) {
+  __shared__ __attribute__((aligned(16))) float2 Us[NCHILD * HU_US * HU_FB];
+  __shared__ __attribute__((aligned(16))) float2 O[2][HM_TILE * NCHILD * HU_FB];   // the two halves' sums
+  // group j's colleagues of half h: 16-bit byte offsets of their rows, swizzle bits set
+  // (row * 32 + 8 * ((row >> 3) & 3)), in words (2 j + h) * HU_SW ..
+  __shared__ uint32_t Sl[HM_TILE * 2 * HU_SW];
+  __shared__ int32_t Ent[HU_NRS];            // the rows' entries of the tile loaded next
+  static_assert((NFAR + 1) * HU_FB * sizeof(float2) + NCOLL * NCHILD * NCHILD * sizeof(int16_t) <= sizeof(Us),
+                "the tables are staged where the spectra go");
+  static_assert(HU_NRS <= HU_TB && HU_NS <= HU_TB, "one entry and one slot per thread");
+  // XCD-aware order: each XCD sweeps a contiguous range; consecutive blocks are the four
+  // quarters (4 frequencies each) of the same tiles' 128-byte spectrum slices
+  constexpr int NQB = 16 / HU_FB;
+  const int64_t Lb = xcd_block(blockIdx.x, gridDim.x);
+  if (Lb >= nreal) return;   // a padding block
+  const int64_t fline = Lb / (NQB * nranges), rr = Lb % (NQB * nranges);
+  const int64_t fi0 = (rr / NQB) * HU_R, fi1 = fi0 + HU_R < nfit ? fi0 + HU_R : nfit;
+  const int f0 = (int)(NQB * fline + rr % NQB) * HU_FB;
+  const int t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+  const int f = wave % HU_FB, kh = wave / HU_FB;   // frequency f0 + f, colleagues kh ? 14 .. 26 : 0 .. 12
+  const int k0 = kh ? SELF_SLOT + 1 : 0;
+  const int j = lane & 15, q = lane >> 4, a = j >> 1;
+  const bool im_row = j & 1;
+  // the first tile's entries, then (with the tables) its spectra and slots and the
+  // second tile's entries
+  int64_t ti = fitl[fi0];
+  if (t < HU_NRS) Ent[t] = ulist[ti * HU_NRS + t];
+  {
+    float2 *Tt = Us;
+    int16_t *F = reinterpret_cast<int16_t *>(Us + (NFAR + 1) * HU_FB);
+    for (int i = t; i < (NFAR + 1) * HU_FB; i += HU_TB) {
+      const int row = i / HU_FB, ph = i % HU_FB;
+      Tt[i] = row < NFAR ? tables[(int64_t)row * SP_NF + f0 + ph] : make_float2(0.f, 0.f);
+    }
+    for (int i = t; i < NCOLL * NCHILD * NCHILD; i += HU_TB) F[i] = pairf[i] < 0 ? (int16_t)NFAR : pairf[i];
+  }
+  lds_barrier();
+  float4 v[HU_NV];
+  uint32_t live;
+  hu_load(S, Ent, f0, t, v, live);
+  uint32_t sw = t < HU_NS ? uslot[ti * HU_NS + t] : 0;
+  int32_t enext = 0;
+  if (fi0 + 1 < fi1 && t < HU_NRS) enext = ulist[(int64_t)fitl[fi0 + 1] * HU_NRS + t];
+  // each lane's table operands: quad (k, b) multiplies child 4 b + q of colleague k into
+  // target a; the real parts' MFMA takes (row Re: T.re, row Im: T.im), the imaginary
+  // parts' (row Re: -T.im, row Im: T.re).  The latter is the neighbouring lane's former
+  // (rows Re and Im of a target are lanes 2 m and 2 m + 1) with row Re negated, so the
+  // imaginary parts' products are accumulated unsigned and row Re subtracted at the end
+  float A1[HU_NQ];
+  {
+    const float2 *Tt = Us;
+    const int16_t *F = reinterpret_cast<const int16_t *>(Us + (NFAR + 1) * HU_FB);
+#pragma unroll
+    for (int kk = 0, qi = 0; kk < HU_KH; kk++) {
+#pragma unroll
+      for (int b = 0; b < 2; b++, qi++) {
+        const float2 T = Tt[F[((k0 + kk) * NCHILD + 4 * b + q) * NCHILD + a] * HU_FB + f];
+        A1[qi] = im_row ? T.y : T.x;
+      }
+    }
+  }
+  lds_barrier();   // the tables are read: Us takes the spectra
+  if (t < NCHILD * HU_FB) Us[hu_spos(t / HU_FB, HU_ZROW, t % HU_FB)] = make_float2(0.f, 0.f);
+  // a tile's 128 target rows, out of LDS (the two halves' sums) 64 bytes each
+  auto write_out = [&](int64_t tl) {
+    const int64_t g0 = (int64_t)tiles[tl] * HM_TILE;
+    for (int i = t; i < HM_TILE * NCHILD * (HU_FB / 2); i += HU_TB) {
+      const int h = i % (HU_FB / 2), ta = i / (HU_FB / 2);
+      const int64_t g = g0 + ta / NCHILD;
+      if (g >= ngroups) continue;
+      const float2 x0 = O[0][hu_opos(ta, 2 * h)], x1 = O[0][hu_opos(ta, 2 * h + 1)];
+      const float2 y0 = O[1][hu_opos(ta, 2 * h)], y1 = O[1][hu_opos(ta, 2 * h + 1)];
+      *reinterpret_cast<float4 *>(U + (NCHILD * g + ta % NCHILD) * SP_NF + f0 + 2 * h) =
+          make_float4(x0.x + y0.x, x0.y + y0.y, x1.x + y1.x, x1.y + y1.y);
+    }
+  };
+  // per tile: stage its spectra (loaded during the previous tile's MFMAs) and write the
+  // previous tile's rows out, then issue the next tile's loads and run this tile's MFMAs;
+  // the rows are written after the next staging, so that no wait for them holds up the
+  // staging's own
+  int64_t tprev = -1;
+  for (int64_t fi = fi0; fi < fi1; fi++) {
+    if (fi > fi0) lds_barrier();   // the previous tile's reads of Us, Sl, Ent done, its O written
+    hu_store(Us, t, v, live);
+    if (t < HU_NS && t % NCOLL != SELF_SLOT) {
+      const int kt = t % NCOLL, h = kt > SELF_SLOT;
+      reinterpret_cast<uint16_t *>(Sl)[(2 * (t / NCOLL) + h) * HU_SW * 2 + kt - (h ? SELF_SLOT + 1 : 0)] =
+          (uint16_t)(sw * HU_FB * sizeof(float2) + sizeof(float2) * hu_swz((int)sw));
+    }
+    if (t < HU_NRS) Ent[t] = enext;
+    if (tprev >= 0) write_out(tprev);
+    const int64_t tcur = ti;
+    lds_barrier();
+    if (fi + 1 < fi1) {   // the next tile's spectra and slots, and the entries after it
+      ti = fitl[fi + 1];
+      hu_load(S, Ent, f0, t, v, live);
+      if (t < HU_NS) sw = uslot[ti * HU_NS + t];
+      if (fi + 2 < fi1 && t < HU_NRS) enext = ulist[(int64_t)fitl[fi + 2] * HU_NRS + t];
+    }
+    // child q of colleague kk's box, frequency f: byte q HU_US 32 + (offset ^ 8 f)
+    // (hu_spos); children 4 + q: + 4 HU_US rows
+    uint32_t slw[HU_SW];
+#pragma unroll
+    for (int w = 0; w < HU_SW; w++) slw[w] = Sl[(2 * j + kh) * HU_SW + w];
+    const char *Ub = reinterpret_cast<const char *>(Us) + q * HU_US * HU_FB * sizeof(float2);
+    const uint32_t fx = (uint32_t)sizeof(float2) * (uint32_t)f;
+    constexpr int HALF = 4 * HU_US * HU_FB * (int)sizeof(float2);   // children 4 .. 7
+    f32x4 acc0 = f32x4{0.f, 0.f, 0.f, 0.f}, acc1 = acc0;   // the real and the imaginary parts' products
+#pragma unroll
+    for (int kk = 0, qi = 0; kk < HU_KH; kk++) {
+#pragma unroll
+      for (int b = 0; b < 2; b++, qi++) {
+        const uint32_t off = ((slw[kk / 2] >> (16 * (kk % 2))) & 0xFFFF) ^ fx;
+        const float2 Sv = *reinterpret_cast<const float2 *>(Ub + off + b * HALF);
+        // quad_perm [1, 0, 3, 2]: lanes 2 m and 2 m + 1 swap
+        const float A2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(A1[qi]), 0xB1, 0xF, 0xF, false));
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[qi], Sv.x, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A2, Sv.y, acc1, 0, 0, 0);
+      }
+    }
+    // lane (q, j) holds targets 2 q and 2 q + 1 of group j, (re, im) each: the two
+    // halves' sums through LDS to 64-byte rows per target
+    const f32x4 s4 = f32x4{acc0[0] - acc1[0], acc0[1] + acc1[1], acc0[2] - acc1[2], acc0[3] + acc1[3]};
+    const int ta0 = j * NCHILD + 2 * q;
+    O[kh][hu_opos(ta0, f)] = make_float2(s4[0], s4[1]);
+    O[kh][hu_opos(ta0 + 1, f)] = make_float2(s4[2], s4[3]);
+    tprev = tcur;
+  }
+  lds_barrier();
+  write_out(tprev);
+}
+
+// ---------------------------------------------------------------------------
 // 3. inverse: x = IFFT(U_t) on the [0, p)^3 corner, surface entries into pL
 // ---------------------------------------------------------------------------
 // One workgroup per target with far pairs: (k2, k1) over k3 (16 inputs read coalesced,
@@ -536,25 +873,65 @@ void spectral_prepare(Plan &plan) {
   Context &ctx = *plan.ctx;
   hipStream_t s = ctx.stream;
   if (plan.precision == FFMM_PREC_MIXED) {
-    // the groups' colleague tables and the tiles with far sources, once per plan
+    // once per plan: the groups' colleague tables and the tiles with far sources (all
+    // levels, one read back), then each active tile's union and whether it fits (one
+    // more read back)
+    const int L = plan.L;
+    static_assert(MAX_KEY_LEVELS + 2 <= NPINNED / 2, "two pinned slots per level");
     plan.spec_gcol.clear();
     plan.spec_tiles.clear();
-    plan.spec_gcol.resize(plan.L);
-    plan.spec_tiles.resize(plan.L);
-    plan.spec_ntiles.assign(plan.L, 0);
-    DBuf<uint8_t> tflag;
-    for (int l = 2; l < plan.L; l++) {
+    plan.spec_gcol.resize(L);
+    plan.spec_tiles.resize(L);
+    plan.spec_ntiles.assign(L, 0);
+    std::vector<DBuf<uint8_t>> tflag(L);
+    for (int l = 2; l < L; l++) {
       const int64_t lo = plan.lo[l], ng = plan.nl[l] / NCHILD, nt = (ng + HM_TILE - 1) / HM_TILE;
       if (ng == 0) continue;
       plan.spec_gcol[l].resize((size_t)ng * NCOLL);
-      tflag.resize(nt);
-      k_fill_u8<<<nblocks(nt, 256), 256, 0, s>>>(tflag.p, nt);
+      tflag[l].resize(nt);
+      k_fill_u8<<<nblocks(nt, 256), 256, 0, s>>>(tflag[l].p, nt);
       k_group_cols<<<nblocks(ng, 256), 256, 0, s>>>(plan.parent.p, plan.child_first.p, plan.col.p, plan.ghost.p,
                                                     plan.has_source.p, lo, ng, plan.nnodes, plan.spec_gcol[l].p,
-                                                    tflag.p);
+                                                    tflag[l].p);
       FFMM_LAUNCH_CHECK();
       plan.spec_tiles[l].resize(nt);
-      plan.spec_ntiles[l] = compact_flags_u8(ctx, tflag.p, plan.spec_tiles[l].p, nt);
+      compact_flags_u8_async(ctx, tflag[l].p, plan.spec_tiles[l].p, nt, l);
+    }
+    ctx.sync();
+    for (int l = 2; l < L; l++) plan.spec_ntiles[l] = plan.nl[l] / NCHILD > 0 ? ctx.pinned[l] : 0;
+    for (auto *v : {&plan.spec_ulist, &plan.spec_fit, &plan.spec_over}) {
+      v->clear();
+      v->resize(L);
+    }
+    plan.spec_uslot.clear();
+    plan.spec_uslot.resize(L);
+    plan.spec_nfit.assign(L, 0);
+    plan.spec_nover.assign(L, 0);
+    std::vector<DBuf<int32_t>> overidx(L);
+    for (int l = 2; l < L; l++) {
+      const int64_t nt = plan.spec_ntiles[l];
+      if (nt == 0) continue;
+      plan.spec_ulist[l].resize((size_t)nt * HU_NRS);
+      plan.spec_uslot[l].resize((size_t)nt * HU_NS);
+      plan.spec_fit[l].resize(nt);
+      overidx[l].resize(nt);
+      tflag[l].resize(2 * nt);   // fit flags, then overflow flags
+      k_tile_union<<<(unsigned)nt, 64, 0, s>>>(plan.spec_gcol[l].p, plan.spec_tiles[l].p, plan.nl[l] / NCHILD,
+                                              plan.spec_ulist[l].p, plan.spec_uslot[l].p, tflag[l].p,
+                                              tflag[l].p + nt);
+      FFMM_LAUNCH_CHECK();
+      compact_flags_u8_async(ctx, tflag[l].p, plan.spec_fit[l].p, nt, l);
+      compact_flags_u8_async(ctx, tflag[l].p + nt, overidx[l].p, nt, NPINNED / 2 + l);
+    }
+    ctx.sync();
+    for (int l = 2; l < L; l++) {
+      if (plan.spec_ntiles[l] == 0) continue;
+      plan.spec_nfit[l] = ctx.pinned[l];
+      const int64_t no = plan.spec_nover[l] = ctx.pinned[NPINNED / 2 + l];
+      if (no == 0) continue;
+      plan.spec_over[l].resize(no);   // the tile ids themselves, for k_spec_hadamard_mfma
+      k_take_i32<<<nblocks(no, 256), 256, 0, s>>>(plan.spec_tiles[l].p, overidx[l].p, no, plan.spec_over[l].p);
+      FFMM_LAUNCH_CHECK();
     }
     return;
   }
@@ -599,11 +976,21 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   {
     StageTimer tm(plan, 8, l);   // the Hadamard launch alone
     if constexpr (std::is_same<T, float>::value) {
-      const int64_t ntiles = plan.spec_ntiles[l];
-      if (ntiles > 0) {
-        const int64_t nwg = (ntiles + HM_W * HM_R - 1) / (HM_W * HM_R), nreal = nwg * (SP_NF / HM_FB);
+      // the union kernel addresses the level's spectra with 32-bit byte offsets
+      const bool small = (int64_t)nl * SP_NF * (int64_t)sizeof(float2) < ((int64_t)1 << 32);
+      const int32_t *over_tiles = small ? plan.spec_over[l].p : plan.spec_tiles[l].p;
+      const int64_t nfit = small ? plan.spec_nfit[l] : 0, nover = small ? plan.spec_nover[l] : plan.spec_ntiles[l];
+      if (nfit > 0) {
+        const int64_t nranges = (nfit + HU_R - 1) / HU_R, nreal = nranges * (SP_NF / HU_FB);
+        k_spec_hadamard_union<<<(unsigned)pad_xcd(nreal), HU_TB, 0, s>>>(
+            S, U, tables, o.pairf.p, plan.spec_tiles[l].p, plan.spec_fit[l].p, plan.spec_ulist[l].p,
+            plan.spec_uslot[l].p, nfit, ng, nranges, nreal);
+        FFMM_LAUNCH_CHECK();
+      }
+      if (nover > 0) {
+        const int64_t nwg = (nover + HM_W * HM_R - 1) / (HM_W * HM_R), nreal = nwg * (SP_NF / HM_FB);
         k_spec_hadamard_mfma<<<(unsigned)pad_xcd(nreal), HM_TB, 0, s>>>(S, U, tables, o.pairf.p, plan.spec_gcol[l].p,
-                                                                        plan.spec_tiles[l].p, ntiles, ng, nwg, nreal);
+                                                                        over_tiles, nover, ng, nwg, nreal);
       }
     }
     else
