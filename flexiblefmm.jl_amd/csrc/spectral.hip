@@ -473,7 +473,8 @@ constexpr int HU_NC = 32;                   // colours of the staged boxes
 constexpr int HU_NR = 224;                  // rows: colour + 32 x (index within the colour)
 constexpr int HU_ZROW = HU_NR;              // a zero box
 constexpr int HU_NRS = 256;                 // staged rows per child: thread t stages row t / 2
-constexpr int HU_US = HU_NRS;               // rows per child in LDS
+constexpr int HU_US = HU_NRS + 1;           // rows per child in LDS: odd, so that children q and 4 + q
+                                            // (read back to back) sit half the banks apart
 constexpr int HU_NS = HM_TILE * NCOLL;      // (group, colleague) slots of a tile
 constexpr int HU_KH = (NCOLL - 1) / 2;      // colleagues per wave: k < 13, then k > 13
 constexpr int HU_SW = (HU_KH + 1) / 2;      // 4-byte words of a group's slot offsets per half in LDS
@@ -596,7 +597,10 @@ __global__ void __launch_bounds__(64) k_tile_union(const int32_t *__restrict__ g
 // colours (distinct mod 32), touch distinct banks.
 __device__ __forceinline__ int hu_swz(int row) { return (row >> 3) & 3; }
 __device__ __forceinline__ int hu_spos(int c, int row, int f) { return (c * HU_US + row) * HU_FB + (f ^ hu_swz(row)); }
-__device__ __forceinline__ int hu_opos(int ta, int f) { return ta * HU_FB + (f ^ ((ta >> 3) & 3)); }
+// the output buffer: frequency-major, one pad slot per 8 targets, so that the 16 groups'
+// target 2 q of one write phase touch distinct banks
+constexpr int HU_OS = HM_TILE * NCHILD + HM_TILE;   // 144 slots per frequency
+__device__ __forceinline__ int hu_opos(int ta, int f) { return f * HU_OS + ta + (ta >> 3); }
 
 // Staging items: 16 bytes (frequencies 2 h, 2 h + 1) of child c of the box in row r.
 // Item n of thread t is (h, r, c) = (t % 2, t / 2, n): 2 consecutive items are one
@@ -647,7 +651,7 @@ __global__ void __launch_bounds__(HU_TB) __attribute__((amdgpu_waves_per_eu(4, 4
     const int32_t *__restrict__ ulist, const uint8_t *__restrict__ uslot, int64_t nfit, int64_t ngroups,
     int64_t nranges, int64_t nreal) {
   __shared__ __attribute__((aligned(16))) float2 Us[NCHILD * HU_US * HU_FB];
-  __shared__ __attribute__((aligned(16))) float2 O[2][HM_TILE * NCHILD * HU_FB];   // the two halves' sums
+  __shared__ __attribute__((aligned(16))) float2 O[2][HU_FB * HU_OS];   // the two halves' sums
   // group j's colleagues of half h: 16-bit byte offsets of their rows, swizzle bits set
   // (row * 32 + 8 * ((row >> 3) & 3)), in words (2 j + h) * HU_SW ..
   __shared__ uint32_t Sl[HM_TILE * 2 * HU_SW];
