@@ -54,7 +54,7 @@ def traffic_from_profile(kname: str, args):
     """HBM bytes per launch of the dominant kernel from the committed two-pass
     rocprofv3 --pmc profile (FETCH_SIZE, then WRITE_SIZE, each scaled by the copy
     probe's reading), when the profile is of this kernel at this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "hadamard_mfma_4M_mixed_traffic.json")
+    path = os.path.join(ROOT, "profiles", "hadamard_spectral_4M_mixed_traffic.json")
     if not os.path.exists(path) or args.n != 4_000_000 or args.ppl != 64:
         return None
     prof = json.load(open(path))
@@ -584,14 +584,16 @@ def main():
     pairs = int(plan.level_pairs.sum())
     if args.m2l == "spectral":
         # the dominant kernel: the Hadamard products of the spectral M2L, one launch per
-        # level; algorithmic work = far pairs x 2304 frequencies x 8 flops (a complex
-        # multiply-add), over the kernel's summed HIP-event time on the plan's stream
+        # level (mixed: the union kernel, then on some levels the per-group kernel for the
+        # tiles the union staging does not take, timed together); algorithmic work = far
+        # pairs x 2304 frequencies x 8 flops (a complex multiply-add), over the summed
+        # HIP-event time on the plan's stream
         nfreq = 9 * 16 * 16
         flops = float(pairs) * nfreq * 8
         t_k = stages["hadamard"] * 1e-3
         peak = PEAK_FP64_TFLOPS if args.precision == "f64" else PEAK_FP32_MFMA_TFLOPS
-        kname = ("k_spec_hadamard_mfma" if args.precision == "mixed" else "k_spec_hadamard<double, 8>") + \
-            " (spectral M2L Hadamard products)"
+        kname = ("k_spec_hadamard_union + k_spec_hadamard_mfma" if args.precision == "mixed"
+                 else "k_spec_hadamard<double, 8>") + " (spectral M2L Hadamard products)"
     else:
         flops = float(pairs) * 2 * m.nsurface ** 2
         t_k = stages["m2l"] * 1e-3
@@ -602,9 +604,11 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": traffic_from_profile(kname, args), "kernel": kname, "launches": nlaunch,
                 "avg_launch_ms": 1e3 * t_k / max(nlaunch, 1), "flops_per_launch": flops / max(nlaunch, 1),
-                "note": ("FP32 MFMA kernel (v_mfma_f32_16x16x4_f32) against the dense FP32 MFMA peak; "
-                         "achieved counts the far pairs' products only, the MFMAs also multiply the near "
-                         "pairs' zero table rows of each 8 x 8 block") if args.precision == "mixed" else
+                "note": ("FP32 MFMA kernels (v_mfma_f32_16x16x4_f32) against the dense FP32 MFMA peak, a "
+                         "launch being one level (the union kernel, then the per-group kernel for the "
+                         "tiles it does not take); achieved counts the far pairs' products only, the "
+                         "MFMAs also multiply the near pairs' zero table rows of each 8 x 8 block and the "
+                         "missing colleagues' zero boxes") if args.precision == "mixed" else
                         "VALU FMA kernel: MI355X's vector FP64 and FP64-MFMA peaks are both 78.6 TFLOP/s",
                 "eval_sources": {"flops": flops_eval, "ms": stages["eval_sources"],
                                  "achieved_tflops": flops_eval / (stages["eval_sources"] * 1e-3) / 1e12}}

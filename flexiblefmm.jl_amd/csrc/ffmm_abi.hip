@@ -1453,6 +1453,17 @@ int ffmm_plan_set_tol(ffmm_plan plan, double tol) {
   });
 }
 
+int ffmm_plan_spectral_tiles(ffmm_plan plan, int64_t *staged, int64_t *per_group, int n) {
+  return guard([&] {
+    if (!plan || !staged || !per_group) fail(FFMM_ERR_ARGUMENT, "null argument");
+    const Plan &p = plan->c;
+    for (int l = 0; l < n; l++) {
+      staged[l] = per_group[l] = 0;
+      if (l < p.L) spectral_tile_counts(p, l, staged[l], per_group[l]);
+    }
+  });
+}
+
 int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n) {
   return guard([&] {
     if (!plan || !ms) fail(FFMM_ERR_ARGUMENT, "null argument");

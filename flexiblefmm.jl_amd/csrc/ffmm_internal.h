@@ -440,6 +440,7 @@ void m2l_dense_level(Plan &plan, int l);
 void spectral_prepare(Plan &plan);
 // the spectra of one level's sources and targets (the largest level's size)
 void spectral_workspace(Plan &plan);
+bool spectral_tile_counts(const Plan &plan, int l, int64_t &staged, int64_t &per_group);
 void m2l_spectral_level(Plan &plan, int l);
 
 // ---- direct sums (direct.hip) ----

@@ -964,6 +964,18 @@ void spectral_prepare(Plan &plan) {
   }
 }
 
+// The tiles of level l run by the union kernel (staged) and by the per-group kernel; true
+// when the union kernel takes part.  It addresses a level's spectra with 32-bit byte
+// offsets: a level whose FP32 spectra reach 4 GiB runs every tile in the per-group kernel.
+bool spectral_tile_counts(const Plan &plan, int l, int64_t &staged, int64_t &per_group) {
+  staged = per_group = 0;
+  if (plan.precision != FFMM_PREC_MIXED || l >= (int)plan.spec_nfit.size()) return false;
+  const bool small = plan.nl[l] * SP_NF * (int64_t)sizeof(float2) < ((int64_t)1 << 32);
+  staged = small ? plan.spec_nfit[l] : 0;
+  per_group = small ? plan.spec_nover[l] : plan.spec_ntiles[l];
+  return small;
+}
+
 template <class T, int FB>
 static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typename cplx_t<T>::type *U,
                            const typename cplx_t<T>::type *tables) {
@@ -980,10 +992,9 @@ static void spectral_level(Plan &plan, int l, typename cplx_t<T>::type *S, typen
   {
     StageTimer tm(plan, 8, l);   // the Hadamard launch alone
     if constexpr (std::is_same<T, float>::value) {
-      // the union kernel addresses the level's spectra with 32-bit byte offsets
-      const bool small = (int64_t)nl * SP_NF * (int64_t)sizeof(float2) < ((int64_t)1 << 32);
-      const int32_t *over_tiles = small ? plan.spec_over[l].p : plan.spec_tiles[l].p;
-      const int64_t nfit = small ? plan.spec_nfit[l] : 0, nover = small ? plan.spec_nover[l] : plan.spec_ntiles[l];
+      int64_t nfit, nover;
+      const bool staged = spectral_tile_counts(plan, l, nfit, nover);
+      const int32_t *over_tiles = staged ? plan.spec_over[l].p : plan.spec_tiles[l].p;
       if (nfit > 0) {
         const int64_t nranges = (nfit + HU_R - 1) / HU_R, nreal = nranges * (SP_NF / HU_FB);
         k_spec_hadamard_union<<<(unsigned)pad_xcd(nreal), HU_TB, 0, s>>>(

@@ -87,6 +87,7 @@ SIGNATURES = {
     "ffmm_matvec": (_i32, [_vp, _vp, _i32, _vp, _i32]),
     "ffmm_plan_download_expansions": (_i32, [_vp, _vp, _vp, _vp]),
     "ffmm_plan_stage_times": (_i32, [_vp, _vp, _i32]),
+    "ffmm_plan_spectral_tiles": (_i32, [_vp, _vp, _vp, _i32]),
     "ffmm_server_create": (_i32, [_i32, _i32, _i64, ctypes.POINTER(_vp)]),
     "ffmm_server_submit": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),
     "ffmm_server_wait": (_i32, [_vp, _i64, ctypes.POINTER(_d)]),

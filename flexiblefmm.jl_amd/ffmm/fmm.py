@@ -791,6 +791,14 @@ class DevicePlan:
         call("ffmm_plan_download_expansions", self._h, P(M), P(Pl), P(Lc))
         return M.T, Pl.T, Lc.T
 
+    def spectral_tiles(self):
+        """Per level, the tiles of the mixed spectral M2L's Hadamard products run by the
+        kernel staging each tile's distinct colleague boxes, and by the per-group kernel."""
+        a, b = np.zeros(self.nlevels, dtype=np.int64), np.zeros(self.nlevels, dtype=np.int64)
+        call("ffmm_plan_spectral_tiles", self._h, a.ctypes.data_as(ctypes.c_void_p),
+             b.ctypes.data_as(ctypes.c_void_p), self.nlevels)
+        return a, b
+
     def stage_times(self) -> dict:
         ms = (ctypes.c_double * 9)()
         call("ffmm_plan_stage_times", self._h, ms, 9)

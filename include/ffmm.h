@@ -307,6 +307,11 @@ int ffmm_plan_set_tol(ffmm_plan plan, double tol);
  * the reference's verbose plan build prints host-side stage times
  * (src/device_passes.jl:331-336). */
 int ffmm_plan_stage_times(ffmm_plan plan, double *ms, int n);
+/* Per level l < n, the tiles (16 sibling groups each) of the mixed-precision spectral
+ * M2L's Hadamard products run by the kernel that stages each tile's distinct colleague
+ * boxes once (staged[l]) and by the per-group kernel (per_group[l]); zeros for other
+ * plans.  No reference counterpart: a diagnostic of this library's two kernels. */
+int ffmm_plan_spectral_tiles(ffmm_plan plan, int64_t *staged, int64_t *per_group, int n);
 
 /* ---- serving: continuous batching of solves on one device ----
  * No reference counterpart.  The nearest pieces of the reference are the plan and

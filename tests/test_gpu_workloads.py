@@ -6,7 +6,7 @@ with the generators of the full-size runs (ffmm.workloads):
 * Gaussian mixture + uniform targets, mixed precision with the spectral M2L (the
   4M-source configuration): potential at the sources, potential and gradient at the
   targets, against the oracle and, on 10,000 sampled points, the direct sum (the 1e-6
-  gate);
+  gate); at 400,000 sources, where both Hadamard kernels of the spectral M2L run;
 * StokesHarmonics p = 10 on ellipsoid surfaces, stokeslets + stresslets with the
   normals (the 2M-point configuration): the mixed and FP64 forms against the harmonic
   oracle and the FP64 Stokes direct sum;
@@ -101,6 +101,35 @@ def test_mixture_mixed_spectral_targets(mixture):
     print(f"[workloads] mixed rel L2 vs direct: sources {e[0]:.2e}, targets {e[1]:.2e}, gradient {e[2]:.2e}",
           flush=True)
     assert max(e) <= 1e-6
+
+
+def test_mixture_400k_mixed_spectral_both_hadamard_kernels():
+    """400,000 Gaussian-mixture sources, mixed precision, spectral M2L. At this size a few
+    tiles of 16 sibling groups name more than 200 distinct colleague boxes, so the
+    Hadamard products run in both kernels (the one staging each tile's boxes once, and
+    the per-group one for those tiles); the potential at the sources within 2e-6
+    (max-norm) of the oracle's FP64 FMM on the same tree, and within the 1e-6 rel L2
+    gate of the FP64 direct sum on 3,000 sampled sources."""
+    import ffmm
+    from ffmm import workloads as W
+    n = 400_000
+    pts = W.gaussian_mixture(n, seed=0)
+    tau = W.charges(n, seed=1)
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    print(f"[workloads] mixture n={n}, mixed spectral, both Hadamard kernels", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=64, bbox=BOX, cache=cache, precision="mixed", m2l="spectral")
+    staged, per_group = plan.spectral_tiles()
+    print(f"[workloads] tiles staged {staged.tolist()}, per group {per_group.tolist()}", flush=True)
+    assert staged.sum() > 0 and per_group.sum() > 0
+    plan.build_expansions(tau)
+    u = plan.evaluate_at_sources(ffmm.Potential).cpu().numpy()
+    tree = plan.download_tree()
+    ref = O.FMM(cache.host_operators(), pts, 64, *BOX, perm=tree.perm)
+    u_ref = ref.matvec(tau, m2l="spectral")
+    assert mrd(u, u_ref) < 2e-6
+    rows = np.sort(np.random.default_rng(11).choice(n, 3000, replace=False))
+    assert rel(u[rows], O.direct_self(pts, tau, rows=rows)) <= 1e-6
 
 
 @pytest.mark.parametrize("precision,tol_oracle", [("mixed", 1e-6), ("f64", 1e-11)])
