@@ -331,7 +331,7 @@ void evaluate_at_sources(Plan &plan, double *out, int output) {
 }
 
 // ---------------------------------------------------------------------------
-// targets: locate (src/tree/locate.jl:10-22), bucket_by_node (src/tree/locate.jl:49-68)
+// targets: locate (src/tree/locate.jl:10-22), bucket_by_node (src/tree/locate.jl:49-67)
 // ---------------------------------------------------------------------------
 __global__ void k_locate(const double *__restrict__ t, int64_t nt, double ox, double oy, double oz, double W0,
                          const double *__restrict__ centers, int64_t nn, const uint8_t *__restrict__ leaf,

@@ -14,7 +14,7 @@ error phrases; ``!`` becomes a trailing underscore):
 ``build_expansions(f, sources, tau)`` (src/fmm.jl:306)  ``build_expansions(f, sources, tau)``
 ``build_expansions!(state, f, sources, tau)`` (:271)    ``build_expansions_(state, f, sources, tau)``
 ``evaluate_at_sources(f, st; output)``                  ``evaluate_at_sources(f, st, output=)``
-  (src/evaluate.jl:129-133)
+  (src/evaluate.jl:129-132)
 ``evaluate(f, st, targets; output)`` (src/evaluate.jl:98)             ``evaluate(f, st, targets, output=)``
 ``DevicePlan(m, xs; ppl, bbox, cache, max_levels)``     ``DevicePlan(m, xs, ppl=, bbox=, ...)``
   (src/device_passes.jl:113-134)
@@ -1082,7 +1082,7 @@ def build_expansions(f: FMM, sources, tau) -> FMMState:
 
 
 def evaluate_at_sources(f: FMM, state: FMMState, g: int = 1, output=Potential) -> np.ndarray:
-    """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:104-133): the
+    """``evaluate_at_sources(fmm, state, g=1; output)`` (src/evaluate.jl:104-132): the
     output at every source of group g (1-based) in the original order."""
     G = getattr(f.plan, "ngroups", 1)
     if not 1 <= g <= G:

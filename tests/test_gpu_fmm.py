@@ -193,7 +193,7 @@ def test_determinism_sorted_inputs_linearity_nan_recovery(setup):
 
 def test_host_api_mirror(setup):
     """FMM / build_expansions / evaluate_at_sources / evaluate, as the reference's
-    host driver is called (src/fmm.jl:31-49, 300-314; src/evaluate.jl:98-133)."""
+    host driver is called (src/fmm.jl:31-49, 300-314; src/evaluate.jl:98-132)."""
     s = setup
     ffmm = s["ffmm"]
     f = ffmm.FMM(s["m"], s["pts"], ppl=30, bbox=BOX, cache=s["cache"])

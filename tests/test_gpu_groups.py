@@ -70,7 +70,7 @@ def test_source_and_target_groups():
 
 def test_host_mirror_groups():
     """FMM / build_expansions / evaluate_at_sources(fmm, state, g) with groups
-    (src/fmm.jl:264-298; src/evaluate.jl:104-133)."""
+    (src/fmm.jl:264-298; src/evaluate.jl:104-132)."""
     import ffmm
     rng = np.random.default_rng(32)
     xs1, xs2 = mixed(rng, 3000), uniform(rng, 800)
