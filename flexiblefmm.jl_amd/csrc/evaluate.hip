@@ -313,6 +313,7 @@ static EvalArgs base_args(Plan &plan, double *out) {
   a.out = out;
   if (plan.ops->method == METHOD_HARMONIC) {
     a.tb = plan.ops->tab();
+    a.tb32 = plan.ops->tab32();
     a.p = plan.ops->p;
     a.nd = plan.nd;
     a.hscale = plan.hscale.p;

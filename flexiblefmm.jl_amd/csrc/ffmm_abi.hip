@@ -506,6 +506,8 @@ int ffmm_opcache_create_harmonic(ffmm_context ctx, int kernel, int p, double kap
       }
     for (int m = 1; m <= p; m++) e[m - 1] = std::sqrt((2.0 * m - 1) / (2.0 * m));
     o.htab.upload(tab.data(), tab.size(), ctx->c.stream);
+    const std::vector<float> tab32(tab.begin(), tab.end());
+    o.htab32.upload(tab32.data(), tab32.size(), ctx->c.stream);
     // the far offsets in far-index order (far_offsets, src/util.jl:87-97), for the Stokes coupling
     std::vector<double> offs;
     for (int u = 0; u < 343; u++) {

@@ -218,9 +218,14 @@ struct HarmonicLevel {
 };
 
 // device pointers of the recurrence tables (HarmonicTables, src/operators/harmonics.jl:163-187)
-struct HTab {
-  const double *a, *b, *cz, *cm, *cp, *e;
+// the recurrence tables of the Schmidt harmonics (a, b, cz, cm, cp: ntri each; e: p),
+// in FP64 and, for the mixed form's evaluation, rounded to FP32
+template <class T>
+struct HTabT {
+  const T *a, *b, *cz, *cm, *cp, *e;
 };
+using HTab = HTabT<double>;
+using HTab32 = HTabT<float>;
 
 struct ClassFactors;
 
@@ -234,6 +239,7 @@ struct OpCache {
   double kappa = 0.0;
   bool has_classes = false;
   DBuf<double> htab;                         // a, b, cz, cm, cp (ntri each), e (p)
+  DBuf<float> htab32;                        // the same, rounded to FP32
   DBuf<int8_t> sgn_t;                        // target-side signs [f * N + i] (the cube: = sgn)
   DBuf<double> offs;                         // 316 x 3 far offsets (the Stokes coupling)
   // the operators by width; entries are added, never replaced or removed while the
@@ -250,6 +256,10 @@ struct OpCache {
   HTab tab() const {
     const double *t = htab.p;
     return HTab{t, t + ntri, t + 2 * ntri, t + 3 * ntri, t + 4 * ntri, t + 5 * ntri};
+  }
+  HTab32 tab32() const {
+    const float *t = htab32.p;
+    return HTab32{t, t + ntri, t + 2 * ntri, t + 3 * ntri, t + 4 * ntri, t + 5 * ntri};
   }
   double d = 0.1, w_ref = 0.0;
   bool ready = false;
@@ -481,6 +491,7 @@ struct EvalArgs {
   double *out;
   // harmonic methods
   HTab tb;
+  HTab32 tb32;
   int p, nd;
   const double *hscale, *khat;
   double kappa;

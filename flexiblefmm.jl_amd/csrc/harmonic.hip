@@ -20,30 +20,32 @@ namespace ffmm {
 
 constexpr int HB = 64;
 constexpr int HT_CK = 128;   // sources per LDS tile of the near field
+constexpr int HT_CK32 = 64;  // the same in the mixed form (twice the workgroups per CU)
 constexpr double HC4PI = 0.07957747154594767;    // 1 / (4 pi)
 constexpr double HC8PI = 0.039788735772973836;   // 1 / (8 pi)
 
 __device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 
 // f(n, Re, Im) of the Schmidt regular harmonics of order m, n = m..nmax, at (x, y, z);
-// the sectoral start is the product of m factors
-template <class F>
-__device__ __forceinline__ void chain(const HTab &tb, double x, double y, double z, int m, int nmax, F &&f) {
+// the sectoral start is the product of m factors.  T is the arithmetic type (FP64, or
+// FP32 in the mixed form's evaluation, with the tables rounded to FP32).
+template <class T, class F>
+__device__ __forceinline__ void chain(const HTabT<T> &tb, T x, T y, T z, int m, int nmax, F &&f) {
   if (m > nmax) return;
-  const double r2 = x * x + y * y + z * z;
-  double sr = 1.0, si = 0.0;
+  const T r2 = x * x + y * y + z * z;
+  T sr = 1, si = 0;
   for (int k = 1; k <= m; k++) {
-    const double e = tb.e[k - 1];
-    const double tr = -e * (x * sr - y * si), ti = -e * (x * si + y * sr);
+    const T e = tb.e[k - 1];
+    const T tr = -e * (x * sr - y * si), ti = -e * (x * si + y * sr);
     sr = tr;
     si = ti;
   }
   f(m, sr, si);
-  double pr = 0.0, pi = 0.0, cr = sr, ci = si;
+  T pr = 0, pi = 0, cr = sr, ci = si;
   for (int n = m + 1; n <= nmax; n++) {
     const int t = tri(n, m);
-    const double az = tb.a[t] * z, br = tb.b[t] * r2;
-    const double nr = az * cr - br * pr, ni = az * ci - br * pi;
+    const T az = tb.a[t] * z, br = tb.b[t] * r2;
+    const T nr = az * cr - br * pr, ni = az * ci - br * pi;
     f(n, nr, ni);
     pr = cr;
     pi = ci;
@@ -230,55 +232,61 @@ __global__ void __launch_bounds__(HB) k_s2m_harm(const int32_t *__restrict__ nod
 // ---------------------------------------------------------------------------
 // evaluation
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void get_entry(const double *L, int n, int m, double &re, double &im) {
+template <class T>
+__device__ __forceinline__ void get_entry(const T *L, int n, int m, T &re, T &im) {
   if (m > n) {
-    re = 0.0;
-    im = 0.0;
+    re = 0;
+    im = 0;
   } else if (m == 0) {
     re = L[n * n];
-    im = 0.0;
+    im = 0;
   } else {
     re = L[n * n + 2 * m - 1];
     im = L[n * n + 2 * m];
   }
 }
 // complex coefficient (n, m) of a block, any sign of m, zero outside the triangle
-__device__ __forceinline__ void cget(const double *L, int p, int n, int m, double &re, double &im) {
+template <class T>
+__device__ __forceinline__ void cget(const T *L, int p, int n, int m, T &re, T &im) {
   if (n > p || abs(m) > n) {
-    re = 0.0;
-    im = 0.0;
+    re = 0;
+    im = 0;
   } else if (m == 0) {
     re = L[n * n];
-    im = 0.0;
+    im = 0;
   } else if (m > 0) {
     re = L[n * n + 2 * m - 1];
     im = L[n * n + 2 * m];
   } else {
     const int k = -m;
-    const double sg = (k & 1) ? -1.0 : 1.0;
+    const T sg = (k & 1) ? T(-1) : T(1);
     re = sg * L[n * n + 2 * k - 1];
     im = -sg * L[n * n + 2 * k];
   }
 }
-__device__ __forceinline__ double hcz(const HTab &tb, int n, int m) { return abs(m) > n ? 0.0 : tb.cz[tri(n, abs(m))]; }
-__device__ __forceinline__ double hcm(const HTab &tb, int n, int m) {
-  if (abs(m) > n) return 0.0;
+template <class T>
+__device__ __forceinline__ T hcz(const HTabT<T> &tb, int n, int m) { return abs(m) > n ? T(0) : tb.cz[tri(n, abs(m))]; }
+template <class T>
+__device__ __forceinline__ T hcm(const HTabT<T> &tb, int n, int m) {
+  if (abs(m) > n) return 0;
   return m >= 0 ? tb.cm[tri(n, m)] : tb.cp[tri(n, -m)];
 }
-__device__ __forceinline__ double hcp(const HTab &tb, int n, int m) {
-  if (abs(m) > n) return 0.0;
+template <class T>
+__device__ __forceinline__ T hcp(const HTabT<T> &tb, int n, int m) {
+  if (abs(m) > n) return 0;
   return m >= 0 ? tb.cp[tri(n, m)] : tb.cm[tri(n, -m)];
 }
 // the scaled-gradient coefficient sets at (n1, m1) of a block: _dcoef
 // (src/operators/harmonics_stokes.jl:243-251); out = (x re, x im, y re, y im, z re, z im)
-__device__ __forceinline__ void dcoef(const double *L, int p, const HTab &tb, int n1, int m1, double o[6]) {
+template <class T>
+__device__ __forceinline__ void dcoef(const T *L, int p, const HTabT<T> &tb, int n1, int m1, T o[6]) {
   const int n = n1 + 1;
-  double ar, ai, br, bi, zr, zi;
+  T ar, ai, br, bi, zr, zi;
   cget(L, p, n, m1 + 1, ar, ai);
   cget(L, p, n, m1 - 1, br, bi);
   cget(L, p, n, m1, zr, zi);
-  const double cmv = -hcm(tb, n, m1 + 1), cpv = hcp(tb, n, m1 - 1), czv = hcz(tb, n, m1);
-  const double dmr = cmv * ar, dmi = cmv * ai, dpr = cpv * br, dpi = cpv * bi;
+  const T cmv = -hcm(tb, n, m1 + 1), cpv = hcp(tb, n, m1 - 1), czv = hcz(tb, n, m1);
+  const T dmr = cmv * ar, dmi = cmv * ai, dpr = cpv * br, dpi = cpv * bi;
   o[0] = (dmr + dpr) / 2;
   o[1] = (dmi + dpi) / 2;
   o[2] = -(dpi - dmi) / 2;
@@ -368,13 +376,171 @@ __device__ __forceinline__ void near_term(double d0, double d1, double d2, const
   }
 }
 
+// a near-field source of the mixed form in LDS: FP64 position (the pair difference is
+// formed in FP64), the densities in FP32 except the vectors whose dot product with the
+// pair difference may cancel (a stresslet's normal nu, a dipole): on a surface, d.nu of
+// two near points on it is O(r^2) where d and nu are O(1), and their FP32 rounding
+// would leave an error O(r eps) in it; those stay FP64 and their dot product is formed
+// in FP64.  Its index for the self-exclusion.
+template <int KIND>
+struct HNearSrc32 {   // Yukawa3D: a charge
+  double x, y, z;
+  float f[1];
+  int32_t id;
+};
+template <>
+struct HNearSrc32<0> {   // Laplace3D: a charge, a dipole
+  double x, y, z;
+  double dp[3];
+  float f[1];
+  int32_t id;
+};
+template <>
+struct HNearSrc32<2> {   // Stokeslet3D: the stokeslet f (0..2), the stresslet g (3..5) and nu
+  double x, y, z;
+  double nu[3];
+  float f[6];
+  int32_t id;
+};
+
+// The mixed form's near field of one source e at the FP64 difference (dx, dy, dz),
+// rounded to FP32; 1/r is v_rsq_f32, and the target's own source enters with r^2 = inf,
+// so that 1/r = 0 and its terms vanish.  Component c of the output sums into s[c]; the
+// potential and the Stokes velocity and pressure with Kahan compensation (the running
+// compensation in k[c]), gradients with plain FP32 sums.
+template <bool KAHAN>
+__device__ __forceinline__ void hsum(float &s, float &k, float y) {
+  if (KAHAN) {
+    const float v = y - k;
+    const float t = s + v;
+    k = (t - s) - v;
+    s = t;
+  } else {
+    s += y;
+  }
+}
+
 template <int KIND, int OUT>
+__device__ __forceinline__ void near_term32(double dx, double dy, double dz, const HNearSrc32<KIND> &e, int nd,
+                                            bool keep, float kappa, float s[4], float k[4]) {
+  constexpr float C4 = (float)HC4PI, C8 = (float)HC8PI;
+  const float d0 = (float)dx, d1 = (float)dy, d2 = (float)dz;
+  const float r2 = fmaf(d2, d2, fmaf(d1, d1, d0 * d0));
+  const float ir = __builtin_amdgcn_rsqf(keep ? r2 : INFINITY);
+  if constexpr (KIND == 2) {
+    const float *f = e.f;
+    const float df = d0 * f[0] + d1 * f[1] + d2 * f[2];
+    float dg = 0.f, dn = 0.f;
+    if (nd == 9) {
+      dg = d0 * f[3] + d1 * f[4] + d2 * f[5];
+      dn = (float)fma(dz, e.nu[2], fma(dy, e.nu[1], dx * e.nu[0]));
+    }
+    if (OUT == FFMM_PRESSURE) {
+      // (d.f) / (4 pi r^3); a stresslet (g, nu): ((g.nu) / r^3 - 3 (d.g)(d.nu) / r^5) / (2 pi)
+      // (src/kernels.jl:267-271, src/sources.jl:237-242)
+      const float ir3 = ir * ir * ir;
+      float pr = (2 * C8) * ir3 * df;
+      if (nd == 9) {
+        const float gn = f[3] * (float)e.nu[0] + f[4] * (float)e.nu[1] + f[5] * (float)e.nu[2];
+        pr += (4 * C8) * (gn * ir3 - 3 * dg * dn * ir3 * ir * ir);
+      }
+      hsum<true>(s[0], k[0], pr);
+    } else {
+      // stokeslet (f.d) d / r^3 + f / r, stresslet -6 (d.g)(d.nu) d / r^5, over 8 pi
+      const float a = C8 * ir, ir2 = ir * ir;
+      float b = a * ir2 * df;
+      if (nd == 9) b -= (6 * C8) * dg * dn * (ir2 * ir2 * ir);
+      hsum<true>(s[0], k[0], fmaf(a, f[0], b * d0));
+      hsum<true>(s[1], k[1], fmaf(a, f[1], b * d1));
+      hsum<true>(s[2], k[2], fmaf(a, f[2], b * d2));
+    }
+  } else {
+    float pv = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    if constexpr (KIND == 0) {
+      if (nd == 4) {
+        // a dipole d: potential d.r / (4 pi r^3), gradient (d / r^3 - 3 (d.r) r / r^5) / (4 pi)
+        const float ir3 = ir * ir * ir;
+        const float dr = (float)fma(dz, e.dp[2], fma(dy, e.dp[1], dx * e.dp[0]));
+        const float gs = -3.f * C4 * dr * ir3 * ir * ir;
+        pv = C4 * dr * ir3;
+        g0 = C4 * (float)e.dp[0] * ir3 + gs * d0;
+        g1 = C4 * (float)e.dp[1] * ir3 + gs * d1;
+        g2 = C4 * (float)e.dp[2] * ir3 + gs * d2;
+      }
+    }
+    float v, dv;
+    if (KIND == 0) {
+      v = C4 * ir;
+      dv = -C4 * ir * ir * ir;
+    } else {
+      const float ex = expf(-kappa * (r2 * ir));
+      v = C4 * ex * ir;
+      dv = -C4 * ex * (kappa * ir + ir * ir) * ir;
+    }
+    const float q = e.f[0], dq = dv * q;
+    if (OUT != FFMM_GRADIENT) hsum<true>(s[0], k[0], fmaf(v, q, pv));
+    if (OUT != FFMM_POTENTIAL) {
+      const int o = OUT == FFMM_GRADIENT ? 0 : 1;
+      s[o] += fmaf(dq, d0, g0);
+      s[o + 1] += fmaf(dq, d1, g1);
+      s[o + 2] += fmaf(dq, d2, g2);
+    }
+  }
+}
+
+// one source of the near field into the mixed form's tile
+template <int KIND>
+__device__ __forceinline__ HNearSrc32<KIND> near_src32(const EvalArgs &a, int32_t q) {
+  HNearSrc32<KIND> e;
+  e.x = a.sx[q];
+  e.y = a.sy[q];
+  e.z = a.sz[q];
+  const double *t = a.tau + (int64_t)q * a.nd;
+  if constexpr (KIND == 2) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) e.f[c] = (float)t[c];
+    const bool st = a.nd == 9;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      e.f[3 + c] = st ? (float)t[3 + c] : 0.f;
+      e.nu[c] = st ? t[6 + c] : 0.0;
+    }
+  } else if constexpr (KIND == 0) {
+    e.f[0] = (float)t[0];
+    const bool dip = a.nd == 4;
+#pragma unroll
+    for (int c = 0; c < 3; c++) e.dp[c] = dip ? t[1 + c] : 0.0;
+  } else {
+    e.f[0] = (float)t[0];
+  }
+  e.id = q;
+  return e;
+}
+
+template <class T>
+__device__ __forceinline__ HTabT<T> eval_tables(const EvalArgs &a);
+template <>
+__device__ __forceinline__ HTab eval_tables<double>(const EvalArgs &a) { return a.tb; }
+template <>
+__device__ __forceinline__ HTab32 eval_tables<float>(const EvalArgs &a) { return a.tb32; }
+
+// T = double: the FP64 form.  T = float: the mixed form; the far field (the folds of the
+// leaf's local expansion, rounded to FP32 in LDS) and the near field run in FP32, the
+// target's offset from the leaf centre rounded once after its FP64 scaling, and the
+// near field's FP32 partial sums are folded into FP64 accumulators after every LDS tile
+// (the FP32 evaluation of the mixed form, src/device.jl:186-206 with Float32 locals and
+// densities).
+template <class T, int KIND, int OUT>
 __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   extern __shared__ double sm[];
-  // doubles per tile entry: position and density components (Stokes 9, Laplace a charge
-  // and a dipole, Yukawa a charge)
+  constexpr bool F32 = sizeof(T) == 4;
+  // FP64: doubles per tile entry, position and density components (Stokes 9, Laplace a
+  // charge and a dipole, Yukawa a charge); FP32: an HNearSrc32 with ND densities
   constexpr int TW = KIND == 2 ? 12 : KIND == 0 ? 7 : 4;
-  __shared__ double tile[HT_CK * TW];
+  constexpr size_t TILE_BYTES = F32 ? HT_CK32 * sizeof(HNearSrc32<KIND>) : HT_CK * TW * sizeof(double);
+  __shared__ __align__(16) char tile_raw[TILE_BYTES];
+  double *tile = reinterpret_cast<double *>(tile_raw);
+  HNearSrc32<KIND> *tile32 = reinterpret_cast<HNearSrc32<KIND> *>(tile_raw);
   const int s = blockIdx.x;
   const int i = a.leaf_nodes[s];
   const bool self = a.tptr == nullptr;
@@ -384,12 +550,12 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
   const int p = a.p, N = (p + 1) * (p + 1);
   constexpr int nb = KIND == 2 ? 4 : 1;
   const int NB = nb * N;
-  const HTab tb = a.tb;
-  double *L = sm;                       // the leaf's local expansion
-  double *uw = sm + NB;                 // per lane: radial weights to p (Yukawa potential)
-  double *ug = uw + HB * (p + 2);       // per lane: radial weights to p + 1 (Yukawa gradient)
+  const HTabT<T> tb = eval_tables<T>(a);
+  T *L = reinterpret_cast<T *>(sm);                   // the leaf's local expansion
+  double *uw = sm + (NB * sizeof(T) + 7) / 8;         // per lane: radial weights to p (Yukawa potential)
+  double *ug = uw + HB * (p + 2);                     // per lane: radial weights to p + 1 (Yukawa gradient)
   const double *Lg = a.Lc + (int64_t)s * NB;
-  for (int k = threadIdx.x; k < NB; k += HB) L[k] = Lg[k];
+  for (int k = threadIdx.x; k < NB; k += HB) L[k] = (T)Lg[k];
   __syncthreads();
   const double c0 = a.centers[i], c1 = a.centers[a.nn + i], c2 = a.centers[2 * a.nn + i];
   const int lv = a.level[i];
@@ -412,22 +578,23 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
     const double y = self ? a.sy[jj] : a.ty[jj];
     const double z = self ? a.sz[jj] : a.tz[jj];
     const double xh = (x - c0) * sc, yh = (y - c1) * sc, zh = (z - c2) * sc;
+    const T xt = (T)xh, yt = (T)yh, zt = (T)zh;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (KIND == 2) {
       // velocity 1/2 [Phi - G x + grad Psi] (_stokes_fold1 / _stokes_output,
       // src/operators/harmonics_stokes.jl:253-301, 321-325), orders split over replicas
-      double Phi[3] = {0, 0, 0}, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, gP[3] = {0, 0, 0};
+      T Phi[3] = {0, 0, 0}, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, gP[3] = {0, 0, 0};
       for (int m = rep; m <= p; m += R)
-        chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
-          const double wt = m == 0 ? 1.0 : 2.0;
+        chain(tb, xt, yt, zt, m, p, [&](int n, T vr, T vi) {
+          const T wt = m == 0 ? T(1) : T(2);
 #pragma unroll
           for (int k = 0; k < 3; k++) {
-            double cr, ci;
+            T cr, ci;
             cget(L + k * N, p, n, m, cr, ci);
             Phi[k] += wt * (cr * vr + ci * vi);
           }
           if (n < p) {
-            double o[6];
+            T o[6];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
               dcoef(L + k * N, p, tb, n, m, o);
@@ -441,12 +608,12 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
         });
       if (OUT == FFMM_PRESSURE) {
         // -s tr(grad Phi) (_stokes_output(::Pressure), src/operators/harmonics_stokes.jl:348-351)
-        acc[0] = -sc * (G[0][0] + G[1][1] + G[2][2]);
+        acc[0] = -sc * (double)(G[0][0] + G[1][1] + G[2][2]);
       } else {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const double gx = G[c][0] * xh + G[c][1] * yh + G[c][2] * zh;
-          acc[c] = (Phi[c] - gx + gP[c]) / 2;
+          const T gx = G[c][0] * xt + G[c][1] * yt + G[c][2] * zt;
+          acc[c] = (double)((Phi[c] - gx + gP[c]) / 2);
         }
       }
     } else {
@@ -458,40 +625,40 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
       }
       if (OUT != FFMM_GRADIENT) {
         // _harmonic_potential (src/operators/harmonics.jl:455-462)
-        double pot = 0.0;
+        T pot = 0;
         for (int m = rep; m <= p; m += R)
-          chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
-            double lr, li;
+          chain(tb, xt, yt, zt, m, p, [&](int n, T vr, T vi) {
+            T lr, li;
             get_entry(L, n, m, lr, li);
-            const double wn = KIND == 1 ? up[n] : 1.0;
+            const T wn = KIND == 1 ? (T)up[n] : T(1);
             pot += m == 0 ? wn * (lr * vr) : wn * (2 * (lr * vr + li * vi));
           });
-        acc[0] = pot;
+        acc[0] = (double)pot;
       }
       if (OUT != FFMM_POTENTIAL) {
         // _harmonic_gradient (src/operators/harmonics.jl:463-485); Yukawa adds
         // x * (potential with the radial weights) (src/operators/harmonics_yukawa.jl:306-313)
-        double g0 = 0.0, g1 = 0.0, g2 = 0.0, pr = 0.0;
+        T g0 = 0, g1 = 0, g2 = 0, pr = 0;
         for (int m1 = rep; m1 <= p - 1; m1 += R)
-          chain(tb, xh, yh, zh, m1, p - 1, [&](int n1, double vr, double vi) {
+          chain(tb, xt, yt, zt, m1, p - 1, [&](int n1, T vr, T vi) {
             const int n = n1 + 1;
-            const double wn = KIND == 1 ? ugr[n] : 1.0;
+            const T wn = KIND == 1 ? (T)ugr[n] : T(1);
             if (m1 == 0) {
-              double lr, li;
+              T lr, li;
               get_entry(L, n, 1, lr, li);
-              const double c = tb.cm[tri(n, 1)];
-              const double ax = -c * lr, ay = -c * li, az = tb.cz[tri(n, 0)] * L[n * n];
+              const T c = tb.cm[tri(n, 1)];
+              const T ax = -c * lr, ay = -c * li, az = tb.cz[tri(n, 0)] * L[n * n];
               g0 += wn * (ax * vr);
               g1 += wn * (ay * vr);
               g2 += wn * (az * vr);
             } else {
-              double ur, ui, dr, di, zr, zi;
+              T ur, ui, dr, di, zr, zi;
               get_entry(L, n, m1 + 1, ur, ui);
               get_entry(L, n, m1 - 1, dr, di);
-              const double cu = tb.cm[tri(n, m1 + 1)] / 2, cd = tb.cp[tri(n, m1 - 1)] / 2;
-              const double axr = -cu * ur + cd * dr, axi = -cu * ui + cd * di;
-              const double ayr = -(cu * ui + cd * di), ayi = cu * ur + cd * dr;
-              const double czv = tb.cz[tri(n, m1)];
+              const T cu = tb.cm[tri(n, m1 + 1)] / 2, cd = tb.cp[tri(n, m1 - 1)] / 2;
+              const T axr = -cu * ur + cd * dr, axi = -cu * ui + cd * di;
+              const T ayr = -(cu * ui + cd * di), ayi = cu * ur + cd * dr;
+              const T czv = tb.cz[tri(n, m1)];
               get_entry(L, n, m1, zr, zi);
               g0 += (2 * wn) * (axr * vr + axi * vi);
               g1 += (2 * wn) * (ayr * vr + ayi * vi);
@@ -500,38 +667,58 @@ __global__ void __launch_bounds__(HB) k_eval_harm(EvalArgs a) {
           });
         if (KIND == 1) {
           for (int m = rep; m <= p; m += R)
-            chain(tb, xh, yh, zh, m, p, [&](int n, double vr, double vi) {
-              double lr, li;
+            chain(tb, xt, yt, zt, m, p, [&](int n, T vr, T vi) {
+              T lr, li;
               get_entry(L, n, m, lr, li);
-              const double wn = kh * kh * ugr[n + 1] / (2 * n + 3);
+              const T wn = (T)(kh * kh * ugr[n + 1] / (2 * n + 3));
               pr += m == 0 ? wn * (lr * vr) : wn * (2 * (lr * vr + li * vi));
             });
         }
         const int o = OUT == FFMM_GRADIENT ? 0 : 1;
-        acc[o] = (g0 + xh * pr) * sc;
-        acc[o + 1] = (g1 + yh * pr) * sc;
-        acc[o + 2] = (g2 + zh * pr) * sc;
+        acc[o] = (double)(g0 + xt * pr) * sc;
+        acc[o + 1] = (double)(g1 + yt * pr) * sc;
+        acc[o + 2] = (double)(g2 + zt * pr) * sc;
       }
     }
     // the near field of the 27 colleagues (src/device.jl:89-100), their sources and
     // densities as one list staged through LDS tiles (broadcast reads)
     const int32_t skip = self ? j : -1;
-    for (int32_t q0 = 0; q0 < nsrc; q0 += HT_CK) {
-      const int nq = min(HT_CK, nsrc - q0);
-      __syncthreads();
-      for (int k = threadIdx.x; k < nq; k += HB) {
-        const int32_t q = near_source(nl, q0 + k);
-        double *e = tile + k * TW;
-        e[0] = a.sx[q];
-        e[1] = a.sy[q];
-        e[2] = a.sz[q];
-        for (int c = 0; c < a.nd; c++) e[3 + c] = a.tau[(int64_t)q * a.nd + c];
-        tile_id[k] = q;
+    if constexpr (!F32) {
+      for (int32_t q0 = 0; q0 < nsrc; q0 += HT_CK) {
+        const int nq = min(HT_CK, nsrc - q0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nq; k += HB) {
+          const int32_t q = near_source(nl, q0 + k);
+          double *e = tile + k * TW;
+          e[0] = a.sx[q];
+          e[1] = a.sy[q];
+          e[2] = a.sz[q];
+          for (int c = 0; c < a.nd; c++) e[3 + c] = a.tau[(int64_t)q * a.nd + c];
+          tile_id[k] = q;
+        }
+        __syncthreads();
+        for (int k = rep; k < nq; k += R) {
+          const double *e = tile + k * TW;
+          near_term<KIND, OUT>(x - e[0], y - e[1], z - e[2], e + 3, a.nd, tile_id[k] != skip, a.kappa, acc);
+        }
       }
-      __syncthreads();
-      for (int k = rep; k < nq; k += R) {
-        const double *e = tile + k * TW;
-        near_term<KIND, OUT>(x - e[0], y - e[1], z - e[2], e + 3, a.nd, tile_id[k] != skip, a.kappa, acc);
+    } else {
+      const float kap = (float)a.kappa;
+      float ks[4] = {0.f, 0.f, 0.f, 0.f}, kc[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int32_t q0 = 0; q0 < nsrc; q0 += HT_CK32) {
+        const int nq = min(HT_CK32, nsrc - q0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nq; k += HB) tile32[k] = near_src32<KIND>(a, near_source(nl, q0 + k));
+        __syncthreads();
+        for (int k = rep; k < nq; k += R) {
+          const HNearSrc32<KIND> e = tile32[k];
+          near_term32<KIND, OUT>(x - e.x, y - e.y, z - e.z, e, a.nd, e.id != skip, kap, ks, kc);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          acc[c] += (double)ks[c] - (double)kc[c];
+          ks[c] = kc[c] = 0.f;
+        }
       }
     }
     for (int off = ntp; off < HB; off <<= 1)
@@ -584,26 +771,32 @@ void harmonic_s2m_level(Plan &plan, int l) {
   FFMM_LAUNCH_CHECK();
 }
 
-void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output) {
-  OpCache &o = *plan.ops;
-  const int64_t nleaves = plan.leaf_off[plan.L];
-  if (nleaves == 0) return;
-  const size_t lds = sizeof(double) * ((size_t)o.Ns + (o.kernel == FFMM_KERNEL_YUKAWA ? 2 * (size_t)HB * (o.p + 2) : 0));
-  hipStream_t s = plan.ctx->stream;
-  const unsigned g = (unsigned)nleaves;
+template <class T>
+static void launch_eval_harm(const OpCache &o, const EvalArgs &a, int output, unsigned g, hipStream_t s) {
+  const size_t lds = ((size_t)o.Ns * sizeof(T) + 7) / 8 * 8 +
+                     (o.kernel == FFMM_KERNEL_YUKAWA ? 2 * sizeof(double) * HB * (o.p + 2) : 0);
   if (o.kernel == FFMM_KERNEL_STOKESLET) {
-    if (output == FFMM_PRESSURE) k_eval_harm<2, FFMM_PRESSURE><<<g, HB, lds, s>>>(a);
-    else k_eval_harm<2, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    if (output == FFMM_PRESSURE) k_eval_harm<T, 2, FFMM_PRESSURE><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<T, 2, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
   } else if (o.kernel == FFMM_KERNEL_LAPLACE) {
-    if (output == FFMM_POTENTIAL) k_eval_harm<0, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
-    else if (output == FFMM_GRADIENT) k_eval_harm<0, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
-    else k_eval_harm<0, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
+    if (output == FFMM_POTENTIAL) k_eval_harm<T, 0, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    else if (output == FFMM_GRADIENT) k_eval_harm<T, 0, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<T, 0, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
   } else {
-    if (output == FFMM_POTENTIAL) k_eval_harm<1, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
-    else if (output == FFMM_GRADIENT) k_eval_harm<1, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
-    else k_eval_harm<1, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
+    if (output == FFMM_POTENTIAL) k_eval_harm<T, 1, FFMM_POTENTIAL><<<g, HB, lds, s>>>(a);
+    else if (output == FFMM_GRADIENT) k_eval_harm<T, 1, FFMM_GRADIENT><<<g, HB, lds, s>>>(a);
+    else k_eval_harm<T, 1, FFMM_POTENTIAL_GRADIENT><<<g, HB, lds, s>>>(a);
   }
   FFMM_LAUNCH_CHECK();
+}
+
+// the mixed form evaluates in FP32 (k_eval_harm<float>), the FP64 form in FP64
+void launch_eval_harmonic(Plan &plan, const EvalArgs &a, int output) {
+  const int64_t nleaves = plan.leaf_off[plan.L];
+  if (nleaves == 0) return;
+  const unsigned g = (unsigned)nleaves;
+  if (plan.precision == FFMM_PREC_MIXED) launch_eval_harm<float>(*plan.ops, a, output, g, plan.ctx->stream);
+  else launch_eval_harm<double>(*plan.ops, a, output, g, plan.ctx->stream);
 }
 
 }  // namespace ffmm

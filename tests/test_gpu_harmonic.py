@@ -230,37 +230,53 @@ def test_harmonic_sorted_densities_linearity_and_errors():
         ps.evaluate_at_sources(ffmm.Gradient)
 
 
-@pytest.mark.parametrize("kind", ["yukawa", "stokes"])
+@pytest.mark.parametrize("kind", ["yukawa", "stokes", "laplace-dipoles"])
 def test_harmonic_mixed_form(kind):
-    """The mixed form of the harmonic methods: the class M2L as FP32 gathers, sgemm (FP32
-    MFMA) and scatters into the FP64 expansions; S2M, M2M, L2L and the evaluation FP64.
-    Against the FP64 plan on the same tree, and against direct sums."""
+    """The mixed form of the harmonic methods: the class M2L on FP32 MFMA, and the
+    evaluation in FP32 (the folds of the local expansion, the near field with FP64 pair
+    differences and Kahan-compensated FP32 sums folded into FP64 per tile); S2M, M2M and
+    L2L FP64. Against the FP64 plan on the same tree (1e-6), at the sources and at
+    targets, and against direct sums (the truncation of p = 10)."""
     import ffmm
     rng = np.random.default_rng(14)
     n = 5000
     pts = mixed(rng, n)
+    tgt = uniform(rng, 400)
     if kind == "yukawa":
         m = ffmm.SphericalHarmonics(ffmm.Yukawa3D(5.0), p=10)
-        dens, out = rng.standard_normal(n), ffmm.PotentialGradient
-    else:
+        dens, outs = rng.standard_normal(n), [ffmm.PotentialGradient, ffmm.Potential, ffmm.Gradient]
+    elif kind == "stokes":
         m = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=10)
         dens = rng.standard_normal((n, 9))
         dens[:, 6:] /= np.linalg.norm(dens[:, 6:], axis=1)[:, None]
-        out = ffmm.Velocity
+        outs = [ffmm.Velocity, ffmm.Pressure]
+    else:
+        m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=10)
+        dens, outs = rng.standard_normal((n, 4)), [ffmm.PotentialGradient, ffmm.Potential, ffmm.Gradient]
     cache = ffmm.OperatorCache(m)
     print(f"[harmonic] mixed form {kind} p=10", flush=True)
     p64 = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache)
     p32 = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=cache, precision="mixed")
-    u64 = p64.matvec(dens, out).cpu().numpy()
-    u32 = p32.matvec(dens, out).cpu().numpy()
-    assert rel(u32, u64) < 1e-6
+    for out in outs:
+        u64 = p64.matvec(dens, out).cpu().numpy()
+        u32 = p32.matvec(dens, out).cpu().numpy()
+        v64 = p64.evaluate(tgt, out).cpu().numpy()
+        v32 = p32.evaluate(tgt, out).cpu().numpy()
+        print(f"[harmonic] mixed {kind} output {out}: {rel(u32, u64):.2e} {rel(v32, v64):.2e}", flush=True)
+        assert rel(u32, u64) < 1e-6 and rel(v32, v64) < 1e-6
     rows = rng.choice(n, 300, replace=False)
+    u32 = p32.matvec(dens, outs[0]).cpu().numpy()
     if kind == "stokes":
         ud = HO.direct("stokes", pts, dens, pts[rows], skip=rows)
         assert rel(u32[rows], ud) < 1e-4
-    else:
+        pd = HO.direct("stokes", pts, dens, pts[rows], 3, skip=rows)
+        assert rel(p32.matvec(dens, ffmm.Pressure).cpu().numpy()[rows], pd) < 1e-3
+    elif kind == "yukawa":
         ud = HO.direct("yukawa", pts, dens, pts[rows], skip=rows, kappa=5.0)
         assert rel(u32[rows, 0], np.ravel(ud)) < 2e-5     # the truncation error of p = 10
+    else:
+        ud = HO.direct("laplace", pts, dens, pts[rows], 2, skip=rows)
+        assert rel(u32[rows], ud) < 3e-5
 
 
 @pytest.mark.parametrize("nd", [3, 9])
@@ -320,35 +336,45 @@ def test_class_m2l_chunks(case, monkeypatch):
     assert rel(many, one) < (1e-12 if case == "laplace-f64" else 1e-6)
 
 
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
 @pytest.mark.parametrize("n", [1, 2, 37])
-def test_dipoles_and_pressure_small_cases(n):
+def test_dipoles_and_pressure_small_cases(n, precision):
     """Laplace dipoles and the Stokes pressure with 1, 2 and 37 points (one leaf: every
     interaction is in the near field) against the FP64 direct sums, at targets and at
-    the sources (a point skips itself)."""
+    the sources (a point skips itself; one point alone gets exact zeros). The mixed
+    form's FP32 near field is held to 1e-6."""
     import ffmm
     rng = np.random.default_rng(16 + n)
     pts = uniform(rng, n)
     tgt = uniform(rng, 20)
-    print(f"[harmonic] small dipole / pressure n={n}", flush=True)
+    tol = 1e-12 if precision == "f64" else 1e-6
+    print(f"[harmonic] small dipole / pressure n={n} {precision}", flush=True)
     m = ffmm.SphericalHarmonics(ffmm.Laplace3D(), p=6)
-    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(m))
+    plan = ffmm.DevicePlan(m, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(m), precision=precision)
     dens = rng.standard_normal((n, 4))
     u = plan.matvec(dens, ffmm.PotentialGradient).cpu().numpy().reshape(n, 4)
     v = plan.evaluate(tgt, ffmm.PotentialGradient).cpu().numpy()
     vd = HO.direct("laplace", pts, dens, tgt, 2)
-    assert rel(v, vd) < 1e-12
+    assert rel(v, vd) < tol
     if n > 1:
-        assert rel(u, HO.direct("laplace", pts, dens, pts, 2, skip=np.arange(n))) < 1e-12
+        assert rel(u, HO.direct("laplace", pts, dens, pts, 2, skip=np.arange(n))) < tol
     else:
         assert np.all(u == 0.0)
     ms = ffmm.StokesHarmonics(ffmm.Stokeslet3D(), p=4)
-    ps = ffmm.DevicePlan(ms, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(ms))
+    ps = ffmm.DevicePlan(ms, pts, ppl=40, bbox=BOX, cache=ffmm.OperatorCache(ms), precision=precision)
     ds = rng.standard_normal((n, 9))
     ds[:, 6:] /= np.linalg.norm(ds[:, 6:], axis=1)[:, None]
     pu = ps.matvec(ds, ffmm.Pressure).cpu().numpy().reshape(n)
     pv = ps.evaluate(tgt, ffmm.Pressure).cpu().numpy()
-    assert rel(pv, HO.direct("stokes", pts, ds, tgt, 3)) < 1e-12
+    assert rel(pv, HO.direct("stokes", pts, ds, tgt, 3)) < tol
     if n > 1:
-        assert rel(pu, HO.direct("stokes", pts, ds, pts, 3, skip=np.arange(n))) < 1e-12
+        assert rel(pu, HO.direct("stokes", pts, ds, pts, 3, skip=np.arange(n))) < tol
     else:
         assert np.all(pu == 0.0)
+    vu = ps.matvec(ds, ffmm.Velocity).cpu().numpy().reshape(n, 3)
+    vv = ps.evaluate(tgt, ffmm.Velocity).cpu().numpy()
+    assert rel(vv, HO.direct("stokes", pts, ds, tgt)) < tol
+    if n > 1:
+        assert rel(vu, HO.direct("stokes", pts, ds, pts, skip=np.arange(n))) < tol
+    else:
+        assert np.all(vu == 0.0)
