@@ -9,8 +9,8 @@ device-resident points: the device tree and its lists (the plan), the upward and
 downward passes, the potential at every source, and the potential and gradient at
 every target.  value = sources / second of one step.
 
-After the timed region: the relative L2 error against the FP64 direct sum on 10,000
-sampled points (5,000 sources, 5,000 targets), the roofline of the dominant kernel
+After the timed region: the relative L2 error against the FP64 direct sum at 10,000
+sampled sources (potential) and 10,000 sampled targets (potential and gradient), the roofline of the dominant kernel
 from its live HIP-event duration, and the CPU baseline: the oracle (the restated
 reference CPU algorithm) timed on the host cores on a bounded sample.
 
