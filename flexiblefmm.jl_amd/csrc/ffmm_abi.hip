@@ -1058,7 +1058,7 @@ void plan_shrink(Plan &p) {
   for (DBuf<double> *b : {&p.Mp, &p.Pl, &p.Lc, &p.tau, &p.pM, &p.tup, &p.Mup, &p.tM, &p.G, &p.tdn, &p.Gc, &p.Yc,
                           &p.spec_S, &p.spec_U, &p.tgt_sorted})
     b->release();
-  for (DBuf<float> *b : {&p.Lc32, &p.tau32, &p.Gc32, &p.Yc32, &p.pM32, &p.B32, &p.tM32, &p.spec_S32, &p.spec_U32})
+  for (DBuf<float> *b : {&p.Lc32, &p.tau32, &p.Gc32, &p.Yc32, &p.pM32, &p.Mp32, &p.tM32, &p.spec_S32, &p.spec_U32})
     b->release();
   for (DBuf<int32_t> *b : {&p.tgt_node, &p.tgt_leafslot, &p.tgt_order, &p.tgt_ptr}) b->release();
   p.tgt_keys.release();

@@ -358,7 +358,7 @@ struct Plan {
   // work buffers
   DBuf<double> pM, tup, Mup, tM, G, tdn, Gc, Yc;
   DBuf<float> Gc32, Yc32, pM32;
-  DBuf<float> B32, tM32;   // the mixed M2M: children block and parents in FP32
+  DBuf<float> Mp32, tM32;  // the cube's mixed M2M: an FP32 mirror of Mp (written beside it), the parents' products
   // spectral M2L workspace
   DBuf<double> spec_S, spec_U;
   DBuf<float> spec_S32, spec_U32;

@@ -159,16 +159,19 @@ __global__ void k_gather_cols(double *__restrict__ dst, const double *__restrict
   dst[g] = src[(int64_t)cols[q] * rows + i];
 }
 // dst[i, cols[q]] = lam[level[cols[q]]] * src[i, q]: the columns of several levels at once,
-// each scaled by its level's width factor
+// each scaled by its level's width factor; dst32 (if given) takes the same value in FP32
 __global__ void k_scatter_cols_lam(double *__restrict__ dst, const double *__restrict__ src,
                                    const int32_t *__restrict__ cols, int rows, int64_t ncols,
-                                   const int32_t *__restrict__ level, const double *__restrict__ lam) {
+                                   const int32_t *__restrict__ level, const double *__restrict__ lam,
+                                   float *__restrict__ dst32) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rows * ncols) return;
   int64_t q = g / rows;
   int i = (int)(g % rows);
   const int32_t c = cols[q];
-  dst[(int64_t)c * rows + i] = lam[level[c]] * src[g];
+  const double v = lam[level[c]] * src[g];
+  dst[(int64_t)c * rows + i] = v;
+  if (dst32) dst32[(int64_t)c * rows + i] = (float)v;
 }
 // dst[i, q] = lam[level[cols[q]]] * src[i, cols[q]]
 __global__ void k_gather_cols_lam(double *__restrict__ dst, const double *__restrict__ src,
@@ -181,20 +184,24 @@ __global__ void k_gather_cols_lam(double *__restrict__ dst, const double *__rest
   const int32_t c = cols[q];
   dst[g] = lam[level[c]] * src[(int64_t)c * rows + i];
 }
-// the parents of the sibling groups starting at node `first`: dst[i, parent[first + 8g]] = alpha src[i, g]
+// the parents of the sibling groups starting at node `first`: dst[i, parent[first + 8g]] =
+// alpha src[i, g]; dst32 (if given) takes the same value in FP32
 template <class T>
 __global__ void k_scatter_parents(double *__restrict__ dst, const T *__restrict__ src,
-                                  const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng, double alpha) {
+                                  const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng, double alpha,
+                                  float *__restrict__ dst32) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)rows * ng) return;
   int64_t q = g / rows;
   int i = (int)(g % rows);
-  dst[(int64_t)parent[first + NCHILD * q] * rows + i] = alpha * (double)src[g];
+  const int64_t o = (int64_t)parent[first + NCHILD * q] * rows + i;
+  const double v = alpha * (double)src[g];
+  dst[o] = v;
+  if (dst32) dst32[o] = (float)v;
 }
-// dst = (float) src
-__global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+__global__ void k_fill32(float *__restrict__ a, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (float)src[i];
+  if (i < n) a[i] = 0.f;
 }
 __global__ void k_gather_parents(double *__restrict__ dst, const double *__restrict__ src,
                                  const int32_t *__restrict__ parent, int64_t first, int rows, int64_t ng) {
@@ -649,7 +656,7 @@ void allocate_pass_buffers(Plan &plan) {
   }
   plan.tM.resize((size_t)Ns * ng);
   if (o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED) {
-    plan.B32.resize((size_t)NCHILD * Ns * ng);
+    plan.Mp32.resize((size_t)Ns * plan.nnodes);
     plan.tM32.resize((size_t)Ns * ng);
   }
   plan.G.resize((size_t)Ns * std::max(nq, ng));
@@ -690,9 +697,17 @@ void upward_pass(Plan &plan) {
   OpCache &o = *plan.ops;
   hipStream_t s = ctx.stream;
   const int Ns = o.Ns, Nl = o.Nl;
-  // every column is written before it is read except by M2M of empty groups: clear
+  // every column is written before it is read except by M2M of empty groups: clear.  The
+  // cube's mixed form keeps an FP32 mirror of the multipoles, written beside them by the
+  // UC2E and M2M scatters, that its FP32 M2M GEMMs read directly
+  const bool mirror = o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED;
+  float *Mp32 = mirror ? plan.Mp32.p : nullptr;
   k_fill<<<nblocks(plan.Mp.n, TB), TB, 0, s>>>(plan.Mp.p, plan.Mp.n, 0.0);
   FFMM_LAUNCH_CHECK();
+  if (mirror) {
+    k_fill32<<<nblocks(plan.Mp32.n, TB), TB, 0, s>>>(Mp32, plan.Mp32.n);
+    FFMM_LAUNCH_CHECK();
+  }
   const int l2 = std::min(2, plan.L);
   const int64_t nq = plan.s2m_off[plan.L] - plan.s2m_off[l2];
   if (o.method == METHOD_CUBE && nq > 0) {
@@ -713,7 +728,7 @@ void upward_pass(Plan &plan) {
     gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
     gemm(ctx, Ns, (int)nq, o.r_up, 1.0, o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
     k_scatter_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq,
-                                                                    plan.level.p, plan.lam_d.p);
+                                                                    plan.level.p, plan.lam_d.p, Mp32);
     FFMM_LAUNCH_CHECK();
   }
   for (int l = plan.L - 1; l >= 2; l--) {
@@ -725,19 +740,16 @@ void upward_pass(Plan &plan) {
       const int64_t ng = plan.nl[l + 1] / NCHILD;
       const double *B = plan.Mp.p + plan.lo[l + 1] * Ns;
       const double *M2M = o.method == METHOD_HARMONIC ? plan.hl[l]->M2M.p : o.M2M.p;
-      if (o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED) {
-        // the mixed form: M2M as an FP32 GEMM (its rounding is not amplified: the
-        // multipoles feed M2L, never a pseudo-inverse)
-        const int64_t nB = (int64_t)NCHILD * Ns * ng;
-        k_to_f32<<<nblocks(nB, TB), TB, 0, s>>>(plan.B32.p, B, nB);
-        FFMM_LAUNCH_CHECK();
-        gemm32(ctx, Ns, (int)ng, NCHILD * Ns, o.M2M32.p, Ns, plan.B32.p, NCHILD * Ns, plan.tM32.p, Ns);
+      if (mirror) {
+        // the mixed form: M2M as an FP32 GEMM over the children's FP32 mirror (its rounding
+        // is not amplified: the multipoles feed M2L, never a pseudo-inverse)
+        gemm32(ctx, Ns, (int)ng, NCHILD * Ns, o.M2M32.p, Ns, Mp32 + plan.lo[l + 1] * Ns, NCHILD * Ns, plan.tM32.p, Ns);
         k_scatter_parents<float><<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM32.p, plan.parent.p,
-                                                                              plan.lo[l + 1], Ns, ng, 1.0);
+                                                                              plan.lo[l + 1], Ns, ng, 1.0, Mp32);
       } else {
         gemm(ctx, Ns, (int)ng, NCHILD * Ns, 1.0, M2M, Ns, B, NCHILD * Ns, 0.0, plan.tM.p, Ns);
         k_scatter_parents<double><<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM.p, plan.parent.p,
-                                                                               plan.lo[l + 1], Ns, ng, 1.0);
+                                                                               plan.lo[l + 1], Ns, ng, 1.0, nullptr);
       }
       FFMM_LAUNCH_CHECK();
     }

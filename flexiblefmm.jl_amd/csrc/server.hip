@@ -58,7 +58,7 @@ size_t plan_bytes(const Plan &p) {
          cap_bytes(li.cstart);
   s += cap_bytes(p.Mp) + cap_bytes(p.Pl) + cap_bytes(p.Lc) + cap_bytes(p.Lc32) + cap_bytes(p.tau) + cap_bytes(p.tau32);
   s += cap_bytes(p.pM) + cap_bytes(p.tup) + cap_bytes(p.Mup) + cap_bytes(p.tM) + cap_bytes(p.G) + cap_bytes(p.tdn) +
-       cap_bytes(p.Gc) + cap_bytes(p.Yc) + cap_bytes(p.Gc32) + cap_bytes(p.Yc32) + cap_bytes(p.pM32) + cap_bytes(p.B32) + cap_bytes(p.tM32);
+       cap_bytes(p.Gc) + cap_bytes(p.Yc) + cap_bytes(p.Gc32) + cap_bytes(p.Yc32) + cap_bytes(p.pM32) + cap_bytes(p.Mp32) + cap_bytes(p.tM32);
   s += cap_bytes(p.spec_S) + cap_bytes(p.spec_U) + cap_bytes(p.spec_S32) + cap_bytes(p.spec_U32);
   s += cap_bytes(p.Zc) + cap_bytes(p.Zc32) + cap_bytes(p.gin) + cap_bytes(p.gout);
   for (const auto &b : p.spec_gptr) s += cap_bytes(b);
