@@ -387,6 +387,14 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     std::vector<float> soa32(soa.begin(), soa.end());
     o.large32.upload(soa32.data(), soa32.size(), s);
     o.Uh_up.upload(Uh_up, (size_t)r_up * Nl, s);
+    std::vector<double> uhp;
+    if (r_up > 240 && r_up <= 304) {
+      const size_t nk = (size_t)(Nl + UC2E_PK - 1) / UC2E_PK * UC2E_PK + 4 * UC2E_PKX;
+      uhp.assign(nk * UC2E_PROWS, 0.0);
+      for (int k = 0; k < Nl; k++)
+        for (int i = 0; i < r_up; i++) uhp[(size_t)k * UC2E_PROWS + i] = Uh_up[(size_t)k * r_up + i];
+      o.Uh_up_pad.upload(uhp.data(), uhp.size(), s);
+    }
     o.Uh_dn.upload(Uh_dn, (size_t)r_dn * Ns, s);
     std::vector<double> vs((size_t)Ns * r_up);
     for (int j = 0; j < r_up; j++)

@@ -228,6 +228,7 @@ using HTab = HTabT<double>;
 using HTab32 = HTabT<float>;
 
 struct ClassFactors;
+constexpr int UC2E_PROWS = 320, UC2E_PK = 64, UC2E_PKX = 2;   // rows, k rounding, extra k-steps
 
 struct OpCache {
   Context *ctx = nullptr;
@@ -268,6 +269,9 @@ struct OpCache {
   DBuf<double> large;                        // 3 x Nl SoA: x[Nl], y[Nl], z[Nl]
   DBuf<float> large32;
   DBuf<double> Uh_up, Vs_up, Uh_dn, Vs_dn;   // Vs = V diag(invS)
+  // the fused S2M + UC2E kernel's Uh (ranks 241..304): UC2E_PROWS rows per check point,
+  // Nl rounded up to UC2E_PK and UC2E_PKX k-steps more, zero-padded (no guards on its loads)
+  DBuf<double> Uh_up_pad;
   DBuf<double> M2M, L2L, class_ops;
   DBuf<float> M2M32, class_ops32;   // FP32 copies for the mixed form
   DBuf<int32_t> far_class, idx, inv;         // idx/inv: [f * Ns + i]

@@ -140,6 +140,277 @@ __global__ void __launch_bounds__(S2M_TB) k_s2m32(const int32_t *__restrict__ no
   }
 }
 
+// S2M fused with the first factor of UC2E: tup[:, q] = Uh pM[:, q] for the childless
+// sourced nodes q (src/device_passes.jl:378-403 and 496-502: s2m_accumulate! into the
+// check potentials, then the factored pseudo-inverse's first GEMM), without the check
+// potentials pM ever reaching device memory (Nl doubles per node: 1352 at p = 8, 4.9 GB
+// at 451k nodes, written once and read once by a library GEMM).
+//
+// A workgroup takes 64 nodes (the product's columns) and stages their sources in LDS,
+// relative to each node's centre.  Its 16 waves split the work by unit, four per SIMD:
+// waves 8..15 compute the check potentials of 64 check points x 64 nodes per chunk into
+// LDS (a wave one node at a time, a lane per check point, so every source read is a
+// broadcast and the lanes loop together), with the arithmetic of k_s2m (FP64) or k_s2m32
+// (the mixed form: FP32 terms over at most 256 sources, folded into FP64); waves 0..7
+// multiply the previous chunk into the product's r x 64 block on FP64 MFMA
+// (v_mfma_f64_16x16x4f64, r <= 304 in 20 row tiles of 16: wave w takes tiles w, w + 8 and,
+// for w < 4, w + 16, so each SIMD holds five; Uh read from L2 a k-step ahead, from a
+// zero-padded copy).  The two buffers of check potentials alternate, one barrier per
+// chunk, so the VALU work of one chunk runs beside the MFMAs of the previous one on
+// every SIMD.  A workgroup whose nodes hold more
+// sources than the LDS stage reads them from global memory instead.
+// ---------------------------------------------------------------------------
+constexpr int SU_NT = 64;                              // nodes per workgroup: 4 column tiles
+constexpr int SU_TB = 1024;                            // waves 0..7 MFMA, 8..15 check potentials
+constexpr int SU_RMIN = 241, SU_RMAX = 304;            // the ranks r this kernel takes
+constexpr int SU_RW = 3;                               // row tiles per MFMA wave: 3 (w < 4) or 2
+constexpr int SU_KC = 64;                              // check points per chunk: 16 k-steps
+constexpr int SU_PS = SU_NT + 1;                       // LDS row stride of the check potentials
+constexpr int SU_PD = 2;                               // k-steps of Uh loads in flight
+constexpr int SU_FOLD = 256;                           // FP32 terms per fold (the mixed form)
+typedef double su_f64x4 __attribute__((ext_vector_type(4)));
+static_assert(SU_KC == UC2E_PK && 16 * 20 == UC2E_PROWS && SU_RMAX <= UC2E_PROWS && SU_PD == UC2E_PKX &&
+                  (SU_KC / 4) % SU_PD == 0,
+              "the padded Uh matches the kernel's tiles and prefetch");
+
+template <bool F32>
+struct SuSrc;
+template <>
+struct SuSrc<true> {
+  using T = float4;
+  static constexpr int CAP = 4096;   // 64 nodes of at most 64 sources
+};
+template <>
+struct SuSrc<false> {
+  using T = double4;
+  static constexpr int CAP = 2048;
+};
+
+// the check potential at (z0, z1, z2) (scaled, relative to the node centre) of the
+// node's sources [j0, j1), fetched by src(j): k_s2m32's FP32 terms folded into FP64 every
+// SU_FOLD sources, or k_s2m's FP64 terms; four sources per step, summed in order
+template <bool F32, class Fetch>
+__device__ __forceinline__ double su_potential(double y0, double y1, double y2, double sc, int32_t j0, int32_t j1,
+                                               Fetch &&src) {
+  if constexpr (F32) {
+    const float z0 = (float)(y0 * sc), z1 = (float)(y1 * sc), z2 = (float)(y2 * sc);
+    auto term = [&](const float4 &v) {
+      const float d0 = z0 - v.x, d1 = z1 - v.y, d2 = z2 - v.z;
+      return __builtin_amdgcn_rsqf(fmaf(d2, d2, fmaf(d1, d1, d0 * d0)));
+    };
+    double a = 0.0;
+    for (int32_t jb = j0; jb < j1; jb += SU_FOLD) {
+      const int32_t je = min(j1, jb + SU_FOLD);
+      float part = 0.f;
+      int32_t j = jb;
+      for (; j + 4 <= je; j += 4) {
+        const float4 v0 = src(j), v1 = src(j + 1), v2 = src(j + 2), v3 = src(j + 3);
+        const float r0 = term(v0), r1 = term(v1), r2 = term(v2), r3 = term(v3);
+        part = fmaf(r0, v0.w, part);
+        part = fmaf(r1, v1.w, part);
+        part = fmaf(r2, v2.w, part);
+        part = fmaf(r3, v3.w, part);
+      }
+      for (; j < je; j++) {
+        const float4 v = src(j);
+        part = fmaf(term(v), v.w, part);
+      }
+      a += (double)part;
+    }
+    return C4PI * a;
+  } else {
+    const double z0 = y0 * sc, z1 = y1 * sc, z2 = y2 * sc;
+    auto term = [&](const double4 &v) {
+      const double d0 = z0 - v.x, d1 = z1 - v.y, d2 = z2 - v.z;
+      return (C4PI * rsqrt_f64(fma(d2, d2, fma(d1, d1, d0 * d0)))) * v.w;
+    };
+    double a = 0.0;
+    int32_t j = j0;
+    for (; j + 4 <= j1; j += 4) {
+      const double4 v0 = src(j), v1 = src(j + 1), v2 = src(j + 2), v3 = src(j + 3);
+      const double t0 = term(v0), t1 = term(v1), t2 = term(v2), t3 = term(v3);
+      a += t0;
+      a += t1;
+      a += t2;
+      a += t3;
+    }
+    for (; j < j1; j++) a += term(src(j));
+    return a;
+  }
+}
+
+template <bool F32>
+__global__ void __launch_bounds__(SU_TB)
+k_s2m_uc2e(const int32_t *__restrict__ nodes, int64_t nq, const double *__restrict__ centers, int64_t nn,
+           const int32_t *__restrict__ level, const double *__restrict__ lam, const double *__restrict__ surf, int Nl,
+           const int32_t *__restrict__ plo, const int32_t *__restrict__ phi, const double *__restrict__ sx,
+           const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ tau,
+           const double *__restrict__ Uhp, int r, double *__restrict__ tup) {
+  using ST = typename SuSrc<F32>::T;
+  constexpr int CAP = SuSrc<F32>::CAP;
+  __shared__ ST src[CAP];
+  __shared__ __attribute__((aligned(16))) double P[2][SU_KC * SU_PS];
+  __shared__ int32_t off[SU_NT + 1];      // node n's sources: [off[n], off[n + 1]) of the tile's list
+  __shared__ int32_t first[SU_NT];        // node n's first source in the sorted points
+  __shared__ double scl[SU_NT], ctr[3][SU_NT];
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t q0 = (int64_t)blockIdx.x * SU_NT;
+  // the nodes: source ranges and their exclusive scan, centres and scales
+  if (t < 64) {
+    const int64_t q = q0 + t;
+    int32_t cnt = 0, a = 0;
+    double sc = 1.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    if (q < nq) {
+      const int node = nodes[q];
+      a = plo[node];
+      cnt = phi[node] - a + 1;
+      sc = lam[level[node]];
+      c0 = centers[node];
+      c1 = centers[nn + node];
+      c2 = centers[2 * nn + node];
+    }
+    int32_t v = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    off[t + 1] = v;
+    if (t == 0) off[0] = 0;
+    first[t] = a;
+    scl[t] = sc;
+    ctr[0][t] = c0;
+    ctr[1][t] = c1;
+    ctr[2][t] = c2;
+  }
+  __syncthreads();
+  const int32_t total = off[SU_NT];
+  const bool staged = total <= CAP;
+  if (staged) {
+    // entry i of the tile's list belongs to the last node n with off[n] <= i
+    for (int32_t i = t; i < total; i += SU_TB) {
+      int n = 0, h = SU_NT;
+      while (h - n > 1) {
+        const int m = (n + h) >> 1;
+        if (off[m] <= i) n = m;
+        else h = m;
+      }
+      const int32_t j = first[n] + (i - off[n]);
+      if constexpr (F32)
+        src[i] = make_float4((float)(sx[j] - ctr[0][n]), (float)(sy[j] - ctr[1][n]), (float)(sz[j] - ctr[2][n]),
+                             (float)tau[j]);
+      else
+        src[i] = make_double4(sx[j] - ctr[0][n], sy[j] - ctr[1][n], sz[j] - ctr[2][n], tau[j]);
+    }
+  }
+  const int nch = (Nl + SU_KC - 1) / SU_KC;
+  const int RT = (r + 15) >> 4;
+  if (w >= 8) {
+    // check potentials: wave 8 + pw takes the nodes pw, pw + 8, ..; lane = check point
+    const int pw = w - 8;
+    __syncthreads();   // the staged sources
+    double ny0 = 0.0, ny1 = 0.0, ny2 = 0.0;
+    if (lane < Nl) {
+      ny0 = surf[lane];
+      ny1 = surf[Nl + lane];
+      ny2 = surf[2 * Nl + lane];
+    }
+    for (int ch = 0; ch < nch; ch++) {
+      const int k = ch * SU_KC + lane;
+      const bool kin = k < Nl;
+      const double y0 = ny0, y1 = ny1, y2 = ny2;
+      if (k + SU_KC < Nl) {   // the next chunk's check point
+        ny0 = surf[k + SU_KC];
+        ny1 = surf[Nl + k + SU_KC];
+        ny2 = surf[2 * Nl + k + SU_KC];
+      }
+      double *Pb = P[ch & 1];
+      for (int n = pw; n < SU_NT; n += 8) {
+        const int32_t j0 = off[n], j1 = off[n + 1];
+        double pot;
+        if (staged) {
+          pot = su_potential<F32>(y0, y1, y2, scl[n], j0, j1, [&](int32_t j) -> ST { return src[j]; });
+        } else {
+          const int32_t fj = first[n] - j0;
+          const double c0 = ctr[0][n], c1 = ctr[1][n], c2 = ctr[2][n];
+          pot = su_potential<F32>(y0, y1, y2, scl[n], j0, j1, [&](int32_t j) -> ST {
+            const int32_t g = fj + j;
+            if constexpr (F32)
+              return make_float4((float)(sx[g] - c0), (float)(sy[g] - c1), (float)(sz[g] - c2), (float)tau[g]);
+            else
+              return make_double4(sx[g] - c0, sy[g] - c1, sz[g] - c2, tau[g]);
+          });
+        }
+        Pb[lane * SU_PS + n] = kin ? pot : 0.0;
+      }
+      __syncthreads();   // chunk ch is in P[ch & 1]; the MFMA waves are done with P[(ch + 1) & 1]
+    }
+    __syncthreads();     // matches the MFMA waves' last chunk
+    return;
+  }
+  // the product's accumulators: row tiles w + 8 i, column tiles c; operand lane (lk, li):
+  // A row / B column li, k = lk
+  const int li = lane & 15, lk = lane >> 4;
+  su_f64x4 acc[SU_RW][4];
+#pragma unroll
+  for (int i = 0; i < SU_RW; i++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[i][c] = su_f64x4{0.0, 0.0, 0.0, 0.0};
+  // A = Uh[row, kg] for k-step g of the whole sweep, from the zero-padded copy (320 rows,
+  // Nl rounded up to the chunk and one k-step more): unguarded loads, so that the next
+  // k-step's stay in flight across this one's MFMAs
+  auto loadA = [&](int g, double *dst) {
+    const double *__restrict__ u = Uhp + (int64_t)(4 * g + lk) * UC2E_PROWS + 16 * w + li;
+#pragma unroll
+    for (int i = 0; i < SU_RW; i++)
+      if (i < 2 || w < 4) dst[i] = u[128 * i];
+  };
+  // the operands of the next SU_PD k-steps in flight (a ring indexed by the k-step)
+  double ring[SU_PD][SU_RW];
+#pragma unroll
+  for (int g = 0; g < SU_PD; g++) loadA(g, ring[g]);
+  __syncthreads();       // the staged sources
+  for (int ch = 0; ch < nch; ch++) {
+    __syncthreads();     // chunk ch is in P[ch & 1]
+    const double *Pb = P[ch & 1];
+#pragma unroll
+    for (int ks = 0; ks < SU_KC / 4; ks++) {
+      double *a = ring[ks % SU_PD];
+      double b[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) b[c] = Pb[(4 * ks + lk) * SU_PS + 16 * c + li];
+      // tiles at or beyond r multiply zeros (the host takes this kernel for 240 < r <= 304)
+#pragma unroll
+      for (int i = 0; i < SU_RW; i++)
+        if (i < 2 || w < 4)
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[c], acc[i][c], 0, 0, 0);
+      loadA(ch * (SU_KC / 4) + ks + SU_PD, a);   // the slot's next k-step
+    }
+  }
+  __syncthreads();       // matches the check-potential waves' last barrier
+  // lane (lk, li) holds rows lk + 4 e of column li of each tile (the FP64 MFMA's layout)
+#pragma unroll
+  for (int i = 0; i < SU_RW; i++) {
+    const int rt = w + 8 * i;
+    if (rt >= RT || (i == 2 && w >= 4)) continue;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int64_t q = q0 + 16 * c + li;
+      if (q >= nq) continue;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int row = 16 * rt + lk + 4 * e;
+        if (row < r) tup[q * r + row] = acc[i][c][e];
+      }
+    }
+  }
+}
+
+// the cube method's S2M is fused with UC2E's first factor where the rank fits the kernel
+static bool fused_s2m(const OpCache &o) { return o.r_up >= SU_RMIN && o.r_up <= SU_RMAX && o.Uh_up_pad.p; }
+
 // dst[i, cols[q]] = alpha * src[i, q]
 __global__ void k_scatter_cols(double *__restrict__ dst, const double *__restrict__ src, const int32_t *__restrict__ cols,
                                int rows, int64_t ncols, double alpha) {
@@ -650,7 +921,7 @@ void allocate_pass_buffers(Plan &plan) {
   const int ndmax = o.method != METHOD_HARMONIC ? 1 : o.nb == 4 ? 9 : o.kernel == FFMM_KERNEL_LAPLACE ? 4 : 1;
   plan.tau.resize((size_t)plan.N * ndmax);
   if (o.method == METHOD_CUBE) {
-    plan.pM.resize((size_t)Nl * nq);
+    if (!fused_s2m(o)) plan.pM.resize((size_t)Nl * nq);   // else S2M is fused with UC2E's first factor
     plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);
     plan.Mup.resize((size_t)Ns * nq);
   }
@@ -716,16 +987,29 @@ void upward_pass(Plan &plan) {
     // (src/device_passes.jl:496-502), as two GEMMs over all of them.  pinv_scale = lam[l]
     // is a power of two, applied exactly in the scatter.
     const int32_t *nodes = plan.s2m_nodes.p + plan.s2m_off[l2];
-    if (plan.precision == FFMM_PREC_MIXED)
-      k_s2m32<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+    const bool mixed = plan.precision == FFMM_PREC_MIXED;
+    if (fused_s2m(o)) {
+      // S2M and the first factor in one kernel: no check potentials in memory
+      const unsigned g = (unsigned)((nq + SU_NT - 1) / SU_NT);
+#define FFMM_SU_ARGS                                                                                          \
+  nodes, nq, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p, o.large.p, Nl, plan.plo.p, plan.phi.p, \
+      plan.sx.p, plan.sy.p, plan.sz.p, plan.tau.p, o.Uh_up_pad.p, o.r_up, plan.tup.p
+      if (mixed) k_s2m_uc2e<true><<<g, SU_TB, 0, s>>>(FFMM_SU_ARGS);
+      else k_s2m_uc2e<false><<<g, SU_TB, 0, s>>>(FFMM_SU_ARGS);
+#undef FFMM_SU_ARGS
+      FFMM_LAUNCH_CHECK();
+    } else {
+      if (mixed)
+        k_s2m32<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
+                                                o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p,
+                                                plan.sz.p, plan.tau.p, plan.pM.p);
+      else
+        k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
                                               o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
                                               plan.tau.p, plan.pM.p);
-    else
-      k_s2m<<<(unsigned)nq, S2M_TB, 0, s>>>(nodes, plan.centers.p, plan.nnodes, plan.level.p, plan.lam_d.p,
-                                            o.large.p, Nl, plan.plo.p, plan.phi.p, plan.sx.p, plan.sy.p, plan.sz.p,
-                                            plan.tau.p, plan.pM.p);
-    FFMM_LAUNCH_CHECK();
-    gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
+      FFMM_LAUNCH_CHECK();
+      gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
+    }
     gemm(ctx, Ns, (int)nq, o.r_up, 1.0, o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
     k_scatter_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq,
                                                                     plan.level.p, plan.lam_d.p, Mp32);
