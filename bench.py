@@ -51,20 +51,23 @@ def interaction_counts(tree, nleaf_targets=None):
 
 
 def gemm_shapes(plan, ops, mixed: bool):
-    """The library GEMMs of one step of the cube method, in launch order: (name, m, n, k,
-    type, flop).  Per step: UC2E as two FP64 factors over the childless sourced nodes of
+    """The GEMMs of one step of the cube method, in launch order: (name, m, n, k, type,
+    flop, library: run by the BLAS library).  Per step: UC2E as two FP64 factors over the childless sourced nodes of
     levels 2 and below, M2M per level (FP32 in the mixed form) from level L-2 up to 2, L2L
     per level (FP64) from level 3 down, DC2E as two FP64 factors over the leaves."""
     Ns, Nl, L = ops.nsmall, ops.nlarge, plan.nlevels
     nq_up, nq_dn = int(plan.level_s2m[2:].sum()), int(plan.level_leaves[2:].sum())
     ru, rd = ops.P_up.rank, ops.P_down.rank
-    out = [("UC2E Uh", ru, nq_up, Nl, "f64"), ("UC2E V", Ns, nq_up, ru, "f64")]
+    # ranks 241..304: UC2E's first factor runs inside k_s2m_uc2e (S2M fused with it), not as
+    # a library GEMM
+    fused = 241 <= ru <= 304
+    out = [("UC2E Uh" + (" (in k_s2m_uc2e)" if fused else ""), ru, nq_up, Nl, "f64"), ("UC2E V", Ns, nq_up, ru, "f64")]
     for l in range(L - 2, 1, -1):
         out.append((f"M2M level {l}", Ns, int(plan.level_count[l + 1]) // 8, 8 * Ns, "f32" if mixed else "f64"))
     for l in range(3, L):
         out.append((f"L2L level {l}", 8 * Ns, int(plan.level_count[l]) // 8, Ns, "f64"))
     out += [("DC2E Uh", rd, nq_dn, Ns, "f64"), ("DC2E V", Nl, nq_dn, rd, "f64")]
-    return [{"name": a, "m": m_, "n": n_, "k": k_, "type": t_, "flop": 2.0 * m_ * n_ * k_}
+    return [{"name": a, "m": m_, "n": n_, "k": k_, "type": t_, "flop": 2.0 * m_ * n_ * k_, "library": "k_s2m_uc2e" not in a}
             for a, m_, n_, k_, t_ in out if m_ > 0 and n_ > 0]
 
 
