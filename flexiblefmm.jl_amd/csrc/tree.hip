@@ -464,9 +464,15 @@ void build_tree(Plan &plan, const double *xyz) {
   levels[0].col.resize(NCOLL);
   k_root_col<<<1, 32, 0, s>>>(levels[0].col.p);
   for (int l = 1; l < L; l++) tag(ctx, levels[l], levels[l - 1]);
-  // 2:1 restriction (src/tree/device_builder.jl:444-460): 0-based levels 1 .. L-3
+  // 2:1 restriction (src/tree/device_builder.jl:444-460): 0-based levels 1 .. L-3.  The
+  // depths are recomputed once per pass: a split at level l makes new leaves (depth 0) at
+  // level l + 1 and changes the depths of levels l .. 0 only, which this pass does not
+  // check again (levels are taken upwards from 1, and a node's flag compares the depths
+  // of its own level), so each check sees the depths a recomputation after every split
+  // would give it
   depths(ctx, levels, L - 1);
-  for (;;) {
+  for (bool pass0 = true;; pass0 = false) {
+    if (!pass0) depths(ctx, levels, L - 1);
     int64_t nsplit = 0;
     for (int l = 1; l <= L - 3; l++) {
       int64_t n = levels[l].n;
@@ -478,7 +484,6 @@ void build_tree(Plan &plan, const double *xyz) {
       if (nb == 0) continue;
       split(ctx, levels[l], which.p, nb, l, levels[l + 1], skeys.p, KL, false);
       tag(ctx, levels[l + 1], levels[l]);
-      depths(ctx, levels, l + 1);
       nsplit += nb;
     }
     if (nsplit == 0) break;
