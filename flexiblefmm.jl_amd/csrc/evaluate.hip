@@ -427,7 +427,7 @@ void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out
   // the largest key (outside the box: all ones), read from pinned memory
   k_last_key<<<1, 64, 0, s>>>(skeys.p + nt - 1, ctx.pinned_dev);
   FFMM_LAUNCH_CHECK();
-  ctx.sync();
+  wait_stream(ctx);
   if ((uint32_t)ctx.pinned[0] == 0xFFFFFFFFu)
     fail(FFMM_ERR_ARGUMENT, nodes ? "a target's node is not a real leaf of the tree" : "a target lies outside the tree's box");
   plan.tgt_ptr.resize(nleaves + 1);

@@ -415,6 +415,8 @@ void exclusive_scan_i32_async(Context &ctx, const int32_t *in, int32_t *out, int
 // indices i (ascending) with flag[i] != 0; returns their number (blocking read back)
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n);
 // the count into the context's pinned slot `slot` (0 .. 63) by the stream, no synchronisation
+// all work queued on the stream so far is done; the host then reads pinned slots only
+void wait_stream(Context &ctx);
 void compact_flags_u8_async(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n, int slot);
 void set_pinned(Context &ctx, int slot, int64_t v);
 constexpr int NPINNED = 64;

@@ -228,7 +228,7 @@ void build_lists(Plan &plan) {
     FFMM_LAUNCH_CHECK();
     exclusive_scan_i32_async(ctx, cnt.p + lo, li.ptr.p, nl, l);
   }
-  ctx.sync();
+  wait_stream(ctx);
   for (int l = 2; l < plan.L; l++) {
     const int64_t np = ctx.pinned[l];
     if (np >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many far pairs on one level for 32-bit pair indices");

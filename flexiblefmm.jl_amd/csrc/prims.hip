@@ -3,6 +3,7 @@
 // tuned implementation of these primitives (decoupled look-back scans, onesweep
 // radix sort) for CDNA; the tree, list, pass and evaluation kernels are this
 // library's own.
+#include <chrono>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -84,6 +85,40 @@ void exclusive_scan_i32_async(Context &ctx, const int32_t *in, int32_t *out, int
   FFMM_HIP(hipGetLastError());
 }
 
+// The value a kernel on the stream stores into pinned slot `slot`, which the host set to -1
+// before launching it: the host polls the slot (coherent host memory, written through by
+// the kernel) instead of synchronising the stream, whose blocking wait returns some tens
+// of microseconds after the GPU is done.  The kernel that stores the value runs after all
+// earlier work of the stream.  Without the value after 2 s the stream is synchronised,
+// which reports a failed launch.
+static int64_t poll_slot(Context &ctx, int slot) {
+  volatile int64_t *v = ctx.pinned + slot;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t k = 1;; k++) {
+    const int64_t x = __atomic_load_n(v, __ATOMIC_ACQUIRE);
+    if (x != -1) return x;
+    if ((k & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+      ctx.sync();
+      const int64_t y = __atomic_load_n(v, __ATOMIC_ACQUIRE);
+      if (y == -1) fail(FFMM_ERR_DEVICE, "a count was not written back by the device");
+      return y;
+    }
+  }
+}
+constexpr int POLL_SLOT = NPINNED - 1;   // the values that the host waits for
+
+__global__ void k_set_flag(int64_t *__restrict__ host) {
+  if (threadIdx.x == 0) host[0] = 1;
+}
+// all work queued on the stream so far is done (as ctx.sync(), for callers that then read
+// only pinned slots written by that work): a flag stored after it, polled
+void wait_stream(Context &ctx) {
+  __atomic_store_n(ctx.pinned + POLL_SLOT, (int64_t)-1, __ATOMIC_RELEASE);
+  k_set_flag<<<1, 64, 0, ctx.stream>>>(ctx.pinned_dev + POLL_SLOT);
+  FFMM_HIP(hipGetLastError());
+  poll_slot(ctx, POLL_SLOT);
+}
+
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n) {
   if (n <= 0) return 0;
   rocprim::counting_iterator<int32_t> it(0);
@@ -94,10 +129,10 @@ int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_
   char *t = static_cast<char *>(scratch(ctx, off + 256));
   int32_t *dcount = reinterpret_cast<int32_t *>(t + off);
   FFMM_HIP(rocprim::select(t, bytes, it, flag, idx, dcount, (size_t)n, ctx.stream));
-  k_copy_count<<<1, 64, 0, ctx.stream>>>(dcount, ctx.pinned_dev);
+  __atomic_store_n(ctx.pinned + POLL_SLOT, (int64_t)-1, __ATOMIC_RELEASE);
+  k_copy_count<<<1, 64, 0, ctx.stream>>>(dcount, ctx.pinned_dev + POLL_SLOT);
   FFMM_HIP(hipGetLastError());
-  ctx.sync();
-  return ctx.pinned[0];
+  return poll_slot(ctx, POLL_SLOT);
 }
 
 __global__ void k_copy_count_slot(const int32_t *__restrict__ dcount, int64_t *__restrict__ host) {

@@ -905,7 +905,7 @@ void spectral_prepare(Plan &plan) {
       plan.spec_tiles[l].resize(nt);
       compact_flags_u8_async(ctx, tflag[l].p, plan.spec_tiles[l].p, nt, l);
     }
-    ctx.sync();
+    wait_stream(ctx);
     for (int l = 2; l < L; l++) plan.spec_ntiles[l] = plan.nl[l] / NCHILD > 0 ? ctx.pinned[l] : 0;
     for (auto *v : {&plan.spec_ulist, &plan.spec_fit, &plan.spec_over}) {
       v->clear();
@@ -931,7 +931,7 @@ void spectral_prepare(Plan &plan) {
       compact_flags_u8_async(ctx, tflag[l].p, plan.spec_fit[l].p, nt, l);
       compact_flags_u8_async(ctx, tflag[l].p + nt, overidx[l].p, nt, NPINNED / 2 + l);
     }
-    ctx.sync();
+    wait_stream(ctx);
     for (int l = 2; l < L; l++) {
       if (plan.spec_ntiles[l] == 0) continue;
       plan.spec_nfit[l] = ctx.pinned[l];

@@ -502,7 +502,7 @@ void build_tree(Plan &plan, const double *xyz) {
     xw[l].resize(levels[l].n);
     compact_flags_u8_async(ctx, levels[l].xlist.p, xw[l].p, levels[l].n, l);
   }
-  ctx.sync();
+  wait_stream(ctx);
   std::vector<int64_t> nxl(L, 0);
   for (int l = 1; l <= L - 2; l++) nxl[l] = ctx.pinned[l];
   for (int l = 1; l <= L - 2; l++) {
@@ -551,7 +551,7 @@ void build_tree(Plan &plan, const double *xyz) {
   for (int l = 0; l <= L; l++) ls.lo[l] = l < L ? plan.lo[l] : nn;
   k_level_bounds<<<1, 64, 0, s>>>(plan.s2m_nodes.p, plan.leaf_nodes.p, ls, L, ctx.pinned_dev);
   FFMM_LAUNCH_CHECK();
-  ctx.sync();
+  wait_stream(ctx);
   for (int l = 0; l <= L; l++) {
     plan.s2m_off[l] = ctx.pinned[2 + l];
     plan.leaf_off[l] = ctx.pinned[2 + (MAX_KEY_LEVELS + 2) + l];
