@@ -52,6 +52,32 @@ def test_matvec_f64_dense_vs_oracle_and_direct(setup):
     assert mrd(u[rows], ud) < 1e-6
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-11), ("mixed", 2e-6)])
+def test_s2m_uc2e_nodes_beyond_the_lds_stage(precision, tol):
+    """ppl = 300 on 12,000 uniform points: the 64 boxes of level 2 are leaves of ~190
+    sources, one group of 64 childless nodes holding more sources than the fused S2M +
+    UC2E kernel stages in LDS (4,096 in FP32, 2,048 in FP64), so it reads them from global
+    memory; against the oracle."""
+    import ffmm
+    rng = np.random.default_rng(46)
+    N, ppl = 12_000, 300
+    pts = uniform(rng, N)
+    tau = rng.standard_normal(N)
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    print(f"[fmm] fused S2M beyond the LDS stage, {precision}, ppl={ppl}", flush=True)
+    plan = ffmm.DevicePlan(m, pts, ppl=ppl, bbox=BOX, cache=cache, precision=precision)
+    # the childless sourced nodes of levels 2 and below, in node order, by 64: some group
+    # holds more sources than either stage
+    t = plan.download_tree()
+    idx = np.nonzero((t.child_first < 0) & (t.phi >= t.plo))[0]
+    cnt = (t.phi - t.plo + 1)[idx[idx >= t.level_start[2]]]
+    assert np.add.reduceat(cnt, np.arange(0, len(cnt), 64)).max() > 4096
+    u = plan.matvec(tau).cpu().numpy()
+    ref = O.FMM(cache.host_operators(), pts, ppl, BOX[0], BOX[1])
+    assert mrd(u, ref.matvec(tau)) < tol
+
+
 @pytest.mark.parametrize("precision,tol_oracle,tol_direct", [("f64", 1e-11, 1e-6), ("mixed", 2e-6, 2e-6)])
 def test_matvec_spectral_vs_oracle_and_direct(setup, precision, tol_oracle, tol_direct):
     """The spectral M2L (batched DFTs + per-pair Hadamard products, the reference CPU's
