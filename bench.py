@@ -59,15 +59,18 @@ def gemm_shapes(plan, ops, mixed: bool):
     nq_up, nq_dn = int(plan.level_s2m[2:].sum()), int(plan.level_leaves[2:].sum())
     ru, rd = ops.P_up.rank, ops.P_down.rank
     # ranks 241..304: UC2E's first factor runs inside k_s2m_uc2e (S2M fused with it), not as
-    # a library GEMM
+    # a library GEMM; with 240 < Ns <= 304 (r_dn likewise), UC2E's second factor and DC2E's
+    # first run as k_gemm_cols
     fused = 241 <= ru <= 304
-    out = [("UC2E Uh" + (" (in k_s2m_uc2e)" if fused else ""), ru, nq_up, Nl, "f64"), ("UC2E V", Ns, nq_up, ru, "f64")]
+    cols_up, cols_dn = 240 < Ns <= 304, 240 < rd <= 304
+    out = [("UC2E Uh" + (" (in k_s2m_uc2e)" if fused else ""), ru, nq_up, Nl, "f64"),
+           ("UC2E V" + (" (in k_gemm_cols)" if cols_up else ""), Ns, nq_up, ru, "f64")]
     for l in range(L - 2, 1, -1):
         out.append((f"M2M level {l}", Ns, int(plan.level_count[l + 1]) // 8, 8 * Ns, "f32" if mixed else "f64"))
     for l in range(3, L):
         out.append((f"L2L level {l}", 8 * Ns, int(plan.level_count[l]) // 8, Ns, "f64"))
-    out += [("DC2E Uh", rd, nq_dn, Ns, "f64"), ("DC2E V", Nl, nq_dn, rd, "f64")]
-    return [{"name": a, "m": m_, "n": n_, "k": k_, "type": t_, "flop": 2.0 * m_ * n_ * k_, "library": "k_s2m_uc2e" not in a}
+    out += [("DC2E Uh" + (" (in k_gemm_cols)" if cols_dn else ""), rd, nq_dn, Ns, "f64"), ("DC2E V", Nl, nq_dn, rd, "f64")]
+    return [{"name": a, "m": m_, "n": n_, "k": k_, "type": t_, "flop": 2.0 * m_ * n_ * k_, "library": " (in " not in a}
             for a, m_, n_, k_, t_ in out if m_ > 0 and n_ > 0]
 
 

@@ -400,6 +400,16 @@ int ffmm_opcache_put_cube(ffmm_opcache cache, double w_ref, const double *small,
     for (int j = 0; j < r_up; j++)
       for (int i = 0; i < Ns; i++) vs[(size_t)j * Ns + i] = V_up[(size_t)j * Ns + i] * invS_up[j];
     o.Vs_up.upload(vs.data(), vs.size(), s);
+    // operand copies of the column GEMMs (rows and k in the kernel's tiles), zero-padded
+    auto pad_rows = [&](DBuf<double> &dst, const double *a, int rows, int K) {
+      const size_t nk = (size_t)(K + UC2E_PK - 1) / UC2E_PK * UC2E_PK + 4 * UC2E_PKX;
+      std::vector<double> ap(nk * UC2E_PROWS, 0.0);
+      for (int k = 0; k < K; k++)
+        for (int i = 0; i < rows; i++) ap[(size_t)k * UC2E_PROWS + i] = a[(size_t)k * rows + i];
+      dst.upload(ap.data(), ap.size(), s);
+    };
+    if (Ns > 240 && Ns <= 304) pad_rows(o.Vs_up_pad, vs.data(), Ns, r_up);
+    if (r_dn > 240 && r_dn <= 304) pad_rows(o.Uh_dn_pad, Uh_dn, r_dn, Ns);
     std::vector<double> vd((size_t)Nl * r_dn);
     for (int j = 0; j < r_dn; j++)
       for (int i = 0; i < Nl; i++) vd[(size_t)j * Nl + i] = V_dn[(size_t)j * Nl + i] * invS_dn[j];

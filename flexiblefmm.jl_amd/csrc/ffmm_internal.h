@@ -272,6 +272,9 @@ struct OpCache {
   // the fused S2M + UC2E kernel's Uh (ranks 241..304): UC2E_PROWS rows per check point,
   // Nl rounded up to UC2E_PK and UC2E_PKX k-steps more, zero-padded (no guards on its loads)
   DBuf<double> Uh_up_pad;
+  // the column GEMMs' operators (k_gemm_cols: UC2E's V diag(invS), DC2E's Uh), the same
+  // layout (UC2E_PROWS rows per k, k rounded up to UC2E_PK and UC2E_PKX k-steps more)
+  DBuf<double> Vs_up_pad, Uh_dn_pad;
   DBuf<double> M2M, L2L, class_ops;
   DBuf<float> M2M32, class_ops32;   // FP32 copies for the mixed form
   DBuf<int32_t> far_class, idx, inv;         // idx/inv: [f * Ns + i]

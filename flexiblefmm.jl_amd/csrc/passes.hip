@@ -408,6 +408,117 @@ k_s2m_uc2e(const int32_t *__restrict__ nodes, int64_t nq, const double *__restri
   }
 }
 
+// The FP64 GEMMs of the pseudo-inverses whose columns belong to scattered nodes, with the
+// gather or the scatter of those columns inside: MODE 0, UC2E's second factor
+// (src/device_passes.jl:496-502), Mp[:, node q] = lam(q) (V diag(invS)) tup[:, q], written
+// straight into the multipoles (and their FP32 mirror); MODE 1, DC2E's first factor
+// (src/device_passes.jl:583-593), tup[:, q] = Uh (lam(q) Pl[:, leaf q]), read straight from
+// the partial locals.  lam is a power of two, so the scaling is exact where it is applied.
+// The layout of k_s2m_uc2e: 64 columns per workgroup, waves 8..15 copy chunks of 64 k of
+// the columns into LDS (a lane per k, coalesced), waves 0..7 multiply the previous chunk
+// on FP64 MFMA with the operator from its zero-padded copy (M <= 304 rows).
+template <int MODE>
+__global__ void __launch_bounds__(SU_TB)
+k_gemm_cols(const double *__restrict__ Ap, int M, int K, const double *__restrict__ B, int ldb,
+            const int32_t *__restrict__ cols, int64_t nq, const int32_t *__restrict__ level,
+            const double *__restrict__ lam, double *__restrict__ out, float *__restrict__ out32, int ldo) {
+  __shared__ __attribute__((aligned(16))) double P[2][SU_KC * SU_PS];
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t q0 = (int64_t)blockIdx.x * SU_NT;
+  const int nch = (K + SU_KC - 1) / SU_KC;
+  // the copy waves: wave 8 + pw takes the columns pw, pw + 8, ..; lane = k of the chunk
+  auto fill = [&](int ch, double *Pb) {
+    const int pw = w - 8, k = ch * SU_KC + lane;
+#pragma unroll
+    for (int j = 0; j < SU_NT / 8; j++) {
+      const int c = pw + 8 * j;
+      const int64_t q = q0 + c;
+      double v = 0.0;
+      if (q < nq && k < K) {
+        if (MODE == 0) {
+          v = B[q * ldb + k];
+        } else {
+          const int32_t n = cols[q];
+          v = lam[level[n]] * B[(int64_t)n * ldb + k];
+        }
+      }
+      Pb[lane * SU_PS + c] = v;
+    }
+  };
+  if (w >= 8) {
+    fill(0, P[0]);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
+      if (ch + 1 < nch) fill(ch + 1, P[(ch + 1) & 1]);
+      __syncthreads();   // chunk ch read by the MFMA waves, chunk ch + 1 written
+    }
+    return;
+  }
+  const int li = lane & 15, lk = lane >> 4;
+  su_f64x4 acc[SU_RW][4];
+#pragma unroll
+  for (int i = 0; i < SU_RW; i++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[i][c] = su_f64x4{0.0, 0.0, 0.0, 0.0};
+  auto loadA = [&](int g, double *dst) {
+    const double *__restrict__ u = Ap + (int64_t)(4 * g + lk) * UC2E_PROWS + 16 * w + li;
+#pragma unroll
+    for (int i = 0; i < SU_RW; i++)
+      if (i < 2 || w < 4) dst[i] = u[128 * i];
+  };
+  double ring[SU_PD][SU_RW];
+#pragma unroll
+  for (int g = 0; g < SU_PD; g++) loadA(g, ring[g]);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const double *Pb = P[ch & 1];
+#pragma unroll
+    for (int ks = 0; ks < SU_KC / 4; ks++) {
+      double *a = ring[ks % SU_PD];
+      double b[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) b[c] = Pb[(4 * ks + lk) * SU_PS + 16 * c + li];
+#pragma unroll
+      for (int i = 0; i < SU_RW; i++)
+        if (i < 2 || w < 4)
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[c], acc[i][c], 0, 0, 0);
+      loadA(ch * (SU_KC / 4) + ks + SU_PD, a);
+    }
+    __syncthreads();   // chunk ch read, chunk ch + 1 written
+  }
+  // lane (lk, li) holds rows lk + 4 e of column li of each tile (the FP64 MFMA's layout)
+#pragma unroll
+  for (int i = 0; i < SU_RW; i++) {
+    if (i == 2 && w >= 4) continue;
+    const int rt = w + 8 * i;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int64_t q = q0 + 16 * c + li;
+      if (q >= nq) continue;
+      int64_t o;
+      double sc = 1.0;
+      if (MODE == 0) {
+        const int32_t n = cols[q];
+        o = (int64_t)n * ldo;
+        sc = lam[level[n]];
+      } else {
+        o = q * ldo;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int row = 16 * rt + lk + 4 * e;
+        if (row < M) {
+          const double v = sc * acc[i][c][e];
+          out[o + row] = v;
+          if (MODE == 0 && out32) out32[o + row] = (float)v;
+        }
+      }
+    }
+  }
+}
+
 // the cube method's S2M is fused with UC2E's first factor where the rank fits the kernel
 static bool fused_s2m(const OpCache &o) { return o.r_up >= SU_RMIN && o.r_up <= SU_RMAX && o.Uh_up_pad.p; }
 
@@ -923,7 +1034,7 @@ void allocate_pass_buffers(Plan &plan) {
   if (o.method == METHOD_CUBE) {
     if (!fused_s2m(o)) plan.pM.resize((size_t)Nl * nq);   // else S2M is fused with UC2E's first factor
     plan.tup.resize((size_t)std::max(o.r_up, o.r_dn) * nq);
-    plan.Mup.resize((size_t)Ns * nq);
+    if (!o.Vs_up_pad.p) plan.Mup.resize((size_t)Ns * nq);   // else UC2E's second factor writes Mp itself
   }
   plan.tM.resize((size_t)Ns * ng);
   if (o.method == METHOD_CUBE && plan.precision == FFMM_PREC_MIXED) {
@@ -1010,9 +1121,15 @@ void upward_pass(Plan &plan) {
       FFMM_LAUNCH_CHECK();
       gemm(ctx, o.r_up, (int)nq, Nl, 1.0, o.Uh_up.p, o.r_up, plan.pM.p, Nl, 0.0, plan.tup.p, o.r_up);
     }
-    gemm(ctx, Ns, (int)nq, o.r_up, 1.0, o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
-    k_scatter_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq,
-                                                                    plan.level.p, plan.lam_d.p, Mp32);
+    if (o.Vs_up_pad.p) {
+      // the second factor scattered into the multipoles by the GEMM itself
+      k_gemm_cols<0><<<(unsigned)((nq + SU_NT - 1) / SU_NT), SU_TB, 0, s>>>(
+          o.Vs_up_pad.p, Ns, o.r_up, plan.tup.p, o.r_up, nodes, nq, plan.level.p, plan.lam_d.p, plan.Mp.p, Mp32, Ns);
+    } else {
+      gemm(ctx, Ns, (int)nq, o.r_up, 1.0, o.Vs_up.p, Ns, plan.tup.p, o.r_up, 0.0, plan.Mup.p, Ns);
+      k_scatter_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.Mp.p, plan.Mup.p, nodes, Ns, nq,
+                                                                      plan.level.p, plan.lam_d.p, Mp32);
+    }
     FFMM_LAUNCH_CHECK();
   }
   for (int l = plan.L - 1; l >= 2; l--) {
@@ -1177,10 +1294,17 @@ void downward_pass(Plan &plan) {
     return;
   }
   // the level's pinv_scale (a power of two) on the gathered check potentials: exact
-  k_gather_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, leaves, Ns, nq, plan.level.p,
-                                                                 plan.lam_d.p);
+  if (o.Uh_dn_pad.p) {
+    // the first factor reading the leaves' partial locals itself
+    k_gemm_cols<1><<<(unsigned)((nq + SU_NT - 1) / SU_NT), SU_TB, 0, s>>>(
+        o.Uh_dn_pad.p, o.r_dn, Ns, plan.Pl.p, Ns, leaves, nq, plan.level.p, plan.lam_d.p, plan.tup.p, nullptr, o.r_dn);
+  } else {
+    k_gather_cols_lam<<<nblocks((int64_t)Ns * nq, TB), TB, 0, s>>>(plan.G.p, plan.Pl.p, leaves, Ns, nq,
+                                                                   plan.level.p, plan.lam_d.p);
+    FFMM_LAUNCH_CHECK();
+    gemm(ctx, o.r_dn, (int)nq, Ns, 1.0, o.Uh_dn.p, o.r_dn, plan.G.p, Ns, 0.0, plan.tup.p, o.r_dn);
+  }
   FFMM_LAUNCH_CHECK();
-  gemm(ctx, o.r_dn, (int)nq, Ns, 1.0, o.Uh_dn.p, o.r_dn, plan.G.p, Ns, 0.0, plan.tup.p, o.r_dn);
   gemm(ctx, Nl, (int)nq, o.r_dn, 1.0, o.Vs_dn.p, Nl, plan.tup.p, o.r_dn, 0.0, Lc, Nl);
 }
 
