@@ -748,60 +748,24 @@ __global__ void __launch_bounds__(FM_TB) __attribute__((amdgpu_waves_per_eu(4)))
       pt[tid] = ctgt[p];
     }
     __syncthreads();
-    // gather: wave w takes the tile's pairs w + 8 j, a lane per coefficient k = lane + 64 i,
-    // all blocks.  A batch's loads (GB pairs, both coefficients of a lane) are issued
-    // together from indices clamped into range (the tile's last pair, coefficient N - 1), so
-    // a wave waits on memory twice per batch (the selection table, then the expansions)
-    // rather than twice per pair and coefficient.  The LDS stores are unconditional, so no
-    // load is moved behind a branch: a coefficient k >= N goes to the column's slack
-    // entries FM_MP .. FM_MP + 3, which no product reads, and a pair beyond the tile's
-    // last to its own columns, whose products the epilogue skips.
-    static_assert(FM_BS >= FM_MP + 4, "slack entries past the operator's K");
-    {
-      constexpr int PW = PPT / (FM_TB / 64);   // pairs per wave: 4 (NB 4), 16 (NB 1)
-      constexpr int GB = NB == 4 ? 2 : 8;      // pairs per batch
-      static_assert(PW % GB == 0, "whole batches");
-      const int wu = __builtin_amdgcn_readfirstlane(w);
-      for (int j0 = 0; j0 < PW && wu + 8 * j0 < npt; j0 += GB) {
-        int32_t fj[GB], g[GB][2];
-        const double *mj[GB];
-#pragma unroll
-        for (int j = 0; j < GB; j++) {
-          const int pl = min(wu + 8 * (j0 + j), npt - 1);
-          fj[j] = __builtin_amdgcn_readfirstlane(pf[pl]);
-          mj[j] = Mp + (int64_t)__builtin_amdgcn_readfirstlane(ps[pl]) * Ns;
-#pragma unroll
-          for (int i = 0; i < 2; i++) g[j][i] = gsel[fj[j] * N + min(lane + 64 * i, N - 1)];
-        }
-        double mv[GB][2][NB];
-#pragma unroll
-        for (int j = 0; j < GB; j++)
-#pragma unroll
-          for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int b = 0; b < NB; b++) mv[j][i][b] = mj[j][(g[j][i] & 0x7fffffff) + b * N];
-#pragma unroll
-        for (int j = 0; j < GB; j++) {
-          const int pl = wu + 8 * (j0 + j);   // < PPT: its NB columns lie in the tile
-#pragma unroll
-          for (int i = 0; i < 2; i++) {
-            const int k = lane + 64 * i;
-            const double sg = g[j][i] < 0 ? -1.0 : 1.0;
-            float *__restrict__ b = Bs + (pl * NB) * FM_BS + (k < N ? k : FM_MP + (lane & 3));
-            if (NB == 4) {
-              const double *__restrict__ of = offs + 3 * fj[j];
-              const double m0 = mv[j][i][0], m1 = mv[j][i][1 % NB], m2 = mv[j][i][2 % NB], m3 = mv[j][i][3 % NB];
-              b[0] = (float)(sg * m0);
-              b[FM_BS] = (float)(sg * m1);
-              b[2 * FM_BS] = (float)(sg * m2);
-              b[3 * FM_BS] = (float)(sg * (m3 + of[0] * m0 + of[1] * m1 + of[2] * m2));
-            } else {
-              b[0] = (float)(sg * mv[j][i][0]);
-            }
-          }
+    // gather: a wave per pair, a lane per coefficient, all blocks
+    for (int pl = w; pl < npt; pl += FM_TB / 64)
+      for (int k = lane; k < N; k += 64) {
+        const int32_t f = pf[pl];
+        const int32_t g = gsel[f * N + k];
+        const double *__restrict__ m = Mp + (int64_t)ps[pl] * Ns + (g & 0x7fffffff);
+        const double sg = g < 0 ? -1.0 : 1.0;
+        if (NB == 4) {
+          const double m0 = m[0], m1 = m[N], m2 = m[2 * N], m3 = m[3 * N];
+          float *__restrict__ b = Bs + (pl * 4) * FM_BS + k;
+          b[0] = (float)(sg * m0);
+          b[FM_BS] = (float)(sg * m1);
+          b[2 * FM_BS] = (float)(sg * m2);
+          b[3 * FM_BS] = (float)(sg * (m3 + offs[3 * f] * m0 + offs[3 * f + 1] * m1 + offs[3 * f + 2] * m2));
+        } else {
+          Bs[pl * FM_BS + k] = (float)(sg * m[0]);
         }
       }
-    }
     // operator row 16 w + li, k = 16 kb + 4 lk + q (q < 4): one 16-byte load per kb from
     // the padded row-major table (zero beyond N), in flight while the other waves gather
     fm_f32x4 a4[8];
