@@ -425,10 +425,10 @@ void evaluate_targets(Plan &plan, const double *targets, int64_t nt, double *out
   FFMM_LAUNCH_CHECK();
   sort_pairs_u32(ctx, keys.p, skeys.p, idx.p, plan.tgt_order.p, nt, 32);
   // the largest key (outside the box: all ones), read from pinned memory
+  arm_slots(ctx, 0, 1);
   k_last_key<<<1, 64, 0, s>>>(skeys.p + nt - 1, ctx.pinned_dev);
   FFMM_LAUNCH_CHECK();
-  wait_stream(ctx);
-  if ((uint32_t)ctx.pinned[0] == 0xFFFFFFFFu)
+  if ((uint32_t)read_slot(ctx, 0) == 0xFFFFFFFFu)
     fail(FFMM_ERR_ARGUMENT, nodes ? "a target's node is not a real leaf of the tree" : "a target lies outside the tree's box");
   plan.tgt_ptr.resize(nleaves + 1);
   k_bucket_ptr<<<nblocks(nleaves + 1, TB), TB, 0, s>>>(skeys.p, nt, nleaves, plan.tgt_ptr.p);

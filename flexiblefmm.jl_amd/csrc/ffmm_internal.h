@@ -417,9 +417,12 @@ int64_t exclusive_scan_i32(Context &ctx, const int32_t *in, int32_t *out, int64_
 void exclusive_scan_i32_async(Context &ctx, const int32_t *in, int32_t *out, int64_t n, int slot);
 // indices i (ascending) with flag[i] != 0; returns their number (blocking read back)
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n);
-// the count into the context's pinned slot `slot` (0 .. 63) by the stream, no synchronisation
-// all work queued on the stream so far is done; the host then reads pinned slots only
-void wait_stream(Context &ctx);
+// pinned slots [first, first + n) set to -1 before the kernels that store them are queued
+// (every earlier store to them read back or synchronised); read_slot polls one slot until
+// the device has stored its value
+void arm_slots(Context &ctx, int first, int n);
+int64_t read_slot(Context &ctx, int slot);
+// the count into the context's pinned slot `slot` (0 .. 62) by the stream, no synchronisation
 void compact_flags_u8_async(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n, int slot);
 void set_pinned(Context &ctx, int slot, int64_t v);
 constexpr int NPINNED = 64;

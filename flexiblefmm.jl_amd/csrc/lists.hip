@@ -211,6 +211,7 @@ void build_lists(Plan &plan) {
   // the pair counts and their scans, all levels queued at once: one read back
   DBuf<int32_t> cnt, ccnt, bounds;
   cnt.resize(std::max<int64_t>(1, plan.nnodes));
+  if (plan.L > 2) arm_slots(ctx, 2, plan.L - 2);
   for (int l = 0; l < plan.L; l++) {
     LevelLists &li = plan.lists[l];
     const int64_t lo = plan.lo[l], nl = plan.nl[l];
@@ -228,9 +229,8 @@ void build_lists(Plan &plan) {
     FFMM_LAUNCH_CHECK();
     exclusive_scan_i32_async(ctx, cnt.p + lo, li.ptr.p, nl, l);
   }
-  wait_stream(ctx);
   for (int l = 2; l < plan.L; l++) {
-    const int64_t np = ctx.pinned[l];
+    const int64_t np = read_slot(ctx, l);
     if (np >= INT32_MAX) fail(FFMM_ERR_ARGUMENT, "too many far pairs on one level for 32-bit pair indices");
     plan.lists[l].np = np;
   }

@@ -107,17 +107,17 @@ static int64_t poll_slot(Context &ctx, int slot) {
 }
 constexpr int POLL_SLOT = NPINNED - 1;   // the values that the host waits for
 
-__global__ void k_set_flag(int64_t *__restrict__ host) {
-  if (threadIdx.x == 0) host[0] = 1;
+// Pinned slots read back one by one: arm_slots sets slots [first, first + n) to -1 before
+// the kernels that store them are queued, and read_slot polls each slot until its value
+// is there (the values read back this way are counts, offsets and keys, never -1).  Each
+// value is awaited itself, so nothing rests on the order in which the device's stores to
+// host memory from different kernels become visible.  Arm a slot only when every earlier
+// store to it has been read back or the stream synchronised since.
+void arm_slots(Context &ctx, int first, int n) {
+  if (first < 0 || n < 0 || first + n > POLL_SLOT) fail(FFMM_ERR_STATE, "pinned slot out of range");
+  for (int i = 0; i < n; i++) __atomic_store_n(ctx.pinned + first + i, (int64_t)-1, __ATOMIC_RELEASE);
 }
-// all work queued on the stream so far is done (as ctx.sync(), for callers that then read
-// only pinned slots written by that work): a flag stored after it, polled
-void wait_stream(Context &ctx) {
-  __atomic_store_n(ctx.pinned + POLL_SLOT, (int64_t)-1, __ATOMIC_RELEASE);
-  k_set_flag<<<1, 64, 0, ctx.stream>>>(ctx.pinned_dev + POLL_SLOT);
-  FFMM_HIP(hipGetLastError());
-  poll_slot(ctx, POLL_SLOT);
-}
+int64_t read_slot(Context &ctx, int slot) { return poll_slot(ctx, slot); }
 
 int64_t compact_flags_u8(Context &ctx, const uint8_t *flag, int32_t *idx, int64_t n) {
   if (n <= 0) return 0;

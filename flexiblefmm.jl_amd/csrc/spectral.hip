@@ -903,10 +903,10 @@ void spectral_prepare(Plan &plan) {
                                                     tflag[l].p);
       FFMM_LAUNCH_CHECK();
       plan.spec_tiles[l].resize(nt);
+      arm_slots(ctx, l, 1);
       compact_flags_u8_async(ctx, tflag[l].p, plan.spec_tiles[l].p, nt, l);
     }
-    wait_stream(ctx);
-    for (int l = 2; l < L; l++) plan.spec_ntiles[l] = plan.nl[l] / NCHILD > 0 ? ctx.pinned[l] : 0;
+    for (int l = 2; l < L; l++) plan.spec_ntiles[l] = plan.nl[l] / NCHILD > 0 ? read_slot(ctx, l) : 0;
     for (auto *v : {&plan.spec_ulist, &plan.spec_fit, &plan.spec_over}) {
       v->clear();
       v->resize(L);
@@ -928,14 +928,15 @@ void spectral_prepare(Plan &plan) {
                                               plan.spec_ulist[l].p, plan.spec_uslot[l].p, tflag[l].p,
                                               tflag[l].p + nt);
       FFMM_LAUNCH_CHECK();
+      arm_slots(ctx, l, 1);
+      arm_slots(ctx, NPINNED / 2 + l, 1);
       compact_flags_u8_async(ctx, tflag[l].p, plan.spec_fit[l].p, nt, l);
       compact_flags_u8_async(ctx, tflag[l].p + nt, overidx[l].p, nt, NPINNED / 2 + l);
     }
-    wait_stream(ctx);
     for (int l = 2; l < L; l++) {
       if (plan.spec_ntiles[l] == 0) continue;
-      plan.spec_nfit[l] = ctx.pinned[l];
-      const int64_t no = plan.spec_nover[l] = ctx.pinned[NPINNED / 2 + l];
+      plan.spec_nfit[l] = read_slot(ctx, l);
+      const int64_t no = plan.spec_nover[l] = read_slot(ctx, NPINNED / 2 + l);
       if (no == 0) continue;
       plan.spec_over[l].resize(no);   // the tile ids themselves, for k_spec_hadamard_mfma
       k_take_i32<<<nblocks(no, 256), 256, 0, s>>>(plan.spec_tiles[l].p, overidx[l].p, no, plan.spec_over[l].p);

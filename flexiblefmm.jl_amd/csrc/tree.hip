@@ -495,16 +495,16 @@ void build_tree(Plan &plan, const double *xyz) {
                                                 levels[l].xlist.p);
     FFMM_LAUNCH_CHECK();
   }
-  // every level's xlist compacted first, the counts read back at once (the flags are
+  // every level's xlist compacted first, the counts read back together (the flags are
   // fixed before any ghost split, and ghosts carry no xlist flag)
   std::vector<DBuf<int32_t>> xw(L);
+  if (L > 2) arm_slots(ctx, 1, L - 2);
   for (int l = 1; l <= L - 2; l++) {
     xw[l].resize(levels[l].n);
     compact_flags_u8_async(ctx, levels[l].xlist.p, xw[l].p, levels[l].n, l);
   }
-  wait_stream(ctx);
   std::vector<int64_t> nxl(L, 0);
-  for (int l = 1; l <= L - 2; l++) nxl[l] = ctx.pinned[l];
+  for (int l = 1; l <= L - 2; l++) nxl[l] = read_slot(ctx, l);
   for (int l = 1; l <= L - 2; l++) {
     if (nxl[l] == 0) continue;
     split(ctx, levels[l], xw[l].p, nxl[l], l, levels[l + 1], skeys.p, KL, true);
@@ -549,12 +549,13 @@ void build_tree(Plan &plan, const double *xyz) {
   compact_flags_u8_async(ctx, plan.leaf.p, plan.leaf_nodes.p, nn, 1);
   LevelStarts ls{};
   for (int l = 0; l <= L; l++) ls.lo[l] = l < L ? plan.lo[l] : nn;
+  arm_slots(ctx, 2, L + 1);
+  arm_slots(ctx, 2 + (MAX_KEY_LEVELS + 2), L + 1);
   k_level_bounds<<<1, 64, 0, s>>>(plan.s2m_nodes.p, plan.leaf_nodes.p, ls, L, ctx.pinned_dev);
   FFMM_LAUNCH_CHECK();
-  wait_stream(ctx);
   for (int l = 0; l <= L; l++) {
-    plan.s2m_off[l] = ctx.pinned[2 + l];
-    plan.leaf_off[l] = ctx.pinned[2 + (MAX_KEY_LEVELS + 2) + l];
+    plan.s2m_off[l] = read_slot(ctx, 2 + l);
+    plan.leaf_off[l] = read_slot(ctx, 2 + (MAX_KEY_LEVELS + 2) + l);
   }
 }
 
