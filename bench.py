@@ -53,7 +53,8 @@ def interaction_counts(tree, nleaf_targets=None):
 def gemm_shapes(plan, ops, mixed: bool):
     """The GEMMs of one step of the cube method, in launch order: (name, m, n, k, type,
     flop, library: run by the BLAS library).  Per step: UC2E as two FP64 factors over the childless sourced nodes of
-    levels 2 and below, M2M per level (FP32 in the mixed form) from level L-2 up to 2, L2L
+    levels 2 and below, M2M per level (FP32 in the mixed form, in calls of at most 10,240
+    columns) from level L-2 up to 2, L2L
     per level (FP64) from level 3 down, DC2E as two FP64 factors over the leaves."""
     Ns, Nl, L = ops.nsmall, ops.nlarge, plan.nlevels
     nq_up, nq_dn = int(plan.level_s2m[2:].sum()), int(plan.level_leaves[2:].sum())
@@ -66,7 +67,13 @@ def gemm_shapes(plan, ops, mixed: bool):
     out = [("UC2E Uh" + (" (in k_s2m_uc2e)" if fused else ""), ru, nq_up, Nl, "f64"),
            ("UC2E V" + (" (in k_gemm_cols)" if cols_up else ""), Ns, nq_up, ru, "f64")]
     for l in range(L - 2, 1, -1):
-        out.append((f"M2M level {l}", Ns, int(plan.level_count[l + 1]) // 8, 8 * Ns, "f32" if mixed else "f64"))
+        ng = int(plan.level_count[l + 1]) // 8
+        if mixed:   # the FP32 M2M runs in calls of at most 10,240 columns
+            for c0 in range(0, ng, 10240):
+                part = f" (columns {c0}-{min(ng, c0 + 10240) - 1})" if ng > 10240 else ""
+                out.append((f"M2M level {l}{part}", Ns, min(10240, ng - c0), 8 * Ns, "f32"))
+        else:
+            out.append((f"M2M level {l}", Ns, ng, 8 * Ns, "f64"))
     for l in range(3, L):
         out.append((f"L2L level {l}", 8 * Ns, int(plan.level_count[l]) // 8, Ns, "f64"))
     out += [("DC2E Uh" + (" (in k_gemm_cols)" if cols_dn else ""), rd, nq_dn, Ns, "f64"), ("DC2E V", Nl, nq_dn, rd, "f64")]

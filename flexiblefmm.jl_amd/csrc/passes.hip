@@ -998,6 +998,8 @@ static void gemm(Context &ctx, int m, int n, int k, double alpha, const double *
                           &beta, C, ldc));
 }
 
+constexpr int64_t M2M32_CHUNK = 10240;   // columns per call of the mixed form's M2M GEMM
+
 static void gemm32(Context &ctx, int m, int n, int k, const float *A, int lda, const float *B, int ldb, float *C,
                    int ldc) {
   if (m == 0 || n == 0) return;
@@ -1143,8 +1145,16 @@ void upward_pass(Plan &plan) {
       const double *M2M = o.method == METHOD_HARMONIC ? plan.hl[l]->M2M.p : o.M2M.p;
       if (mirror) {
         // the mixed form: M2M as an FP32 GEMM over the children's FP32 mirror (its rounding
-        // is not amplified: the multipoles feed M2L, never a pseudo-inverse)
-        gemm32(ctx, Ns, (int)ng, NCHILD * Ns, o.M2M32.p, Ns, Mp32 + plan.lo[l + 1] * Ns, NCHILD * Ns, plan.tM32.p, Ns);
+        // is not amplified: the multipoles feed M2L, never a pseudo-inverse).  Wide levels
+        // go in column chunks: for n above ~11k the library takes 256 x 256 tiles, at most
+        // 192 workgroups for m = 296, each a 0.5 ms tile on one CU (levels of 14k-25k
+        // groups all took ~0.55 ms), where a chunk takes a one-wave grid of smaller tiles
+        const float *Bc = Mp32 + plan.lo[l + 1] * Ns;
+        for (int64_t c0 = 0; c0 < ng; c0 += M2M32_CHUNK) {
+          const int nc = (int)std::min<int64_t>(M2M32_CHUNK, ng - c0);
+          gemm32(ctx, Ns, nc, NCHILD * Ns, o.M2M32.p, Ns, Bc + c0 * NCHILD * Ns, NCHILD * Ns, plan.tM32.p + c0 * Ns,
+                 Ns);
+        }
         k_scatter_parents<float><<<nblocks((int64_t)Ns * ng, TB), TB, 0, s>>>(plan.Mp.p, plan.tM32.p, plan.parent.p,
                                                                               plan.lo[l + 1], Ns, ng, 1.0, Mp32);
       } else {

@@ -6,7 +6,9 @@ with the generators of the full-size runs (ffmm.workloads):
 * Gaussian mixture + uniform targets, mixed precision with the spectral M2L (the
   4M-source configuration): potential at the sources, potential and gradient at the
   targets, against the oracle and, on 10,000 sampled points, the direct sum (the 1e-6
-  gate); at 400,000 sources, where both Hadamard kernels of the spectral M2L run;
+  gate); at 400,000 sources, where both Hadamard kernels of the spectral M2L run; at
+  3,000,000 sources against the plain FP64 plan, where the mixed M2M runs in column
+  chunks;
 * StokesHarmonics p = 10 on ellipsoid surfaces, stokeslets + stresslets with the
   normals (the 2M-point configuration): the mixed and FP64 forms against the harmonic
   oracle and the FP64 Stokes direct sum;
@@ -130,6 +132,34 @@ def test_mixture_400k_mixed_spectral_both_hadamard_kernels():
     assert mrd(u, u_ref) < 2e-6
     rows = np.sort(np.random.default_rng(11).choice(n, 3000, replace=False))
     assert rel(u[rows], O.direct_self(pts, tau, rows=rows)) <= 1e-6
+
+
+def test_mixture_3m_mixed_m2m_in_column_chunks():
+    """3,000,000 Gaussian-mixture sources: some levels have more than 10,240 sibling
+    groups, so the mixed form's FP32 M2M runs in several column chunks there.  The
+    potential at every source agrees with the plain FP64 plan on the same points (which
+    makes one M2M call per level) to the mixed form's 1e-6 rel L2."""
+    import ffmm
+    from ffmm import workloads as W
+    n = 3_000_000
+    pts = W.gaussian_mixture(n, seed=5)
+    tau = W.charges(n, seed=6)
+    m = ffmm.CubeKernelIndependent(ffmm.Laplace3D(), p=8)
+    cache = ffmm.OperatorCache(m)
+    print(f"[workloads] mixture n={n}, mixed vs FP64, chunked M2M", flush=True)
+    mixed = ffmm.DevicePlan(m, pts, ppl=64, bbox=BOX, cache=cache, precision="mixed", m2l="spectral")
+    groups = np.asarray(mixed.level_count) // 8
+    print(f"[workloads] sibling groups per level {groups.tolist()}", flush=True)
+    assert groups[3:].max() > 10240
+    mixed.build_expansions(tau)
+    u = mixed.evaluate_at_sources(ffmm.Potential).cpu().numpy()
+    del mixed
+    plain = ffmm.DevicePlan(m, pts, ppl=64, bbox=BOX, cache=cache, precision="f64", m2l="spectral")
+    plain.build_expansions(tau)
+    u64 = plain.evaluate_at_sources(ffmm.Potential).cpu().numpy()
+    err = rel(u, u64)
+    print(f"[workloads] mixed vs FP64 rel L2 {err:.2e}", flush=True)
+    assert err <= 1e-6
 
 
 @pytest.mark.parametrize("precision,tol_oracle", [("mixed", 1e-6), ("f64", 1e-11)])
