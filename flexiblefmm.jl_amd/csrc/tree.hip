@@ -351,7 +351,7 @@ __global__ void k_flag_s2m(const int32_t *__restrict__ cf, const uint8_t *__rest
 struct LevelStarts {
   int64_t lo[MAX_KEY_LEVELS + 2];
 };
-static_assert(2 + 2 * (MAX_KEY_LEVELS + 2) <= NPINNED, "pinned slots of the level boundaries");
+static_assert(2 + 2 * (MAX_KEY_LEVELS + 2) <= NPINNED - 1, "pinned slots of the level boundaries, below the polled slot");
 // per level l (and l = L): the position in each node list (ascending node ids; their
 // lengths in pinned slots 0 and 1) of the first node >= lo[l]; into pinned slots 2 + l
 // and 2 + (MAX_KEY_LEVELS + 2) + l
