@@ -1,7 +1,7 @@
 """Host-side logic that needs no GPU: the Stokes front end's density rows and error
-phrases (src/stokes_api.jl:72-83), and the seeded workload generators (the ellipsoid
+phrases (src/stokes_api.jl:72-83), the seeded workload generators (the ellipsoid
 surfaces of the Stokes workload: points on the surfaces, unit outward normals,
-area-uniform sampling, determinism)."""
+area-uniform sampling, determinism), and bench.py's record of the step's GEMMs."""
 import numpy as np
 import pytest
 
@@ -76,3 +76,33 @@ def test_area_uniform_sampling_on_a_flat_ellipsoid():
     ref = np.sum(dA[np.abs(v[:, 2]) > 0.9]) / np.sum(dA)
     assert abs(share - ref) < 0.01
     assert share > 0.15
+
+
+def test_bench_gemm_record_splits_the_wide_mixed_m2m():
+    """bench.py's record of the cube step's GEMMs, in launch order: the mixed form's M2M
+    of a level with more than 10,240 sibling groups is recorded as the library calls it
+    is made of (at most 10,240 columns each, covering the level's groups once, the same
+    flops in total); the FP64 form keeps one call per level."""
+    import types
+    import bench
+    Ns, Nl = 296, 1352
+    ops = types.SimpleNamespace(nsmall=Ns, nlarge=Nl, P_up=types.SimpleNamespace(rank=296),
+                                P_down=types.SimpleNamespace(rank=296))
+    counts = np.array([1, 8, 64, 512, 8 * 25_000, 800])
+    plan = types.SimpleNamespace(nlevels=len(counts), level_count=counts, level_s2m=np.full(len(counts), 1000),
+                                 level_leaves=np.full(len(counts), 900))
+    mixed, plain = bench.gemm_shapes(plan, ops, True), bench.gemm_shapes(plan, ops, False)
+    m3 = [g for g in mixed if g["name"].startswith("M2M level 3")]
+    assert [g["n"] for g in m3] == [10_240, 10_240, 4_520]
+    assert [g["name"] for g in m3] == ["M2M level 3 (columns 0-10239)", "M2M level 3 (columns 10240-20479)",
+                                       "M2M level 3 (columns 20480-24999)"]
+    assert all(g["library"] and g["type"] == "f32" and g["m"] == Ns and g["k"] == 8 * Ns for g in m3)
+    p3 = [g for g in plain if g["name"].startswith("M2M level 3")]
+    assert len(p3) == 1 and p3[0]["n"] == 25_000 and p3[0]["type"] == "f64"
+    assert sum(g["flop"] for g in m3) == p3[0]["flop"] == 2.0 * Ns * 25_000 * 8 * Ns
+    # launch order: M2M from level L-2 up to 2, then L2L from level 3 down
+    names = [g["name"].split(" (")[0] for g in mixed]
+    assert names == ["UC2E Uh", "UC2E V", "M2M level 4", "M2M level 3", "M2M level 3", "M2M level 3", "M2M level 2",
+                     "L2L level 3", "L2L level 4", "L2L level 5", "DC2E Uh", "DC2E V"]
+    # narrower levels are one call, named as before
+    assert [g["name"] for g in mixed if g["name"].startswith("M2M level 4")] == ["M2M level 4"]
